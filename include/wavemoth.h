@@ -1,0 +1,120 @@
+/*
+ * wavemoth.h -- public C ABI of libwavemoth_hip.so, the MI355X (gfx950)
+ * a_lm -> HEALPix-map synthesis engine.
+ *
+ * The first block keeps the reference's stable driver API unchanged, so a
+ * caller of the reference links against this library instead:
+ *   reference: src/wavemoth.h:20-46 (declarations), src/wavemoth.c (impl).
+ * Layouts are the reference's:
+ *   input   complex128 a_lm, m-major triangle, idx(l,m) = m(2 lmax - m + 3)/2
+ *           + (l - m), each entry holding nmaps interleaved complex values
+ *           (src/wavemoth.c:1016-1018, wavemoth/lib.pyx:119-135);
+ *   output  float64 map (npix, nmaps) C order, HEALPix RING order from the
+ *           north pole (src/wavemoth.c:1182-1187, 1199-1232).
+ * Ownership follows the reference: the plan stores the caller's host
+ * pointers (src/wavemoth.c:396-397); they must outlive the plan and are read
+ * / written at execute time.
+ *
+ * Differences, all deliberate and documented:
+ *   - errors never abort(): plan creation returns NULL and every other call
+ *     returns/records an error; wavemoth_last_error() says why;
+ *   - nthreads must be > 0 (as src/wavemoth.c:411-414) but is otherwise
+ *     unused: the work runs on the current HIP device;
+ *   - no resource file is needed: Lambda_l^m is generated on the GPU in FP64.
+ *     A resource file passed to wavemoth_plan_to_healpix is validated against
+ *     (Nside, lmax, mmax) through its header and otherwise unused.
+ */
+#ifndef WAVEMOTH_H_
+#define WAVEMOTH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct _wavemoth_plan *wavemoth_plan;
+
+/* src/wavemoth.h:22-27 */
+#define WAVEMOTH_MMAJOR 0x0
+#define WAVEMOTH_ESTIMATE 0x0
+#define WAVEMOTH_MEASURE 0x1
+#define WAVEMOTH_NO_RESOURCE_COPY 0x10
+/* Extension flag: generate the per-(m, ring) Lambda seeds inside every
+   execute instead of once at plan time. */
+#define WAVEMOTH_SEEDS_ON_THE_FLY 0x100
+
+/* ---- stable driver API (replaces src/wavemoth.h:33-46) ---------------- */
+
+/* replaces src/wavemoth.h:33 (impl src/wavemoth.c:129-135) */
+void wavemoth_configure(char *resource_dir);
+
+/* replaces src/wavemoth.h:35-39 (impl src/wavemoth.c:383-615) */
+wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
+                                       int nthreads, double *input, double *output,
+                                       int ordering, unsigned flags,
+                                       char *resource_filename);
+
+/* replaces src/wavemoth.h:41 (impl src/wavemoth.c:837-874); frees everything */
+void wavemoth_destroy_plan(wavemoth_plan plan);
+
+/* replaces src/wavemoth.h:42 (impl src/wavemoth.c:1271-1283): synchronous
+   host input -> device -> host output */
+void wavemoth_execute(wavemoth_plan plan);
+
+/* replaces src/wavemoth.h:44 (impl src/wavemoth.c:876-885): multiply-adds
+   counted as 2 flops for nvecs = 2 over the (l, ring) elements the plan
+   actually evaluates for (m, odd) (4 * element_count, as the reference) */
+int64_t wavemoth_get_legendre_flops(wavemoth_plan plan, int m, int odd);
+
+/* replaces src/wavemoth.h:46 (impl src/wavemoth.c:144-155) */
+int wavemoth_query_resourcefile(char *filename, int *out_Nside, int *out_lmax);
+
+/* ---- stage-wise hooks (replace src/wavemoth_private.h:132-138, used by
+        wavemoth/lib.pyx:55-57,80-81,151-152,187-193) --------------------- */
+
+/* replaces wavemoth_perform_legendre_transforms (src/wavemoth.c:933-941):
+   copies the host input to the device and runs the Legendre stage only */
+void wavemoth_perform_legendre_transforms(wavemoth_plan plan);
+/* replaces wavemoth_perform_backward_ffts (src/wavemoth.c:1193-1195): runs
+   the ring stage on the current work buffer and copies the map to host */
+void wavemoth_perform_backward_ffts(wavemoth_plan plan);
+/* replaces wavemoth_disable_phase_shifting (src/wavemoth.c:1240-1248) */
+void wavemoth_disable_phase_shifting(wavemoth_plan plan);
+
+/* ---- extensions (no counterpart in the reference) ---------------------- */
+
+/* Last error of the calling thread ("" if none). */
+const char *wavemoth_last_error(void);
+
+/* Copy the Legendre-stage output to host in the reference's get_work layout
+   (wavemoth/lib.pyx:158-179): complex128 [mmax+1][2 (odd)][2 Nside][nmaps],
+   rings ordered from the equator to the pole.  Returns 0 on success. */
+int wavemoth_get_work(wavemoth_plan plan, double *out);
+
+/* Asynchronous execute on device buffers: d_input (same layout as input)
+   and d_output (same layout as output) are device pointers, `stream` a
+   hipStream_t (NULL = default stream).  Returns 0 on success. */
+int wavemoth_execute_device(wavemoth_plan plan, const double *d_input, double *d_output,
+                            void *stream);
+
+/* Stage timing: with profiling on, every execute records HIP events on its
+   stream around (0) a_lm prescale (plus the seed kernels with
+   WAVEMOTH_SEEDS_ON_THE_FLY), (1) the Legendre sweep, (2) the ring stage.
+   wavemoth_stage_times waits for them and returns the average milliseconds
+   of each stage over the executes recorded since the last call, then
+   forgets them.  Both return 0 on success. */
+int wavemoth_set_profiling(wavemoth_plan plan, int on);
+int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
+
+/* Plan facts: 0 Nside, 1 lmax, 2 mmax, 3 nmaps, 4 npix, 5 dense triples T
+   (sum over m of (lmax-m+1) * 2 Nside), 6 trimmed triples T_eps evaluated by
+   the plan, 7 flags, 8 legendre-kernel launches per execute, 9 device bytes
+   held by the plan.  Returns -1 for an unknown key. */
+int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAVEMOTH_H_ */

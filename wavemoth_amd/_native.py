@@ -1,0 +1,74 @@
+"""ctypes binding of libwavemoth_hip.so (the C ABI of include/wavemoth.h).
+
+The library is built in-tree by wavemoth_amd.build; loading fails loudly
+when it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwavemoth_hip.so")
+_LIB = None
+
+# Every symbol include/wavemoth.h declares, with its ctypes signature.
+_c_int, _c_u, _c_i64 = ctypes.c_int, ctypes.c_uint, ctypes.c_int64
+_vp, _dp, _cp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+SIGNATURES = {
+    "wavemoth_configure": (None, [_cp]),
+    "wavemoth_plan_to_healpix": (_vp, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
+                                       _c_int, _c_u, _cp]),
+    "wavemoth_destroy_plan": (None, [_vp]),
+    "wavemoth_execute": (None, [_vp]),
+    "wavemoth_get_legendre_flops": (_c_i64, [_vp, _c_int, _c_int]),
+    "wavemoth_query_resourcefile": (_c_int, [_cp, ctypes.POINTER(_c_int),
+                                             ctypes.POINTER(_c_int)]),
+    "wavemoth_perform_legendre_transforms": (None, [_vp]),
+    "wavemoth_perform_backward_ffts": (None, [_vp]),
+    "wavemoth_disable_phase_shifting": (None, [_vp]),
+    "wavemoth_last_error": (_cp, []),
+    "wavemoth_get_work": (_c_int, [_vp, _vp]),
+    "wavemoth_execute_device": (_c_int, [_vp, _vp, _vp, _vp]),
+    "wavemoth_set_profiling": (_c_int, [_vp, _c_int]),
+    "wavemoth_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int)]),
+    "wavemoth_plan_info": (_c_i64, [_vp, _c_int]),
+}
+
+WAVEMOTH_MMAJOR = 0x0
+WAVEMOTH_ESTIMATE = 0x0
+WAVEMOTH_MEASURE = 0x1
+WAVEMOTH_NO_RESOURCE_COPY = 0x10
+WAVEMOTH_SEEDS_ON_THE_FLY = 0x100
+
+
+def _bind_torch_runtime():
+    """PyTorch-ROCm ships its own libamdhip64 / libhsa-runtime64 (same
+    sonames as /opt/rocm's).  Two HIP runtimes cannot share a process, so
+    torch is loaded first when it is installed: the dynamic loader then
+    resolves this library's runtime dependencies to torch's copies."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        _bind_torch_runtime()
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libwavemoth_hip.so is not built (run `python -m wavemoth_amd.build`); "
+                "wavemoth_amd has no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    e = lib().wavemoth_last_error()
+    return e.decode() if e else ""

@@ -1,0 +1,319 @@
+// Legendre stage of the a_lm -> map synthesis on gfx950, in FP64.
+//
+// For every (m, ring pair r) the engine needs the even and odd partial sums
+//   E[r][m] = sum_{l-m even} Lambda_l^m(x_r) a_lm,
+//   O[r][m] = sum_{l-m odd}  Lambda_l^m(x_r) a_lm,
+// which the reference computes per (m, parity) with the x^2 two-step
+// recurrence from stored seeds (src/legendre_transform.c.in:680-747,
+// src/wavemoth.c:958-1028) and keeps in work_q[m][odd][ring].
+//
+// Here Lambda is generated on the fly with the one-step recurrence written
+// in monic form.  With Lambda_{l+1} = a_l x Lambda_l - b_l Lambda_{l-1}
+// (the normalised recurrence of libpshtlight/ylmgen_c.c:604-619, 643-732)
+// and Lambda_l = A_l L~_l, A_{l+1} = a_l A_l, the scaled values obey
+//   L~_{l+1} = x L~_l - c~_l L~_{l-1},   c~_l = (l^2 - m^2) / (4 l^2 - 1),
+// i.e. one multiply and one FMA per (l, ring) instead of three operations.
+// A_l is reset to 1 at the start of every kChunk-long l-chunk; the state
+// is multiplied by the chunk's end product once per chunk and the
+// contraction uses a~_lm = A_l a_lm, prescaled once per execute.
+// Both parities come out of the same sweep (odd and even l alternate).
+//
+// Seeds: Lambda_mm = (-1)^m sqrt(prod (2i+1)/(2i)) / sqrt(4 pi) sin^m theta
+// underflows FP64 for large m near the poles, so the seed kernel carries a
+// separate binary exponent, runs the recurrence through the negligible
+// region and records, per (m, ring), the chunk-aligned l where |Lambda|
+// first reaches 2^-107 together with the two recurrence states there.  This
+// is the GPU form of Ylmgen's firstl and the reference's stripify start
+// rows (wavemoth/lib.pyx:301-368, 397-487); terms skipped before that
+// point are below 1e-32 in magnitude.
+#include "wm_internal.h"
+
+namespace wm {
+
+// -------------------------------------------------------------- seeds
+
+// One thread per (m, ring).  Writes seed_l / seed_v.
+__global__ __launch_bounds__(256) void k_seeds(LegTables t) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int R = t.R, L = t.L;
+  if (gid >= (int64_t)(t.M + 1) * R) return;
+  const int m = (int)(gid / R), r = (int)(gid - (int64_t)m * R);
+  const double x = t.ring_x[r];
+
+  // log2 |Lambda_mm| = mfac[m] + m log2 sin(theta), in double-double so
+  // the seed is good to ~1 ulp even when the exponent is ~ -10^4.
+  const double2 mf = t.mfac[m];
+  const double2 ls = t.ring_ls[r];
+  const double dm = (double)m;
+  double p = dm * ls.x;
+  double pe = fma(dm, ls.x, -p) + dm * ls.y;
+  double hi = mf.x + p;
+  double bb = hi - mf.x;
+  double lo = (mf.x - (hi - bb)) + (p - bb) + mf.y + pe;
+  double E = floor(hi);
+  double frac = (hi - E) + lo;
+  if (frac >= 1.0) { E += 1.0; frac -= 1.0; }
+  if (frac < 0.0) { E -= 1.0; frac += 1.0; }
+  int iE = (int)E;
+  double cur = exp2(frac);
+  if (m & 1) cur = -cur;
+  double prev = 0.0;
+
+  const int64_t offm = t.off[m];
+  const double *ct = t.ctil + (offm - m);
+  const double *cs = t.cscale + offm / kChunk;
+  int found = kSeedNone;
+  double sp = 0.0, sc = 0.0;
+  int c = 0;
+  for (int l0 = m; l0 <= L; l0 += kChunk, ++c) {
+    const double cp = prev, cc = cur;
+    const int cE = iE;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const double nx = fma(x, cur, -ct[l0 + j] * prev);
+      prev = cur;
+      cur = nx;
+    }
+    const double S = cs[c];
+    prev *= S;
+    cur *= S;
+    const double big = fmax(fabs(prev), fabs(cur));
+    if (!(big > 0.0)) break;
+    const int e = ilogb(big);
+    if (e + iE >= -107) {
+      found = l0;
+      sp = ldexp(cp, cE);
+      sc = ldexp(cc, cE);
+      break;
+    }
+    prev = ldexp(prev, -e);
+    cur = ldexp(cur, -e);
+    iE += e;
+  }
+  t.seed_l[gid] = found;
+  t.seed_v[gid] = make_double2(sp, sc);
+}
+
+// One thread per (m, wave): the wave-uniform first l and the last l at
+// which some lane of the wave still has to pick up its seed.
+__global__ __launch_bounds__(256) void k_wave_bounds(LegTables t) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)(t.M + 1) * t.W) return;
+  const int m = (int)(gid / t.W), w = (int)(gid - (int64_t)m * t.W);
+  int lo = kSeedNone, hi = -1;
+  const int rings = 64 * t.RL;
+  for (int i = 0; i < rings; ++i) {
+    const int r = w * rings + i;
+    if (r >= t.R) break;
+    const int s = t.seed_l[(int64_t)m * t.R + r];
+    if (s < lo) lo = s;
+    if (s <= t.L && s > hi) hi = s;
+  }
+  t.wstart[gid] = lo;
+  t.wseedmax[gid] = hi;
+}
+
+// ------------------------------------------------------- a_lm prescale
+
+// atil[kb][j][k'] = A_l * a_{l m}[kb*KB + k'] on the padded l-grid, zero
+// past lmax and for map slots past nmaps: map-block-major so that a wave
+// of map block kb reads KB consecutive complex values per l.
+__global__ __launch_bounds__(256) void k_prep_alm(LegTables t, const double2 *__restrict__ alm,
+                                                  int KB, int nkb) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= t.nlpad) return;
+  const int m = t.chunk_m[j / kChunk];
+  const int64_t jj = j - t.off[m];
+  const int l = m + (int)jj;
+  const int K = t.K, L = t.L;
+  const bool live = l <= L;
+  const double s = live ? t.ascale[j] : 0.0;
+  const double2 *src = alm + (live ? ((int64_t)m * (2 * L - m + 3) / 2 + jj) * K : 0);
+  for (int kb = 0; kb < nkb; ++kb) {
+    double2 *dst = t.atil + ((int64_t)kb * t.nlpad + j) * KB;
+    for (int k = 0; k < KB; ++k) {
+      const int kk = kb * KB + k;
+      double2 v = make_double2(0.0, 0.0);
+      if (live && kk < K) {
+        v = src[kk];
+        v.x *= s;
+        v.y *= s;
+      }
+      dst[k] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------ main sweep
+
+// Block = 4 waves.  Block b runs on XCD b % 8; every m's blocks are kept
+// on one XCD (m = xcd + 8 * i) so the per-m coefficient rows are fetched
+// into one L2 only, and each XCD walks its m's from cheap-to-start (small
+// m, long l-range) to short ones.
+// Wave (m, ring chunk w, map block kb): lane handles rings
+// w*64*RL + lane + 64*i, i < RL; KB maps per pass.
+template <int RL, int KB>
+__global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x;
+  const int s = b >> 3;
+  const int m = (b & 7) + 8 * (s / bpm);
+  if (m > t.M) return;
+  const int item = (s % bpm) * 4 + wv;
+  if (item >= t.W * nkb) return;
+  const int kb = item / t.W, w = item - kb * t.W;
+  const int L = t.L, R = t.R, M = t.M, K = t.K;
+  const int kbase = kb * KB;
+
+  int l0 = __builtin_amdgcn_readfirstlane(t.wstart[(int64_t)m * t.W + w]);
+  const int lsmax = __builtin_amdgcn_readfirstlane(t.wseedmax[(int64_t)m * t.W + w]);
+  const int64_t offm = t.off[m];
+  const double *__restrict__ ct = t.ctil + (offm - m);
+  const double2 *__restrict__ at = t.atil + ((int64_t)kb * t.nlpad + offm - m) * KB;
+  const double *__restrict__ cs = t.cscale + offm / kChunk;
+
+  double x[RL], prev[RL], cur[RL];
+  int sl[RL], rr[RL];
+  double acc[RL][2][KB][2];
+#pragma unroll
+  for (int i = 0; i < RL; ++i) {
+    rr[i] = w * 64 * RL + lane + 64 * i;
+    const bool ok = rr[i] < R;
+    x[i] = ok ? t.ring_x[rr[i]] : 0.0;
+    sl[i] = ok ? t.seed_l[(int64_t)m * R + rr[i]] : kSeedNone;
+    prev[i] = 0.0;
+    cur[i] = 0.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int k = 0; k < KB; ++k) acc[i][p][k][0] = acc[i][p][k][1] = 0.0;
+  }
+
+  for (; l0 <= L; l0 += kChunk) {
+    if (l0 <= lsmax) {
+#pragma unroll
+      for (int i = 0; i < RL; ++i) {
+        if (sl[i] == l0) {
+          const double2 v = t.seed_v[(int64_t)m * R + rr[i]];
+          prev[i] = v.x;
+          cur[i] = v.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; j += 2) {
+      const int l = l0 + j;
+      const double c0 = ct[l], c1 = ct[l + 1];
+      double2 a0[KB], a1[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        a0[k] = at[(int64_t)l * KB + k];
+        a1[k] = at[(int64_t)(l + 1) * KB + k];
+      }
+#pragma unroll
+      for (int i = 0; i < RL; ++i) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          acc[i][0][k][0] = fma(cur[i], a0[k].x, acc[i][0][k][0]);
+          acc[i][0][k][1] = fma(cur[i], a0[k].y, acc[i][0][k][1]);
+        }
+        const double n1 = fma(x[i], cur[i], -(c0 * prev[i]));
+        prev[i] = cur[i];
+        cur[i] = n1;
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          acc[i][1][k][0] = fma(cur[i], a1[k].x, acc[i][1][k][0]);
+          acc[i][1][k][1] = fma(cur[i], a1[k].y, acc[i][1][k][1]);
+        }
+        const double n2 = fma(x[i], cur[i], -(c1 * prev[i]));
+        prev[i] = cur[i];
+        cur[i] = n2;
+      }
+    }
+    const double S = cs[(l0 - m) / kChunk];
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      prev[i] *= S;
+      cur[i] *= S;
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < RL; ++i) {
+    if (rr[i] < R) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        double2 *dst = t.qbuf + (((int64_t)rr[i] * 2 + p) * (M + 1) + m) * K + kbase;
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+          if (kbase + k < K) dst[k] = make_double2(acc[i][p][k][0], acc[i][p][k][1]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_seeds(const LegTables &t, hipStream_t s) {
+  const int64_t n = (int64_t)(t.M + 1) * t.R;
+  hipLaunchKernelGGL(k_seeds, dim3(nblocks(n, 256)), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_wave_bounds(const LegTables &t, hipStream_t s) {
+  const int64_t n = (int64_t)(t.M + 1) * t.W;
+  hipLaunchKernelGGL(k_wave_bounds, dim3(nblocks(n, 256)), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+// Map-block size and rings per lane chosen from nmaps.
+struct LegShape {
+  int RL, KB;
+};
+static LegShape leg_shape(int K) {
+  if (K == 1) return {2, 1};
+  if (K == 2) return {2, 2};
+  if (K == 3) return {1, 3};
+  if (K == 4) return {1, 4};
+  return {1, 8};
+}
+
+int legendre_rings_per_lane(int K) { return leg_shape(K).RL; }
+int legendre_padded_maps(int K) {
+  const LegShape sh = leg_shape(K);
+  return ((K + sh.KB - 1) / sh.KB) * sh.KB;
+}
+
+hipError_t launch_prep_alm(const LegTables &t, const double2 *alm, hipStream_t s) {
+  const int KB = leg_shape(t.K).KB;
+  hipLaunchKernelGGL(k_prep_alm, dim3(nblocks(t.nlpad, 256)), dim3(256), 0, s, t, alm, KB,
+                     (t.K + KB - 1) / KB);
+  return hipGetLastError();
+}
+
+template <int RL, int KB>
+static hipError_t launch_leg_t(const LegTables &t, hipStream_t s) {
+  const int nkb = (t.K + KB - 1) / KB;
+  const int bpm = (t.W * nkb + 3) / 4;
+  const int mgroups = (t.M + 1 + 7) / 8;
+  const unsigned grid = (unsigned)(8 * mgroups * bpm);
+  hipLaunchKernelGGL((k_legendre<RL, KB>), dim3(grid), dim3(256), 0, s, t, nkb, bpm);
+  return hipGetLastError();
+}
+
+hipError_t launch_legendre(const LegTables &t, hipStream_t s) {
+  const LegShape sh = leg_shape(t.K);
+  if (sh.RL != t.RL) return hipErrorInvalidValue;
+  switch (sh.KB) {
+    case 1: return launch_leg_t<2, 1>(t, s);
+    case 2: return launch_leg_t<2, 2>(t, s);
+    case 3: return launch_leg_t<1, 3>(t, s);
+    case 4: return launch_leg_t<1, 4>(t, s);
+    default: return launch_leg_t<1, 8>(t, s);
+  }
+}
+
+}  // namespace wm

@@ -1,0 +1,649 @@
+// Host side of libwavemoth_hip.so: the C ABI of include/wavemoth.h.
+//
+// A plan owns every device buffer and table of one (Nside, lmax, mmax,
+// nmaps) synthesis on the HIP device that was current when it was made.
+// Plan creation corresponds to the reference's wavemoth_plan_to_healpix
+// (src/wavemoth.c:383-615) plus its offline resource precompute
+// (wavemoth/lib.pyx:376-592): recurrence coefficient tables, per-(m, ring)
+// seeds (computed on the GPU in FP64), ring geometry and the Bluestein
+// tables of the ring stage.  Execute runs three kernels on one stream:
+// a_lm prescale -> Legendre sweep -> ring synthesis.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wavemoth.h"
+#include "wm_internal.h"
+
+using namespace wm;
+
+static thread_local std::string g_err;
+static std::string g_resource_dir;
+
+static void set_err(const std::string &s) { g_err = s; }
+
+#define WM_TRY(expr, what)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      set_err(std::string(what) + ": " + hipGetErrorString(e_));                  \
+      return false;                                                               \
+    }                                                                             \
+  } while (0)
+
+struct _wavemoth_plan {
+  int N = 0, L = 0, M = 0, K = 0, nthreads = 0;
+  unsigned flags = 0;
+  double *h_in = nullptr, *h_out = nullptr;
+  int64_t nalm = 0, npix = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  LegTables lt;
+  RingTables rt;
+  double2 *d_alm = nullptr;
+  double *d_map = nullptr;
+  bool phase_shifts = true;
+  bool fly = false;
+  int64_t T_dense = 0, T_eps = 0;
+  std::vector<int64_t> elems;  // per (m, odd): (l, ring) elements evaluated
+  std::vector<void *> allocs;
+  size_t dev_bytes = 0;
+  // stage timing (extension): events recorded around prep / sweep / rings
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_free;
+  std::vector<std::array<hipEvent_t, 4>> ev_used;
+};
+
+template <typename T>
+static bool dalloc(wavemoth_plan p, T **ptr, size_t count, const char *what) {
+  void *v = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&v, bytes);
+  if (e != hipSuccess) {
+    set_err(std::string("hipMalloc ") + what + " (" + std::to_string(bytes) +
+            " bytes): " + hipGetErrorString(e));
+    return false;
+  }
+  p->allocs.push_back(v);
+  p->dev_bytes += bytes;
+  *ptr = (T *)v;
+  return true;
+}
+
+template <typename T>
+static bool upload(wavemoth_plan p, T **ptr, const std::vector<T> &h, const char *what) {
+  if (!dalloc(p, ptr, h.size(), what)) return false;
+  if (!h.empty())
+    WM_TRY(hipMemcpy(*ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), what);
+  return true;
+}
+
+static void release_events(wavemoth_plan p) {
+  for (auto &a : p->ev_used)
+    for (hipEvent_t e : a) p->ev_free.push_back(e);
+  p->ev_used.clear();
+}
+
+static void free_plan(wavemoth_plan p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  release_events(p);
+  for (hipEvent_t e : p->ev_free) (void)hipEventDestroy(e);
+  for (void *v : p->allocs) (void)hipFree(v);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+// double-double from long double
+static double2 dd(long double v) {
+  double hi = (double)v;
+  double lo = (double)(v - (long double)hi);
+  return make_double2(hi, lo);
+}
+
+// z of the 2N northern rings, equator -> pole (wavemoth/healpix.py:5-18).
+static double ring_z(int N, int r) {
+  const int i = 2 * N - r;
+  if (i <= N) return 1.0 - (double)i * (double)i / (3.0 * (double)N * (double)N);
+  return 4.0 / 3.0 - 2.0 * (double)i / (3.0 * (double)N);
+}
+
+static bool build_legendre_tables(wavemoth_plan p) {
+  LegTables &t = p->lt;
+  const int N = p->N, L = p->L, M = p->M, K = p->K, R = 2 * N;
+  t.N = N;
+  t.L = L;
+  t.M = M;
+  t.K = K;
+  t.R = R;
+  t.RL = legendre_rings_per_lane(K);
+  t.W = (R + 64 * t.RL - 1) / (64 * t.RL);
+
+  std::vector<int64_t> off(M + 1);
+  int64_t tot = 0;
+  for (int m = 0; m <= M; ++m) {
+    off[m] = tot;
+    const int64_t len = L - m + 1;
+    tot += (len + kChunk - 1) / kChunk * kChunk;
+  }
+  t.nlpad = tot;
+  std::vector<int> chunk_m(tot / kChunk);
+  std::vector<double> ctil(tot), ascale(tot), cscale(tot / kChunk);
+  for (int m = 0; m <= M; ++m) {
+    const int64_t len = L - m + 1;
+    const int64_t lp = (len + kChunk - 1) / kChunk * kChunk;
+    const double dm = (double)m;
+    for (int64_t c = 0; c < lp / kChunk; ++c) {
+      chunk_m[off[m] / kChunk + c] = m;
+      double A = 1.0;
+      for (int j = 0; j < kChunk; ++j) {
+        const int64_t jj = c * kChunk + j;
+        const double l = dm + (double)jj;
+        ctil[off[m] + jj] = (l * l - dm * dm) / (4.0 * l * l - 1.0);
+        ascale[off[m] + jj] = (jj < len) ? A : 0.0;
+        const double l1 = l + 1.0;
+        A *= std::sqrt((4.0 * l1 * l1 - 1.0) / (l1 * l1 - dm * dm));
+      }
+      cscale[off[m] / kChunk + c] = A;
+    }
+  }
+  // log2 of sqrt(prod_{i<=m} (2i+1)/(2i)) / sqrt(4 pi)
+  std::vector<double2> mfac(M + 1);
+  {
+    long double acc = -0.5L * log2l(4.0L * 3.141592653589793238462643383279502884L);
+    for (int m = 0; m <= M; ++m) {
+      if (m > 0) acc += 0.5L * log2l((2.0L * m + 1.0L) / (2.0L * m));
+      mfac[m] = dd(acc);
+    }
+  }
+  // Ring cos/sin as the reference sees them: theta = arccos(z) in double
+  // (wavemoth/lib.pyx:382-383), x = cos(theta), sin(theta).
+  std::vector<double> rx(R);
+  std::vector<double2> rls(R);
+  for (int r = 0; r < R; ++r) {
+    const double th = std::acos(ring_z(N, r));
+    rx[r] = std::cos(th);
+    rls[r] = dd(log2l((long double)std::sin(th)));
+  }
+
+  if (!upload(p, &t.off, off, "off")) return false;
+  if (!upload(p, &t.chunk_m, chunk_m, "chunk_m")) return false;
+  if (!upload(p, &t.ctil, ctil, "ctil")) return false;
+  if (!upload(p, &t.ascale, ascale, "ascale")) return false;
+  if (!upload(p, &t.cscale, cscale, "cscale")) return false;
+  if (!upload(p, &t.mfac, mfac, "mfac")) return false;
+  if (!upload(p, &t.ring_x, rx, "ring_x")) return false;
+  if (!upload(p, &t.ring_ls, rls, "ring_ls")) return false;
+  const size_t nseed = (size_t)(M + 1) * R;
+  if (!dalloc(p, &t.seed_l, nseed, "seed_l")) return false;
+  if (!dalloc(p, &t.seed_v, nseed, "seed_v")) return false;
+  if (!dalloc(p, &t.wstart, (size_t)(M + 1) * t.W, "wstart")) return false;
+  if (!dalloc(p, &t.wseedmax, (size_t)(M + 1) * t.W, "wseedmax")) return false;
+  if (!dalloc(p, &t.atil, (size_t)t.nlpad * legendre_padded_maps(K), "atil")) return false;
+  if (!dalloc(p, &t.qbuf, (size_t)R * 2 * (M + 1) * K, "qbuf")) return false;
+
+  WM_TRY(launch_seeds(t, p->stream), "seed kernel");
+  WM_TRY(launch_wave_bounds(t, p->stream), "wave-bounds kernel");
+  WM_TRY(hipStreamSynchronize(p->stream), "seed stage");
+
+  // Work accounting from the seeds: T_eps and per-(m, odd) element counts.
+  std::vector<int> sl(nseed);
+  WM_TRY(hipMemcpy(sl.data(), t.seed_l, nseed * sizeof(int), hipMemcpyDeviceToHost),
+         "seed_l download");
+  p->elems.assign((size_t)(M + 1) * 2, 0);
+  p->T_dense = 0;
+  p->T_eps = 0;
+  for (int m = 0; m <= M; ++m) {
+    p->T_dense += (int64_t)(L - m + 1) * R;
+    for (int r = 0; r < R; ++r) {
+      const int s = sl[(size_t)m * R + r];
+      if (s > L) continue;
+      const int64_t n = L - s + 1;
+      p->T_eps += n;
+      // l in [s, L] with l - m even / odd
+      const int64_t ne = (L - m) / 2 - (s - m + 1) / 2 + 1;
+      p->elems[2 * m] += std::max<int64_t>(ne, 0);
+      p->elems[2 * m + 1] += n - std::max<int64_t>(ne, 0);
+    }
+  }
+  return true;
+}
+
+static int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+static bool build_ring_tables(wavemoth_plan p) {
+  RingTables &rt = p->rt;
+  const int N = p->N;
+  const int nrings = 4 * N - 1, mid = 2 * N - 1;
+  rt.nrings = nrings;
+  std::vector<RingInfo> info(nrings);
+  std::vector<double> cost(nrings);
+  // distinct Bluestein lengths
+  std::vector<int> hlist, plist;
+  std::vector<int64_t> choff, vhoff;
+  std::vector<int> h_index(2 * N + 1, -1);
+  int64_t chtot = 0, vhtot = 0;
+  int max_lds = 0;
+  int64_t pix = 0;
+  // Ring length of iring, and phi0 = pi * num / den of the grid entry iring
+  // (src/wavemoth.c:1199-1232).
+  auto geom = [&](int iring, int &n, int &num, int &den) {
+    if (iring <= N - 1) {
+      n = 4 * (iring + 1);
+      num = 1;
+      den = 4 * (iring + 1);
+    } else if (iring > 3 * N - 1) {
+      n = 4 * (nrings - iring);
+      num = 1;
+      den = 4 * (nrings - iring);
+    } else {
+      n = 4 * N;
+      num = iring % 2;
+      den = 4 * N;
+    }
+  };
+  for (int iring = 0; iring < nrings; ++iring) {
+    int n, num, den, n_s;
+    geom(iring, n, num, den);
+    // Both rings of a pair are phase-shifted with the phi0 of the southern
+    // ring mid + r, as src/wavemoth.c:542 (ri->phi0 = phi0s[mid_ring + ...])
+    // does; it differs from the ring's own phi0 only for Nside = 1.
+    const int rpair = (iring <= mid) ? (mid - iring) : (iring - mid);
+    geom(mid + rpair, n_s, num, den);
+    RingInfo &ri = info[iring];
+    ri.pix_off = pix;
+    pix += n;
+    ri.n = n;
+    ri.r = (iring <= mid) ? (mid - iring) : (iring - mid);
+    ri.sgn = (iring <= mid) ? 1 : -1;
+    ri.ph_num = p->phase_shifts ? num : 0;
+    ri.ph_den = den;
+    const int h = n / 2;
+    if (is_pow2(h)) {
+      ri.P = 0;
+      ri.chirp_off = ri.vhat_off = 0;
+      max_lds = std::max(max_lds, h + 1);
+      cost[iring] = (double)h * std::log2((double)h + 1.0) + (double)(p->M + 1);
+    } else {
+      const int P = next_pow2(2 * h - 1);
+      if (h_index[h] < 0) {
+        h_index[h] = (int)hlist.size();
+        hlist.push_back(h);
+        plist.push_back(P);
+        choff.push_back(chtot);
+        vhoff.push_back(vhtot);
+        chtot += h;
+        vhtot += P;
+      }
+      const int hi = h_index[h];
+      ri.P = P;
+      ri.chirp_off = choff[hi];
+      ri.vhat_off = vhoff[hi];
+      max_lds = std::max(max_lds, P);
+      cost[iring] = 2.0 * (double)P * std::log2((double)P) + (double)(p->M + 1);
+    }
+  }
+  p->npix = pix;
+  if (max_lds > kMaxFFT + 1) {
+    set_err("ring stage: Nside too large for the LDS-resident ring FFT (Nside <= 2048)");
+    return false;
+  }
+  rt.max_lds_complex = max_lds;
+  std::vector<int> order(nrings);
+  for (int i = 0; i < nrings; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<double2> tw(kMaxFFT);
+  for (int j = 0; j < kMaxFFT; ++j) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * j / kMaxFFT;
+    tw[j] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  if (!upload(p, &rt.info, info, "ring info")) return false;
+  if (!upload(p, &rt.order, order, "ring order")) return false;
+  if (!upload(p, &rt.twiddle, tw, "twiddle")) return false;
+  if (!dalloc(p, &rt.chirp, (size_t)chtot, "chirp")) return false;
+  if (!dalloc(p, &rt.vhat, (size_t)vhtot, "vhat")) return false;
+  if (!hlist.empty()) {
+    int *d_h = nullptr, *d_p = nullptr;
+    int64_t *d_co = nullptr, *d_vo = nullptr;
+    if (!upload(p, &d_h, hlist, "hlist")) return false;
+    if (!upload(p, &d_p, plist, "plist")) return false;
+    if (!upload(p, &d_co, choff, "choff")) return false;
+    if (!upload(p, &d_vo, vhoff, "vhoff")) return false;
+    WM_TRY(launch_bluestein_prep(rt, d_h, d_co, d_vo, d_p, (int)hlist.size(), p->stream),
+           "bluestein prep");
+    WM_TRY(hipStreamSynchronize(p->stream), "bluestein prep");
+  }
+  return true;
+}
+
+// Rewrites only the phase fields of the ring table (phase on/off).
+static bool refresh_phases(wavemoth_plan p) {
+  std::vector<RingInfo> info(p->rt.nrings);
+  WM_TRY(hipMemcpy(info.data(), p->rt.info, info.size() * sizeof(RingInfo),
+                   hipMemcpyDeviceToHost),
+         "ring info download");
+  for (auto &ri : info) ri.ph_num = 0;
+  WM_TRY(hipMemcpy(p->rt.info, info.data(), info.size() * sizeof(RingInfo),
+                   hipMemcpyHostToDevice),
+         "ring info upload");
+  return true;
+}
+
+static hipEvent_t take_event(wavemoth_plan p) {
+  if (!p->ev_free.empty()) {
+    hipEvent_t e = p->ev_free.back();
+    p->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Event slot i (0..3) of the current execute, when profiling.
+static bool mark(wavemoth_plan p, int i, hipStream_t s) {
+  if (!p->profiling) return true;
+  if (i == 0) p->ev_used.push_back({nullptr, nullptr, nullptr, nullptr});
+  hipEvent_t e = take_event(p);
+  if (!e) {
+    set_err("hipEventCreate failed");
+    return false;
+  }
+  p->ev_used.back()[i] = e;
+  WM_TRY(hipEventRecord(e, s), "hipEventRecord");
+  return true;
+}
+
+static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
+  if (!mark(p, 0, s)) return false;
+  if (p->fly) {
+    WM_TRY(launch_seeds(p->lt, s), "seed kernel");
+    WM_TRY(launch_wave_bounds(p->lt, s), "wave-bounds kernel");
+  }
+  WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");
+  if (!mark(p, 1, s)) return false;
+  WM_TRY(launch_legendre(p->lt, s), "legendre kernel");
+  if (!mark(p, 2, s)) return false;
+  return true;
+}
+
+static bool run_rings(wavemoth_plan p, double *d_map, hipStream_t s) {
+  WM_TRY(launch_ring_synth(p->lt, p->rt, d_map, s), "ring kernel");
+  if (p->profiling && !p->ev_used.empty() && p->ev_used.back()[2] != nullptr)
+    if (!mark(p, 3, s)) return false;
+  return true;
+}
+
+extern "C" {
+
+void wavemoth_configure(char *resource_dir) {
+  g_resource_dir = resource_dir ? resource_dir : "";
+}
+
+int wavemoth_query_resourcefile(char *filename, int *out_Nside, int *out_lmax) {
+  if (!filename) return -1;
+  FILE *fd = std::fopen(filename, "rb");
+  if (!fd) return -1;
+  int64_t fields[3];
+  const size_t got = std::fread(fields, sizeof(fields), 1, fd);
+  std::fclose(fd);
+  if (got != 1) {
+    set_err(std::string("could not read resource header: ") + filename);
+    return -1;
+  }
+  *out_lmax = (int)fields[0];
+  *out_Nside = (int)fields[2];
+  return 0;
+}
+
+wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps, int nthreads,
+                                       double *input, double *output, int ordering,
+                                       unsigned flags, char *resource_filename) {
+  g_err.clear();
+  if (nthreads <= 0) {
+    set_err("Require nthreads > 0");
+    return nullptr;
+  }
+  if (ordering != WAVEMOTH_MMAJOR) {
+    set_err("only WAVEMOTH_MMAJOR ordering is supported");
+    return nullptr;
+  }
+  if (Nside < 1 || Nside > 2048) {
+    set_err("Nside must be in [1, 2048]");
+    return nullptr;
+  }
+  if (lmax < 0 || mmax < 0 || mmax > lmax) {
+    set_err("require 0 <= mmax <= lmax");
+    return nullptr;
+  }
+  if (nmaps < 1) {
+    set_err("nmaps must be >= 1");
+    return nullptr;
+  }
+  if ((int64_t)lmax + kChunk > (1 << 28)) {
+    set_err("lmax too large");
+    return nullptr;
+  }
+  if (resource_filename) {
+    FILE *fd = std::fopen(resource_filename, "rb");
+    if (!fd) {
+      set_err(std::string("Error in loading resource ") + resource_filename);
+      return nullptr;
+    }
+    int64_t hdr[3];
+    const size_t got = std::fread(hdr, sizeof(hdr), 1, fd);
+    std::fclose(fd);
+    if (got != 1) {
+      set_err(std::string("could not read resource header: ") + resource_filename);
+      return nullptr;
+    }
+    if (hdr[2] != Nside) {
+      set_err("Incompatible Nside");
+      return nullptr;
+    }
+    if (hdr[1] != mmax) {
+      set_err("Incompatible mmax");
+      return nullptr;
+    }
+    if (hdr[0] != lmax) {
+      set_err("Incompatible lmax");
+      return nullptr;
+    }
+  }
+  wavemoth_plan p = new _wavemoth_plan();
+  p->N = Nside;
+  p->L = lmax;
+  p->M = mmax;
+  p->K = nmaps;
+  p->nthreads = nthreads;
+  p->flags = flags;
+  p->h_in = input;
+  p->h_out = output;
+  p->fly = (flags & WAVEMOTH_SEEDS_ON_THE_FLY) != 0;
+  p->nalm = 0;
+  for (int m = 0; m <= mmax; ++m) p->nalm += lmax - m + 1;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    set_err("no HIP device");
+    delete p;
+    return nullptr;
+  }
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_err("hipStreamCreate failed");
+    delete p;
+    return nullptr;
+  }
+  if (!build_legendre_tables(p) || !build_ring_tables(p) ||
+      !dalloc(p, &p->d_alm, (size_t)p->nalm * nmaps, "alm") ||
+      !dalloc(p, &p->d_map, (size_t)p->npix * nmaps, "map")) {
+    std::string e = g_err;
+    free_plan(p);
+    g_err = e;
+    return nullptr;
+  }
+  return p;
+}
+
+void wavemoth_destroy_plan(wavemoth_plan plan) { free_plan(plan); }
+
+static bool execute_host(wavemoth_plan p, bool leg, bool rings) {
+  if (!p) {
+    set_err("NULL plan");
+    return false;
+  }
+  hipStream_t s = p->stream;
+  if (leg) {
+    if (!p->h_in) {
+      set_err("plan has no host input");
+      return false;
+    }
+    WM_TRY(hipMemcpyAsync(p->d_alm, p->h_in, (size_t)p->nalm * p->K * sizeof(double2),
+                          hipMemcpyHostToDevice, s),
+           "input upload");
+    if (!run_legendre(p, p->d_alm, s)) return false;
+  }
+  if (rings) {
+    if (!p->h_out) {
+      set_err("plan has no host output");
+      return false;
+    }
+    if (!run_rings(p, p->d_map, s)) return false;
+    WM_TRY(hipMemcpyAsync(p->h_out, p->d_map, (size_t)p->npix * p->K * sizeof(double),
+                          hipMemcpyDeviceToHost, s),
+           "output download");
+  }
+  WM_TRY(hipStreamSynchronize(s), "execute");
+  return true;
+}
+
+void wavemoth_execute(wavemoth_plan plan) {
+  g_err.clear();
+  (void)execute_host(plan, true, true);
+}
+
+void wavemoth_perform_legendre_transforms(wavemoth_plan plan) {
+  g_err.clear();
+  (void)execute_host(plan, true, false);
+}
+
+void wavemoth_perform_backward_ffts(wavemoth_plan plan) {
+  g_err.clear();
+  (void)execute_host(plan, false, true);
+}
+
+void wavemoth_disable_phase_shifting(wavemoth_plan plan) {
+  g_err.clear();
+  if (!plan) return;
+  plan->phase_shifts = false;
+  (void)refresh_phases(plan);
+}
+
+int64_t wavemoth_get_legendre_flops(wavemoth_plan plan, int m, int odd) {
+  if (!plan || m < 0 || m > plan->M || odd < 0 || odd > 1) return -1;
+  return 4 * plan->elems[(size_t)2 * m + odd];
+}
+
+const char *wavemoth_last_error(void) { return g_err.c_str(); }
+
+int wavemoth_get_work(wavemoth_plan p, double *out) {
+  g_err.clear();
+  if (!p || !out) {
+    set_err("NULL argument");
+    return -1;
+  }
+  const int R = p->lt.R, M = p->M, K = p->K;
+  std::vector<double2> q((size_t)R * 2 * (M + 1) * K);
+  if (hipMemcpy(q.data(), p->lt.qbuf, q.size() * sizeof(double2), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    set_err("work download failed");
+    return -1;
+  }
+  double2 *o = (double2 *)out;
+  for (int r = 0; r < R; ++r)
+    for (int odd = 0; odd < 2; ++odd)
+      for (int m = 0; m <= M; ++m)
+        for (int k = 0; k < K; ++k)
+          o[(((size_t)m * 2 + odd) * R + r) * K + k] =
+              q[(((size_t)r * 2 + odd) * (M + 1) + m) * K + k];
+  return 0;
+}
+
+int wavemoth_execute_device(wavemoth_plan p, const double *d_input, double *d_output,
+                            void *stream) {
+  g_err.clear();
+  if (!p || !d_input || !d_output) {
+    set_err("NULL argument");
+    return -1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (!run_legendre(p, (const double2 *)d_input, s)) return -1;
+  if (!run_rings(p, d_output, s)) return -1;
+  return 0;
+}
+
+int wavemoth_set_profiling(wavemoth_plan p, int on) {
+  g_err.clear();
+  if (!p) return -1;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  release_events(p);
+  p->profiling = on != 0;
+  return 0;
+}
+
+int wavemoth_stage_times(wavemoth_plan p, double *avg_ms, int *count) {
+  g_err.clear();
+  if (!p || !avg_ms || !count) return -1;
+  double acc[3] = {0, 0, 0};
+  int n = 0;
+  for (auto &a : p->ev_used) {
+    if (!a[0] || !a[1] || !a[2] || !a[3]) continue;
+    if (hipEventSynchronize(a[3]) != hipSuccess) {
+      set_err("hipEventSynchronize failed");
+      return -1;
+    }
+    for (int i = 0; i < 3; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, a[i], a[i + 1]) != hipSuccess) {
+        set_err("hipEventElapsedTime failed");
+        return -1;
+      }
+      acc[i] += ms;
+    }
+    ++n;
+  }
+  for (int i = 0; i < 3; ++i) avg_ms[i] = n ? acc[i] / n : 0.0;
+  *count = n;
+  release_events(p);
+  return 0;
+}
+
+int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
+  if (!p) return -1;
+  switch (key) {
+    case 0: return p->N;
+    case 1: return p->L;
+    case 2: return p->M;
+    case 3: return p->K;
+    case 4: return p->npix;
+    case 5: return p->T_dense;
+    case 6: return p->T_eps;
+    case 7: return p->flags;
+    case 8: return 1;
+    case 9: return (int64_t)p->dev_bytes;
+    default: return -1;
+  }
+}
+
+}  // extern "C"

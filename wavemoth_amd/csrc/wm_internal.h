@@ -1,0 +1,85 @@
+// Internal declarations shared by the HIP translation units of
+// libwavemoth_hip.so.  Not part of the public ABI (include/wavemoth.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace wm {
+
+// Length of an l-chunk of the Legendre recurrence.  Each m's l-range is
+// padded to a multiple of it; seeds and the monic rescaling live on the
+// chunk grid anchored at l = m.
+constexpr int kChunk = 16;
+// seed_l value of a (m, ring) column whose |Lambda| never reaches the
+// start threshold for l <= lmax.
+constexpr int kSeedNone = 0x3fffffff;
+// Largest power-of-two FFT length held in LDS by the ring kernel
+// (8192 complex doubles = 128 KiB); bounds Nside at 2048.
+constexpr int kMaxFFT = 8192;
+constexpr int kRingThreads = 512;
+
+// Device tables of the Legendre stage, all owned by one plan.
+struct LegTables {
+  int N = 0, L = 0, M = 0, K = 0, R = 0;
+  int RL = 0;        // rings per lane
+  int W = 0;         // ring chunks (waves) per m
+  int64_t nlpad = 0; // padded l entries over all m
+  int64_t *off = nullptr;     // [M+1] start of m's padded l-range
+  int *chunk_m = nullptr;     // [nlpad/kChunk] m owning each chunk
+  double *ctil = nullptr;     // [nlpad] monic coefficient (l^2-m^2)/(4l^2-1)
+  double *ascale = nullptr;   // [nlpad] A_l within its chunk, 0 past lmax
+  double *cscale = nullptr;   // [nlpad/kChunk] A at the chunk end
+  double2 *mfac = nullptr;    // [M+1] log2 Lambda_mm/sin^m as double-double
+  double *ring_x = nullptr;   // [R] cos(theta_r)
+  double2 *ring_ls = nullptr; // [R] log2 sin(theta_r) as double-double
+  int *seed_l = nullptr;      // [(M+1)*R] chunk-aligned start l
+  double2 *seed_v = nullptr;  // [(M+1)*R] (Lambda_{l-1} a_{l-1}, Lambda_l)
+  int *wstart = nullptr;      // [(M+1)*W] min seed_l over a wave's rings
+  int *wseedmax = nullptr;    // [(M+1)*W] max seed_l (<= lmax) over a wave
+  double2 *atil = nullptr;    // [nlpad*K] prescaled a_lm (per execute)
+  double2 *qbuf = nullptr;    // [R][2][M+1][K] even/odd Legendre sums
+};
+
+// One ring of the output map, for the ring (phase + alias + c2r) stage.
+struct RingInfo {
+  int64_t pix_off;   // first pixel of the ring
+  int64_t chirp_off; // Bluestein chirp table offset (h entries)
+  int64_t vhat_off;  // Bluestein kernel spectrum offset (P entries)
+  int n;             // pixels in the ring
+  int r;             // ring pair index, 0 = equator
+  int sgn;           // +1 north (even + odd), -1 south (even - odd)
+  int ph_num;        // phi0 = pi * ph_num / ph_den
+  int ph_den;
+  int P;             // Bluestein FFT length, 0 when n/2 is a power of two
+};
+
+struct RingTables {
+  int nrings = 0;
+  RingInfo *info = nullptr;   // [nrings]
+  int *order = nullptr;       // [nrings] launch order (costliest first)
+  double2 *twiddle = nullptr; // [kMaxFFT] e^{-2 pi i t / kMaxFFT}
+  double2 *chirp = nullptr;
+  double2 *vhat = nullptr;
+  int max_lds_complex = 0;    // LDS complex slots needed by the ring kernel
+};
+
+// Shape of the Legendre sweep for nmaps = K.
+int legendre_rings_per_lane(int K);
+int legendre_padded_maps(int K);
+
+// Launchers (return hipError_t of the launch).
+hipError_t launch_seeds(const LegTables &t, hipStream_t s);
+hipError_t launch_wave_bounds(const LegTables &t, hipStream_t s);
+hipError_t launch_prep_alm(const LegTables &t, const double2 *alm, hipStream_t s);
+hipError_t launch_legendre(const LegTables &t, hipStream_t s);
+hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const int64_t *choff,
+                                 const int64_t *vhoff, const int *plist, int nh,
+                                 hipStream_t s);
+hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
+                             hipStream_t s);
+
+}  // namespace wm
