@@ -65,6 +65,11 @@ def lib():
                                               ctypes.c_int]
         L.or_plan_trimmed_triples.restype = ctypes.c_int64
         L.or_num_threads.restype = ctypes.c_int
+        L.or_ylm_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.or_ylm_new.restype = ctypes.c_void_p
+        L.or_ylm_free.argtypes = [ctypes.c_void_p]
+        L.or_ylm_column.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _d]
+        L.or_ylm_column.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -118,6 +123,22 @@ def lambda_matrix(m, thetas, lmax, eps=1e-300):
     compute_normalized_associated_legendre (legendre.pyx:24-57) *without*
     its cross-column m_crit memory."""
     rows = [lambda_column(m, lmax, t, eps)[0][m:] for t in np.atleast_1d(thetas)]
+    return np.array(rows)
+
+
+def lambda_matrix_shared(m, thetas, lmax, eps=1e-300):
+    """Like lambda_matrix but with one generator over all columns in order,
+    as legendre.pyx:24-57 does (so the m_crit skip carries across columns)."""
+    L = lib()
+    g = L.or_ylm_new(lmax, lmax, eps)
+    out = np.zeros(lmax + 1)
+    rows = []
+    try:
+        for t in np.atleast_1d(thetas):
+            L.or_ylm_column(g, m, float(t), _ptr(out))
+            rows.append(out[m:].copy())
+    finally:
+        L.or_ylm_free(g)
     return np.array(rows)
 
 
