@@ -148,3 +148,15 @@ def test_torch_device_path_matches_host_path():
     host = wm.ShtPlan(N, L, L, alm, np.zeros((12 * N * N, K)), 'mmajor').execute().copy()
     dev = synthesis(torch.from_numpy(alm).cuda(), N, L).cpu().numpy()
     assert np.array_equal(host, dev)
+
+
+def test_large_mmax_small_nside(oracle):
+    """mmax above one LDS staging round of the ring stage (4608 values of m)."""
+    N, L = 4, 4700
+    print("case N=%d L=%d" % (N, L), flush=True)
+    plan, alm = _plan(N, L, L, 1, seed=77)
+    gpu = plan.execute().copy()
+    ref = oracle.OraclePlan(N, L, L, 1).synthesize(alm)
+    err = rel_rms(gpu, ref)
+    print("rel rms", err)
+    assert err <= 1e-12

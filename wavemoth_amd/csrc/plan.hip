@@ -226,56 +226,53 @@ static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 static bool build_ring_tables(wavemoth_plan p) {
   RingTables &rt = p->rt;
   const int N = p->N;
-  const int nrings = 4 * N - 1, mid = 2 * N - 1;
-  rt.nrings = nrings;
-  std::vector<RingInfo> info(nrings);
-  std::vector<double> cost(nrings);
-  // distinct Bluestein lengths
+  const int nrings = 4 * N - 1, mid = 2 * N - 1, R = 2 * N;
+  // Ring length of iring and phi0 = pi * num / den of grid entry iring
+  // (src/wavemoth.c:1199-1232); den always equals the ring length.
+  auto geom = [&](int iring, int &n, int &num) {
+    if (iring <= N - 1) {
+      n = 4 * (iring + 1);
+      num = 1;
+    } else if (iring > 3 * N - 1) {
+      n = 4 * (nrings - iring);
+      num = 1;
+    } else {
+      n = 4 * N;
+      num = iring % 2;
+    }
+  };
+  std::vector<int64_t> offs(nrings + 1, 0);
+  for (int i = 0; i < nrings; ++i) {
+    int n, num;
+    geom(i, n, num);
+    offs[i + 1] = offs[i] + n;
+  }
+  p->npix = offs[nrings];
+  rt.npairs = R;
+  std::vector<RingPair> pairs(R);
+  std::vector<double> cost(R);
+  std::vector<int> need(R);
   std::vector<int> hlist, plist;
   std::vector<int64_t> choff, vhoff;
   std::vector<int> h_index(2 * N + 1, -1);
   int64_t chtot = 0, vhtot = 0;
-  int max_lds = 0;
-  int64_t pix = 0;
-  // Ring length of iring, and phi0 = pi * num / den of the grid entry iring
-  // (src/wavemoth.c:1199-1232).
-  auto geom = [&](int iring, int &n, int &num, int &den) {
-    if (iring <= N - 1) {
-      n = 4 * (iring + 1);
-      num = 1;
-      den = 4 * (iring + 1);
-    } else if (iring > 3 * N - 1) {
-      n = 4 * (nrings - iring);
-      num = 1;
-      den = 4 * (nrings - iring);
-    } else {
-      n = 4 * N;
-      num = iring % 2;
-      den = 4 * N;
-    }
-  };
-  for (int iring = 0; iring < nrings; ++iring) {
-    int n, num, den, n_s;
-    geom(iring, n, num, den);
-    // Both rings of a pair are phase-shifted with the phi0 of the southern
-    // ring mid + r, as src/wavemoth.c:542 (ri->phi0 = phi0s[mid_ring + ...])
-    // does; it differs from the ring's own phi0 only for Nside = 1.
-    const int rpair = (iring <= mid) ? (mid - iring) : (iring - mid);
-    geom(mid + rpair, n_s, num, den);
-    RingInfo &ri = info[iring];
-    ri.pix_off = pix;
-    pix += n;
-    ri.n = n;
-    ri.r = (iring <= mid) ? (mid - iring) : (iring - mid);
-    ri.sgn = (iring <= mid) ? 1 : -1;
-    ri.ph_num = p->phase_shifts ? num : 0;
-    ri.ph_den = den;
+  for (int r = 0; r < R; ++r) {
+    // The pair is processed with the southern ring's length and phi0, as
+    // src/wavemoth.c:532-546 sets ri->phi0 / ri->length from mid_ring + r;
+    // it differs from the northern ring's own phi0 only for Nside = 1.
+    int n, num;
+    geom(mid + r, n, num);
+    RingPair &rp = pairs[r];
+    rp.pix_n = offs[mid - r];
+    rp.pix_s = (r == 0) ? -1 : offs[mid + r];
+    rp.n = n;
+    rp.s = p->phase_shifts ? num : 0;
+    rp.r = r;
     const int h = n / 2;
     if (is_pow2(h)) {
-      ri.P = 0;
-      ri.chirp_off = ri.vhat_off = 0;
-      max_lds = std::max(max_lds, h + 1);
-      cost[iring] = (double)h * std::log2((double)h + 1.0) + (double)(p->M + 1);
+      rp.P = 0;
+      rp.chirp_off = rp.vhat_off = 0;
+      cost[r] = (double)h * std::log2((double)h + 1.0) + (double)(p->M + 1);
     } else {
       const int P = next_pow2(2 * h - 1);
       if (h_index[h] < 0) {
@@ -288,29 +285,48 @@ static bool build_ring_tables(wavemoth_plan p) {
         vhtot += P;
       }
       const int hi = h_index[h];
-      ri.P = P;
-      ri.chirp_off = choff[hi];
-      ri.vhat_off = vhoff[hi];
-      max_lds = std::max(max_lds, P);
-      cost[iring] = 2.0 * (double)P * std::log2((double)P) + (double)(p->M + 1);
+      rp.P = P;
+      rp.chirp_off = choff[hi];
+      rp.vhat_off = vhoff[hi];
+      cost[r] = 2.0 * (double)P * std::log2((double)P) + (double)(p->M + 1);
+    }
+    need[r] = ring_lds_slots(n, rp.P, p->M);
+    if (rp.P > kMaxFFT || (rp.P == 0 && h > kMaxFFT)) {
+      set_err("ring stage: Nside too large for the LDS-resident ring FFT (Nside <= 2048)");
+      return false;
     }
   }
-  p->npix = pix;
-  if (max_lds > kMaxFFT + 1) {
-    set_err("ring stage: Nside too large for the LDS-resident ring FFT (Nside <= 2048)");
-    return false;
+  // Two launch groups: rings whose LDS need exceeds 80 KiB (one block per
+  // CU) and the rest (two or more per CU); costliest first within each.
+  const int big_slots = (80 * 1024) / (int)sizeof(double2);
+  auto cls = [&](int r) { return need[r] > big_slots ? 1 : 0; };
+  std::vector<int> order(R);
+  for (int i = 0; i < R; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (cls(a) != cls(b)) return cls(a) > cls(b);
+    return cost[a] > cost[b];
+  });
+  rt.nlaunch = 0;
+  int max_need = 0;
+  for (int i = 0; i < R; ++i) {
+    if (rt.nlaunch == 0 || cls(order[i]) != cls(order[rt.first[rt.nlaunch - 1]])) {
+      rt.first[rt.nlaunch] = i;
+      rt.count[rt.nlaunch] = 0;
+      rt.lds[rt.nlaunch] = 0;
+      ++rt.nlaunch;
+    }
+    const int g = rt.nlaunch - 1;
+    rt.count[g]++;
+    rt.lds[g] = std::max(rt.lds[g], need[order[i]]);
+    max_need = std::max(max_need, need[order[i]]);
   }
-  rt.max_lds_complex = max_lds;
-  std::vector<int> order(nrings);
-  for (int i = 0; i < nrings; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return cost[a] > cost[b]; });
+  WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");
   std::vector<double2> tw(kMaxFFT);
   for (int j = 0; j < kMaxFFT; ++j) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * j / kMaxFFT;
     tw[j] = make_double2((double)cosl(a), (double)sinl(a));
   }
-  if (!upload(p, &rt.info, info, "ring info")) return false;
+  if (!upload(p, &rt.pairs, pairs, "ring pairs")) return false;
   if (!upload(p, &rt.order, order, "ring order")) return false;
   if (!upload(p, &rt.twiddle, tw, "twiddle")) return false;
   if (!dalloc(p, &rt.chirp, (size_t)chtot, "chirp")) return false;
@@ -329,16 +345,16 @@ static bool build_ring_tables(wavemoth_plan p) {
   return true;
 }
 
-// Rewrites only the phase fields of the ring table (phase on/off).
+// Clears the phi0 phase of every ring pair (phase shifting off).
 static bool refresh_phases(wavemoth_plan p) {
-  std::vector<RingInfo> info(p->rt.nrings);
-  WM_TRY(hipMemcpy(info.data(), p->rt.info, info.size() * sizeof(RingInfo),
+  std::vector<RingPair> pairs(p->rt.npairs);
+  WM_TRY(hipMemcpy(pairs.data(), p->rt.pairs, pairs.size() * sizeof(RingPair),
                    hipMemcpyDeviceToHost),
-         "ring info download");
-  for (auto &ri : info) ri.ph_num = 0;
-  WM_TRY(hipMemcpy(p->rt.info, info.data(), info.size() * sizeof(RingInfo),
+         "ring table download");
+  for (auto &rp : pairs) rp.s = 0;
+  WM_TRY(hipMemcpy(p->rt.pairs, pairs.data(), pairs.size() * sizeof(RingPair),
                    hipMemcpyHostToDevice),
-         "ring info upload");
+         "ring table upload");
   return true;
 }
 
@@ -585,6 +601,10 @@ int wavemoth_execute_device(wavemoth_plan p, const double *d_input, double *d_ou
   g_err.clear();
   if (!p || !d_input || !d_output) {
     set_err("NULL argument");
+    return -1;
+  }
+  if (((uintptr_t)d_input | (uintptr_t)d_output) & 15) {
+    set_err("device buffers must be 16-byte aligned");
     return -1;
   }
   hipStream_t s = (hipStream_t)stream;

@@ -1,24 +1,49 @@
 // Ring stage of the synthesis: even/odd fold, phi0 phase, aliasing into the
-// ring's half spectrum and the unnormalised complex-to-real DFT, one
-// workgroup per ring, everything in LDS.
+// rings' half spectra and the unnormalised complex-to-real DFTs.  One
+// workgroup per ring pair (the northern ring and its southern mirror share
+// the Legendre sums E, O), everything else in LDS.
 //
-// Semantics follow src/wavemoth.c:1083-1191: for ring pair r and
-// hemisphere s (+1 north, -1 south), G_m = (E_m + s O_m) e^{i m phi0};
-// bin (m mod n) receives G_m and bin ((n - m) mod n) receives conj(G_m)
-// (m > 0), each only when the bin is <= n/2; then an FFTW-style c2r of
-// length n (imaginary parts of the DC and Nyquist bins ignored, no
-// normalisation).
+// Semantics follow src/wavemoth.c:1083-1191: for hemisphere sign
+// s_h = +1 (north) / -1 (south), G_m = (E_m + s_h O_m) e^{i m phi0}; bin
+// (m mod n) receives G_m and bin ((n - m) mod n) receives conj(G_m) when
+// m > 0, each only if the bin is <= n/2; then an FFTW-style c2r of length n
+// (imaginary parts of the DC and Nyquist bins ignored, no normalisation).
 //
-// The length-n c2r is computed as a complex DFT of length h = n/2:
-//   Z_k = (X_k + conj X_{h-k}) + i e^{i pi k/h} (X_k - conj X_{h-k}),
+// On the HEALPix grid phi0 = pi s / n with s in {0, 1} for every ring, so
+// with m = b + j n the phase is e^{i pi s b/n} (-1)^{s j}: one complex
+// factor per bin multiplies the signed alias sum
+//   X_b = e^{i pi s b/n} [ sum_j (-1)^{sj} g_{b+jn} + sum_{j>=1} (-1)^{sj} conj g_{jn-b} ].
+//
+// The length-n c2r is a complex DFT of length h = n/2:
+//   Z_k = (X_k + conj X_{h-k}) + i e^{2 pi i k/n} (X_k - conj X_{h-k}),
 //   z = IDFT_h(Z),  f[2p] = Re z_p,  f[2p+1] = Im z_p.
-// h is a power of two for the 2N+1 belt rings (radix-4/2 Stockham in LDS);
-// the polar-cap lengths h = 2i use Bluestein's chirp-z convolution with a
-// power-of-two FFT of P >= 2h-1 points, the chirp and the spectrum of its
-// conjugate being tabulated per h at plan time.
+// Power-of-two h (the belt, 2N+1 rings) uses a radix-8/4/2 Stockham FFT in
+// LDS; other h (polar caps, h = 2i) use Bluestein's chirp-z convolution with
+// a power-of-two FFT of P >= 2h-1 points, the chirp and the spectrum of its
+// conjugate tabulated per h at plan time.  LDS arrays are padded by one
+// slot per 8 (pidx) so the Stockham writes of the first passes do not pile
+// onto one bank.
 #include "wm_internal.h"
 
 namespace wm {
+
+constexpr int NT = kRingThreads;
+constexpr int kBins = (kMaxFFT / 2 + 1 + NT - 1) / NT;  // bins per thread
+constexpr int kPairs = (kMaxFFT / 4 + 1 + NT - 1) / NT; // Z pairs per thread
+constexpr int kOut = (kMaxFFT / 2 + NT - 1) / NT;       // output pairs per thread
+constexpr int kFull = (kMaxFFT + NT - 1) / NT;          // FFT points per thread
+constexpr int kStage = (kMaxFFT / 2 + 1 + NT - 1) / NT; // staged m per thread
+constexpr int kStageM = kStage * NT;                    // m staged per round
+
+__host__ __device__ __forceinline__ int pidx(int d) { return d + (d >> 3); }
+
+// Number of j-groups that share the alias sum of one bin (B bins per ring),
+// so that short rings still spread their (M+1) loads over the block.
+__host__ __device__ __forceinline__ int ring_groups(int B) {
+  int G = 1;
+  while (2 * G * B <= NT) G *= 2;
+  return G;
+}
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -26,104 +51,138 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 // multiply by +i / -i
 __device__ __forceinline__ double2 cmul_i(double2 a) { return make_double2(-a.y, a.x); }
 __device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
 
-// In-place FFT of length P (power of two, <= kMaxFFT) held in LDS, by the
-// whole block (kRingThreads threads).  Stockham autosort, radix 4 while
-// P/Ns is divisible by 4, then one radix-2 stage.  Each stage reads all
-// its butterfly inputs into registers, syncs, writes, syncs.
-// INV = false: e^{-2 pi i jk/P};  INV = true: e^{+2 pi i jk/P}.
 template <bool INV>
-__device__ void lds_fft(double2 *buf, int P, const double2 *__restrict__ tw) {
+__device__ __forceinline__ void dft4(double2 &a, double2 &b, double2 &c, double2 &d) {
+  const double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
+  a = cadd(t0, t2);
+  c = csub(t0, t2);
+  if (INV) {
+    b = cadd(t1, cmul_i(t3));
+    d = cadd(t1, cmul_mi(t3));
+  } else {
+    b = cadd(t1, cmul_mi(t3));
+    d = cadd(t1, cmul_i(t3));
+  }
+}
+
+// In-register 8-point DFT, y_k = sum_r v_r e^{-/+ 2 pi i r k / 8}.
+template <bool INV>
+__device__ __forceinline__ void dft8(double2 v[8]) {
+  constexpr double c = 0.70710678118654752440;
+  dft4<INV>(v[0], v[2], v[4], v[6]);
+  dft4<INV>(v[1], v[3], v[5], v[7]);
+  // after dft4: e_k in v[0], v[2], v[4], v[6] (k = 0..3), o_k likewise odd
+  const double2 w1 = INV ? make_double2(c, c) : make_double2(c, -c);
+  const double2 w3 = INV ? make_double2(-c, c) : make_double2(-c, -c);
+  const double2 o1 = cmul(v[3], w1);
+  const double2 o2 = INV ? cmul_i(v[5]) : cmul_mi(v[5]);
+  const double2 o3 = cmul(v[7], w3);
+  const double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+// Twiddle table in LDS: e^{-2 pi i t / kMaxFFT} = C[t >> 6] * F[t & 63],
+// C[u] = e^{-2 pi i 64 u / kMaxFFT} (kMaxFFT/64 entries), F[v] =
+// e^{-2 pi i v / kMaxFFT} (64 entries).
+constexpr int kTwC = kMaxFFT / 64;
+constexpr int kTwSlots = kTwC + 64;
+
+__device__ __forceinline__ double2 twiddle(const double2 *twl, int t) {
+  return cmul(twl[t >> 6], twl[kTwC + (t & 63)]);
+}
+
+// One Stockham pass of radix R over P points held in LDS (padded).
+template <int R, bool INV>
+__device__ __forceinline__ void stockham_pass(double2 *buf, int P, int Ns,
+                                              const double2 *twl) {
+  constexpr int B = (kMaxFFT / R + NT - 1) / NT;
   const int tid = threadIdx.x;
-  constexpr int NT = kRingThreads;
-  constexpr int B4 = (kMaxFFT / 4 + NT - 1) / NT;
-  constexpr int B2 = (kMaxFFT / 2 + NT - 1) / NT;
+  const int nb = P / R;
+  const int tstep = kMaxFFT / (Ns * R);
+  double2 v[B][R];
+#pragma unroll
+  for (int it = 0; it < B; ++it) {
+    const int j = tid + it * NT;
+    if (j < nb) {
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = buf[pidx(j + r * nb)];
+      if (k) {
+        double2 w1 = twiddle(twl, k * tstep);
+        if (INV) w1.y = -w1.y;
+        double2 w = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[it][r] = cmul(v[it][r], w);
+          if (r + 1 < R) w = cmul(w, w1);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < B; ++it) {
+    const int j = tid + it * NT;
+    if (j < nb) {
+      const int k = j & (Ns - 1);
+      if (R == 8) {
+        dft8<INV>(v[it]);
+      } else if (R == 4) {
+        dft4<INV>(v[it][0], v[it][1], v[it][2], v[it][3]);
+      } else {
+        const double2 a = v[it][0];
+        v[it][0] = cadd(a, v[it][1]);
+        v[it][1] = csub(a, v[it][1]);
+      }
+      const int d = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pidx(d + r * Ns)] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place power-of-two FFT (P <= kMaxFFT) in padded LDS by the whole block.
+// INV = false: e^{-2 pi i jk/P};  INV = true: e^{+2 pi i jk/P}.  Expects the
+// data visible to all threads on entry; leaves it so on exit.
+template <bool INV>
+__device__ void lds_fft(double2 *buf, int P, const double2 *twl) {
   int Ns = 1;
   while (Ns < P) {
-    if (((P / Ns) & 3) == 0) {
-      const int nb = P >> 2;
-      const int tstep = kMaxFFT / (Ns * 4);
-      double2 v[B4][4];
-#pragma unroll
-      for (int it = 0; it < B4; ++it) {
-        const int j = tid + it * NT;
-        if (j < nb) {
-          const int k = j & (Ns - 1);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) v[it][r] = buf[j + r * nb];
-          if (k) {
-#pragma unroll
-            for (int r = 1; r < 4; ++r) {
-              double2 w = tw[k * r * tstep];
-              if (INV) w.y = -w.y;
-              v[it][r] = cmul(v[it][r], w);
-            }
-          }
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < B4; ++it) {
-        const int j = tid + it * NT;
-        if (j < nb) {
-          const int k = j & (Ns - 1);
-          const double2 a0 = cadd(v[it][0], v[it][2]);
-          const double2 a1 = csub(v[it][0], v[it][2]);
-          const double2 a2 = cadd(v[it][1], v[it][3]);
-          const double2 a3 = csub(v[it][1], v[it][3]);
-          const int d = (j - k) * 4 + k;
-          buf[d] = cadd(a0, a2);
-          buf[d + 2 * Ns] = csub(a0, a2);
-          if (INV) {
-            buf[d + Ns] = cadd(a1, cmul_i(a3));
-            buf[d + 3 * Ns] = cadd(a1, cmul_mi(a3));
-          } else {
-            buf[d + Ns] = cadd(a1, cmul_mi(a3));
-            buf[d + 3 * Ns] = cadd(a1, cmul_i(a3));
-          }
-        }
-      }
-      __syncthreads();
-      Ns <<= 2;
+    const int rem = P / Ns;
+    if (rem >= 8) {
+      stockham_pass<8, INV>(buf, P, Ns, twl);
+      Ns *= 8;
+    } else if (rem == 4) {
+      stockham_pass<4, INV>(buf, P, Ns, twl);
+      Ns *= 4;
     } else {
-      const int nb = P >> 1;
-      const int tstep = kMaxFFT / (Ns * 2);
-      double2 v[B2][2];
-#pragma unroll
-      for (int it = 0; it < B2; ++it) {
-        const int j = tid + it * NT;
-        if (j < nb) {
-          const int k = j & (Ns - 1);
-          v[it][0] = buf[j];
-          v[it][1] = buf[j + nb];
-          if (k) {
-            double2 w = tw[k * tstep];
-            if (INV) w.y = -w.y;
-            v[it][1] = cmul(v[it][1], w);
-          }
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < B2; ++it) {
-        const int j = tid + it * NT;
-        if (j < nb) {
-          const int k = j & (Ns - 1);
-          const int d = (j - k) * 2 + k;
-          buf[d] = cadd(v[it][0], v[it][1]);
-          buf[d + Ns] = csub(v[it][0], v[it][1]);
-        }
-      }
-      __syncthreads();
-      Ns <<= 1;
+      stockham_pass<2, INV>(buf, P, Ns, twl);
+      Ns *= 2;
     }
   }
 }
 
-// e^{i pi num/den} with num reduced exactly modulo 2 den first.
+// Fills the LDS twiddle table from the global one (block-wide; syncs).
+__device__ __forceinline__ void load_twiddles(double2 *twl, const double2 *__restrict__ tw) {
+  for (int u = threadIdx.x; u < kTwSlots; u += NT)
+    twl[u] = (u < kTwC) ? tw[u * 64] : tw[u - kTwC];
+  __syncthreads();
+}
+
+// e^{i pi num/den}, num reduced exactly modulo 2 den first.
 __device__ __forceinline__ double2 expipi(int64_t num, int64_t den) {
   int64_t t = num % (2 * den);
   if (t < 0) t += 2 * den;
@@ -132,143 +191,254 @@ __device__ __forceinline__ double2 expipi(int64_t num, int64_t den) {
   return make_double2(c, s);
 }
 
-// Half-spectrum bin b of one ring for map k (see the header comment).
-__device__ double2 gather_bin(int b, int n, int M, const double2 *__restrict__ qe,
-                              const double2 *__restrict__ qo, int K, int k, double sgn,
-                              int ph_num, int ph_den) {
-  double2 acc = make_double2(0.0, 0.0);
-  for (int m = b; m <= M; m += n) {
-    const double2 e = qe[(int64_t)m * K + k], o = qo[(int64_t)m * K + k];
-    const double2 g = make_double2(fma(sgn, o.x, e.x), fma(sgn, o.y, e.y));
-    acc = cadd(acc, cmul(g, expipi((int64_t)m * ph_num, ph_den)));
-  }
-  for (int m = n - b; m <= M; m += n) {
-    const double2 e = qe[(int64_t)m * K + k], o = qo[(int64_t)m * K + k];
-    const double2 g = make_double2(fma(sgn, o.x, e.x), fma(sgn, o.y, e.y));
-    acc = cadd(acc, cconj(cmul(g, expipi((int64_t)m * ph_num, ph_den))));
-  }
-  return acc;
-}
-
-__global__ __launch_bounds__(kRingThreads) void k_ring_synth(LegTables t, RingTables rt,
-                                                             double *__restrict__ map) {
+__global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, int first,
+                                                   int lds_slots, double *__restrict__ map) {
   extern __shared__ double2 lds[];
-  const int tid = threadIdx.x;
-  constexpr int NT = kRingThreads;
-  const RingInfo ri = rt.info[rt.order[blockIdx.x]];
-  const int n = ri.n, h = n / 2, K = t.K, M = t.M;
-  const double sgn = (double)ri.sgn;
-  const double2 *qe = t.qbuf + ((int64_t)ri.r * 2 + 0) * (M + 1) * K;
-  const double2 *qo = t.qbuf + ((int64_t)ri.r * 2 + 1) * (M + 1) * K;
-  const int P = ri.P;
-  double *out = map + ri.pix_off * K;
+  double2 *twl = lds + lds_slots - kTwSlots;  // twiddle table at the end of LDS
+  const RingPair rp = rt.pairs[rt.order[first + blockIdx.x]];
+  const int n = rp.n, h = n / 2, B = h + 1, K = t.K, M = t.M, P = rp.P;
+  const bool south = rp.pix_s >= 0;
+  const double2 *qe = t.qbuf + ((int64_t)rp.r * 2 + 0) * (M + 1) * K;
+  const double2 *qo = t.qbuf + ((int64_t)rp.r * 2 + 1) * (M + 1) * K;
+  const double2 *chirp = rt.chirp + rp.chirp_off;
+  const double2 *vhat = rt.vhat + rp.vhat_off;
+  const int xoff = (P == 0) ? 0 : (P - h - 1);
+  // alias sums: G groups of j per bin when the ring is short
+  const int G = ring_groups(B);
+  const int J = (M + h) / n + 2;   // j runs over [0, J)
+  const int JG = (J + G - 1) / G;  // j-iterations per group
+  load_twiddles(twl, rt.twiddle);
 
+  const int nhemi = south ? 2 : 1;
   for (int k = 0; k < K; ++k) {
-    // X region: [0, h] for power-of-two h, [P-h-1, P) for Bluestein.
-    double2 *X = (P == 0) ? lds : lds + (P - h - 1);
-    for (int b = tid; b <= h; b += NT) {
-      double2 v = gather_bin(b, n, M, qe, qo, K, k, sgn, ri.ph_num, ri.ph_den);
-      if (b == 0 || b == h) v.y = 0.0;
-      X[b] = v;
-    }
-    __syncthreads();
-    if (P == 0) {
-      for (int a = tid; a <= h / 2; a += NT) {
-        const int c = h - a;
-        const double2 xa = X[a], xc = X[c];
-        const double2 wa = expipi(a, h);
-        const double2 za = cadd(cadd(xa, cconj(xc)), cmul_i(cmul(wa, csub(xa, cconj(xc)))));
-        double2 zc = make_double2(0.0, 0.0);
-        const bool two = (a != 0) && (a != c);
-        if (two) {
-          const double2 wc = expipi(c, h);
-          zc = cadd(cadd(xc, cconj(xa)), cmul_i(cmul(wc, csub(xc, cconj(xa)))));
+    for (int hemi = 0; hemi < nhemi; ++hemi) {
+      // Opaque copy of the thread index: keeps the per-bin LDS/global
+      // addresses from being hoisted out of the map/hemisphere loops (they
+      // would stay live across the FFTs and spill).
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      const double hs = hemi ? -1.0 : 1.0;  // E + O north, E - O south
+      double *out = map + (hemi ? rp.pix_s : rp.pix_n) * K;
+      // 1.-2. g_m = E_m +/- O_m staged in LDS, kStage*NT values of m at a
+      // time (one round of independent loads), then the signed alias sums
+      // per bin: slot (i) -> bin b = slot % B, group g = slot / B; direct
+      // terms m = b + j n, conjugate terms m = j n - b.
+      double2 xv[kBins];
+#pragma unroll
+      for (int i = 0; i < kBins; ++i) xv[i] = make_double2(0.0, 0.0);
+      for (int mb = 0; mb <= M; mb += kStageM) {
+        const int me = (mb + kStageM <= M + 1) ? mb + kStageM : M + 1;
+#pragma unroll
+        for (int i = 0; i < kStage; ++i) {
+          const int m = mb + tid + NT * i;
+          if (m < me) {
+            const double2 e = qe[(int64_t)m * K + k], o = qo[(int64_t)m * K + k];
+            lds[pidx(m - mb)] = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
+          }
         }
-        lds[a] = za;
-        if (two) lds[c] = zc;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kBins; ++i) {
+          const int slot = tid + NT * i;
+          const int g = slot / B, b = slot - g * B;
+          if (g < G) {
+            for (int jj = 0; jj < JG; ++jj) {
+              const int j = g + jj * G;
+              const double sg = (rp.s && (j & 1)) ? -1.0 : 1.0;
+              const int m1 = b + j * n, m2 = j * n - b;
+              if (m1 >= mb && m1 < me) {
+                const double2 v = lds[pidx(m1 - mb)];
+                xv[i].x = fma(sg, v.x, xv[i].x);
+                xv[i].y = fma(sg, v.y, xv[i].y);
+              }
+              if (j >= 1 && m2 >= mb && m2 < me) {
+                const double2 v = lds[pidx(m2 - mb)];
+                xv[i].x = fma(sg, v.x, xv[i].x);
+                xv[i].y = fma(-sg, v.y, xv[i].y);
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (G > 1) {
+        // reduce the G partial sums of each bin in a fixed order
+        if (tid < G * B) lds[tid] = xv[0];
+        __syncthreads();
+        if (tid < B) {
+          double2 acc = make_double2(0.0, 0.0);
+          for (int g = 0; g < G; ++g) acc = cadd(acc, lds[g * B + tid]);
+          xv[0] = acc;
+        }
+        __syncthreads();
+      }
+      // 3. phase e^{i pi s b / n}, c2r semantics for DC / Nyquist, into LDS.
+      // The phases of a thread's bins b = tid + NT i follow by angle
+      // addition from two sincospi evaluations (error ~ kBins ulp).
+      {
+        const double2 step = expipi((int64_t)rp.s * NT, n);
+        double2 w = expipi((int64_t)rp.s * tid, n);
+#pragma unroll
+        for (int i = 0; i < kBins; ++i) {
+          const int b = tid + NT * i;
+          if (b < B) {
+            double2 x = cmul(xv[i], w);
+            if (b == 0 || b == h) x.y = 0.0;
+            lds[pidx(xoff + b)] = x;
+          }
+          w = cmul(w, step);
+        }
       }
       __syncthreads();
-      lds_fft<true>(lds, h, rt.twiddle);
-      for (int p = tid; p < h; p += NT) {
-        const double2 z = lds[p];
-        out[(int64_t)(2 * p) * K + k] = z.x;
-        out[(int64_t)(2 * p + 1) * K + k] = z.y;
-      }
-    } else {
-      const double2 *chirp = rt.chirp + ri.chirp_off;
-      const double2 *vhat = rt.vhat + ri.vhat_off;
-      for (int a = tid; a < h; a += NT) {
-        const double2 xa = X[a], xc = X[h - a];
-        const double2 wa = expipi(a, h);
-        const double2 za = cadd(cadd(xa, cconj(xc)), cmul_i(cmul(wa, csub(xa, cconj(xc)))));
-        lds[a] = cmul(za, chirp[a]);
+      // 4. Z pairs (a, h - a), twiddle e^{2 pi i a / n} by angle addition
+      {
+        const double2 step = expipi(2 * NT, n);
+        double2 om = expipi(2 * tid, n);
+#pragma unroll
+        for (int i = 0; i < kPairs; ++i) {
+          const int a = tid + NT * i;
+          if (a <= h / 2) {
+            const int c = h - a;
+            const double2 xa = lds[pidx(xoff + a)], xc = lds[pidx(xoff + c)];
+            const double2 za =
+                cadd(cadd(xa, cconj(xc)), cmul_i(cmul(om, csub(xa, cconj(xc)))));
+            const bool both = (a != 0) && (a != c);
+            double2 zc = make_double2(0.0, 0.0);
+            if (both) {
+              const double2 oc = make_double2(-om.x, om.y);  // e^{2 pi i c/n}
+              zc = cadd(cadd(xc, cconj(xa)), cmul_i(cmul(oc, csub(xc, cconj(xa)))));
+            }
+            if (P == 0) {
+              lds[pidx(a)] = za;
+              if (both) lds[pidx(c)] = zc;
+            } else {
+              lds[pidx(a)] = cmul(za, chirp[a]);
+              if (both) lds[pidx(c)] = cmul(zc, chirp[c]);
+            }
+          }
+          om = cmul(om, step);
+        }
       }
       __syncthreads();
-      for (int a = h + tid; a < P; a += NT) lds[a] = make_double2(0.0, 0.0);
-      __syncthreads();
-      lds_fft<false>(lds, P, rt.twiddle);
-      for (int a = tid; a < P; a += NT) lds[a] = cmul(lds[a], vhat[a]);
-      __syncthreads();
-      lds_fft<true>(lds, P, rt.twiddle);
-      for (int p = tid; p < h; p += NT) {
-        const double2 z = cmul(lds[p], chirp[p]);
-        out[(int64_t)(2 * p) * K + k] = z.x;
-        out[(int64_t)(2 * p + 1) * K + k] = z.y;
+      // 5. transforms and output
+      if (P == 0) {
+        lds_fft<true>(lds, h, twl);
+#pragma unroll
+        for (int i = 0; i < kOut; ++i) {
+          const int p = tid + NT * i;
+          if (p < h) {
+            const double2 z = lds[pidx(p)];
+            if (K == 1) {
+              *reinterpret_cast<double2 *>(out + 2 * p) = z;
+            } else {
+              out[(int64_t)(2 * p) * K + k] = z.x;
+              out[(int64_t)(2 * p + 1) * K + k] = z.y;
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kFull; ++i) {
+          const int a = h + tid + NT * i;
+          if (a < P) lds[pidx(a)] = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        lds_fft<false>(lds, P, twl);
+#pragma unroll
+        for (int i = 0; i < kFull; ++i) {
+          const int a = tid + NT * i;
+          if (a < P) lds[pidx(a)] = cmul(lds[pidx(a)], vhat[a]);
+        }
+        __syncthreads();
+        lds_fft<true>(lds, P, twl);
+#pragma unroll
+        for (int i = 0; i < kOut; ++i) {
+          const int p = tid + NT * i;
+          if (p < h) {
+            const double2 z = cmul(lds[pidx(p)], chirp[p]);
+            if (K == 1) {
+              *reinterpret_cast<double2 *>(out + 2 * p) = z;
+            } else {
+              out[(int64_t)(2 * p) * K + k] = z.x;
+              out[(int64_t)(2 * p + 1) * K + k] = z.y;
+            }
+          }
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 // Plan-time Bluestein tables, one block per distinct h:
 // chirp[a] = e^{i pi a^2 / h} (a < h), and vhat = FFT_P(v) / P with
 // v[d] = conj(chirp[|d|]) placed cyclically (d in (-h, h)).
-__global__ __launch_bounds__(kRingThreads) void k_bluestein_prep(RingTables rt,
-                                                                 const int *__restrict__ hlist,
-                                                                 const int64_t *__restrict__ choff,
-                                                                 const int64_t *__restrict__ vhoff,
-                                                                 const int *__restrict__ plist) {
+__global__ __launch_bounds__(NT) void k_bluestein_prep(RingTables rt,
+                                                       const int *__restrict__ hlist,
+                                                       const int64_t *__restrict__ choff,
+                                                       const int64_t *__restrict__ vhoff,
+                                                       const int *__restrict__ plist) {
   extern __shared__ double2 lds[];
   const int tid = threadIdx.x;
-  constexpr int NT = kRingThreads;
   const int h = hlist[blockIdx.x], P = plist[blockIdx.x];
+  double2 *twl = lds + pidx(kMaxFFT - 1) + 1;
+  load_twiddles(twl, rt.twiddle);
   double2 *chirp = rt.chirp + choff[blockIdx.x];
   double2 *vhat = rt.vhat + vhoff[blockIdx.x];
-  for (int a = tid; a < P; a += NT) lds[a] = make_double2(0.0, 0.0);
+  for (int a = tid; a < P; a += NT) lds[pidx(a)] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int a = tid; a < h; a += NT) {
     const int64_t a2 = (int64_t)a * a;
     const double2 c = expipi(a2 % (2 * (int64_t)h), h);
     chirp[a] = c;
-    lds[a] = cconj(c);
-    if (a > 0) lds[P - a] = cconj(c);
+    lds[pidx(a)] = cconj(c);
+    if (a > 0) lds[pidx(P - a)] = cconj(c);
   }
   __syncthreads();
-  lds_fft<false>(lds, P, rt.twiddle);
+  lds_fft<false>(lds, P, twl);
   const double inv = 1.0 / (double)P;
-  for (int a = tid; a < P; a += NT) vhat[a] = make_double2(lds[a].x * inv, lds[a].y * inv);
+  for (int a = tid; a < P; a += NT) vhat[a] = cscale(lds[pidx(a)], inv);
+}
+
+int ring_lds_slots(int n, int P, int M) {
+  const int h = n / 2;
+  int need = (P == 0) ? pidx(h) + 1 : pidx(P - 1) + 1;
+  const int G = ring_groups(h + 1);
+  if (G > 1 && G * (h + 1) > need) need = G * (h + 1);
+  const int ms = (M < kStageM - 1) ? M : kStageM - 1;
+  if (pidx(ms) + 1 > need) need = pidx(ms) + 1;  // staged g_m
+  return need + kTwSlots;
 }
 
 hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const int64_t *choff,
                                  const int64_t *vhoff, const int *plist, int nh, hipStream_t s) {
   if (nh <= 0) return hipSuccess;
-  const size_t lds = (size_t)kMaxFFT * sizeof(double2);
+  const size_t lds = (size_t)(pidx(kMaxFFT - 1) + 1 + kTwSlots) * sizeof(double2);
   hipError_t e = hipFuncSetAttribute((const void *)k_bluestein_prep,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bluestein_prep, dim3(nh), dim3(kRingThreads), lds, s, rt, hlist, choff,
-                     vhoff, plist);
+  hipLaunchKernelGGL(k_bluestein_prep, dim3(nh), dim3(NT), lds, s, rt, hlist, choff, vhoff,
+                     plist);
   return hipGetLastError();
+}
+
+hipError_t ring_set_lds_limit(int slots) {
+  return hipFuncSetAttribute((const void *)k_ring_pairs,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)((size_t)slots * sizeof(double2)));
 }
 
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
                              hipStream_t s) {
-  const size_t lds = (size_t)rt.max_lds_complex * sizeof(double2);
-  hipError_t e = hipFuncSetAttribute((const void *)k_ring_synth,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ring_synth, dim3(rt.nrings), dim3(kRingThreads), lds, s, t, rt, map);
-  return hipGetLastError();
+  for (int i = 0; i < rt.nlaunch; ++i) {
+    if (rt.count[i] == 0) continue;
+    const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
+    hipLaunchKernelGGL(k_ring_pairs, dim3(rt.count[i]), dim3(NT), lds, s, t, rt, rt.first[i],
+                       rt.lds[i], map);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace wm

@@ -44,27 +44,34 @@ struct LegTables {
   double2 *qbuf = nullptr;    // [R][2][M+1][K] even/odd Legendre sums
 };
 
-// One ring of the output map, for the ring (phase + alias + c2r) stage.
-struct RingInfo {
-  int64_t pix_off;   // first pixel of the ring
+// One ring pair of the output map (northern ring + its southern mirror),
+// for the ring (phase + alias + c2r) stage.
+struct RingPair {
+  int64_t pix_n;     // first pixel of the northern ring
+  int64_t pix_s;     // first pixel of the southern ring, -1 for the equator
   int64_t chirp_off; // Bluestein chirp table offset (h entries)
   int64_t vhat_off;  // Bluestein kernel spectrum offset (P entries)
-  int n;             // pixels in the ring
-  int r;             // ring pair index, 0 = equator
-  int sgn;           // +1 north (even + odd), -1 south (even - odd)
-  int ph_num;        // phi0 = pi * ph_num / ph_den
-  int ph_den;
+  int n;             // pixels per ring
+  int s;             // phi0 = pi * s / n, s in {0, 1}
   int P;             // Bluestein FFT length, 0 when n/2 is a power of two
+  int r;             // ring pair index, 0 = equator
 };
 
+constexpr int kMaxLaunches = 8;
+
 struct RingTables {
-  int nrings = 0;
-  RingInfo *info = nullptr;   // [nrings]
-  int *order = nullptr;       // [nrings] launch order (costliest first)
+  int npairs = 0;
+  RingPair *pairs = nullptr;  // [npairs]
+  int *order = nullptr;       // [npairs] pairs grouped by launch, costliest first
   double2 *twiddle = nullptr; // [kMaxFFT] e^{-2 pi i t / kMaxFFT}
   double2 *chirp = nullptr;
   double2 *vhat = nullptr;
-  int max_lds_complex = 0;    // LDS complex slots needed by the ring kernel
+  // launch groups (host side): order[first[i] .. first[i]+count[i]) with
+  // lds[i] complex slots of dynamic LDS
+  int nlaunch = 0;
+  int first[kMaxLaunches] = {0};
+  int count[kMaxLaunches] = {0};
+  int lds[kMaxLaunches] = {0};
 };
 
 // Shape of the Legendre sweep for nmaps = K.
@@ -79,6 +86,8 @@ hipError_t launch_legendre(const LegTables &t, hipStream_t s);
 hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const int64_t *choff,
                                  const int64_t *vhoff, const int *plist, int nh,
                                  hipStream_t s);
+int ring_lds_slots(int n, int P, int M);
+hipError_t ring_set_lds_limit(int slots);
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
                              hipStream_t s);
 
