@@ -156,6 +156,7 @@ def main():
             "flop_convention": "(5 + 4*nmaps) per evaluated (l, m, ring-pair) triple",
             "triples_eps": T_eps,
             "triples_dense": T_dense,
+            "triples_executed": info["triples_executed"],
             "achieved_dense_equiv": (5 + 4 * K) * T_dense / leg_s / 1e12,
             "kernel_ms": stage_ms[1],
         },

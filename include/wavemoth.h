@@ -110,7 +110,8 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
 /* Plan facts: 0 Nside, 1 lmax, 2 mmax, 3 nmaps, 4 npix, 5 dense triples T
    (sum over m of (lmax-m+1) * 2 Nside), 6 trimmed triples T_eps evaluated by
    the plan, 7 flags, 8 legendre-kernel launches per execute, 9 device bytes
-   held by the plan.  Returns -1 for an unknown key. */
+   held by the plan, 10 (l, ring-pair) slots the Legendre sweep evaluates
+   including its wave/chunk padding.  Returns -1 for an unknown key. */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 
 #ifdef __cplusplus

@@ -50,7 +50,7 @@ struct _wavemoth_plan {
   double *d_map = nullptr;
   bool phase_shifts = true;
   bool fly = false;
-  int64_t T_dense = 0, T_eps = 0;
+  int64_t T_dense = 0, T_eps = 0, T_exec = 0;
   std::vector<int64_t> elems;  // per (m, odd): (l, ring) elements evaluated
   std::vector<void *> allocs;
   size_t dev_bytes = 0;
@@ -196,6 +196,21 @@ static bool build_legendre_tables(wavemoth_plan p) {
   std::vector<int> sl(nseed);
   WM_TRY(hipMemcpy(sl.data(), t.seed_l, nseed * sizeof(int), hipMemcpyDeviceToHost),
          "seed_l download");
+  // (l, ring) pairs the sweep actually evaluates: every wave runs its
+  // 64*RL ring slots from its first seed to the end of its last l-chunk.
+  {
+    std::vector<int> ws((size_t)(M + 1) * t.W);
+    WM_TRY(hipMemcpy(ws.data(), t.wstart, ws.size() * sizeof(int), hipMemcpyDeviceToHost),
+           "wstart download");
+    p->T_exec = 0;
+    for (int m = 0; m <= M; ++m) {
+      const int64_t lend = m + (L - m + kChunk) / kChunk * kChunk;  // exclusive
+      for (int w = 0; w < t.W; ++w) {
+        const int s0 = ws[(size_t)m * t.W + w];
+        if (s0 <= L) p->T_exec += (lend - s0) * 64 * t.RL;
+      }
+    }
+  }
   p->elems.assign((size_t)(M + 1) * 2, 0);
   p->T_dense = 0;
   p->T_eps = 0;
@@ -662,6 +677,7 @@ int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
     case 7: return p->flags;
     case 8: return 1;
     case 9: return (int64_t)p->dev_bytes;
+    case 10: return p->T_exec;
     default: return -1;
   }
 }

@@ -113,7 +113,7 @@ class ShtPlan:
 
     def info(self):
         keys = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
-                "flags", "legendre_launches", "device_bytes"]
+                "flags", "legendre_launches", "device_bytes", "triples_executed"]
         return {k: int(lib().wavemoth_plan_info(self._plan, i)) for i, k in enumerate(keys)}
 
 
