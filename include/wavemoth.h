@@ -98,6 +98,19 @@ int wavemoth_get_work(wavemoth_plan plan, double *out);
 int wavemoth_execute_device(wavemoth_plan plan, const double *d_input, double *d_output,
                             void *stream);
 
+/* Adjoint of the synthesis, b = Y^T g (the unweighted analysis):
+   b_lm = c_m sum_p g_p Lambda_l^m(theta_p) e^{-i m phi_p}, c_0 = 1, c_m = 2.
+   It is the gradient of sum_p g_p f_p with respect to (Re a, Im a) and the
+   backward of the synthesis.  The direction of the reference's only GPU
+   kernel (wavemoth/cuda/legendre_transform.cu.in:589-679, Lambda^T q).
+   _device: d_map (npix, nmaps) float64 and d_alm (nalm, nmaps) complex128
+   on the device, asynchronous on `stream`; returns 0 on success.
+   wavemoth_execute_adjoint: synchronous, reads the plan's host *output*
+   array and overwrites its host *input* array. */
+int wavemoth_execute_adjoint_device(wavemoth_plan plan, const double *d_map, double *d_alm,
+                                    void *stream);
+void wavemoth_execute_adjoint(wavemoth_plan plan);
+
 /* Stage timing: with profiling on, every execute records HIP events on its
    stream around (0) a_lm prescale (plus the seed kernels with
    WAVEMOTH_SEEDS_ON_THE_FLY), (1) the Legendre sweep, (2) the ring stage.

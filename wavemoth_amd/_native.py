@@ -28,6 +28,8 @@ SIGNATURES = {
     "wavemoth_last_error": (_cp, []),
     "wavemoth_get_work": (_c_int, [_vp, _vp]),
     "wavemoth_execute_device": (_c_int, [_vp, _vp, _vp, _vp]),
+    "wavemoth_execute_adjoint_device": (_c_int, [_vp, _vp, _vp, _vp]),
+    "wavemoth_execute_adjoint": (None, [_vp]),
     "wavemoth_set_profiling": (_c_int, [_vp, _c_int]),
     "wavemoth_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int)]),
     "wavemoth_plan_info": (_c_i64, [_vp, _c_int]),

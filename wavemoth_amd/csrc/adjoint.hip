@@ -1,0 +1,299 @@
+// Adjoint of the synthesis (map -> a_lm): b = Y^T g.
+//
+// The synthesis evaluates, exactly at every pixel (the aliased c2r of the
+// ring stage is an exact trigonometric-polynomial evaluation),
+//   f(theta_p, phi_p) = sum_{l, m} c_m Re(a_lm Lambda_l^m(theta_p) e^{i m phi_p}),
+// c_0 = 1, c_m = 2 (m > 0).  With the real inner products <f, g> = sum f g and
+// <a, b> = sum Re(conj(a) b), its adjoint is
+//   b_lm = c_m sum_p g_p Lambda_l^m(theta_p) e^{-i m phi_p},
+// the unweighted analysis.  b is also the gradient d<f, g>/dRe(a) +
+// i d<f, g>/dIm(a) (torch's convention for complex leaves).  The reference's
+// only GPU code computes the transposed Legendre stage of this direction
+// (wavemoth/cuda/legendre_transform.cu.in:297-679).
+//
+// Stages:
+//  1. k_ring_adjoint: per ring pair, F_m(ring) = e^{-i m phi0} G^[m mod n]
+//     with G^ the forward real DFT of the ring (conjugate-symmetric for
+//     bins past n/2), then q_even = F_north + F_south, q_odd = F_north -
+//     F_south (the equator ring feeds both), into the [R][2][M+1][K] buffer.
+//  2. k_legendre_adj: b~_l = sum_rings L~_l(x_r) q_{parity(l-m)}(r) with the
+//     same monic recurrence and seeds as the synthesis; lanes evaluate rings,
+//     a 64 x 64 LDS tile transposes them so lanes then own l.
+//  3. k_adj_finish: b_lm = c_m A_l sum_groups b~ into the m-major layout.
+#include "wm_fft.h"
+
+namespace wm {
+
+using namespace fft;
+
+__global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt, int first,
+                                                     int lds_slots,
+                                                     const double *__restrict__ map) {
+  extern __shared__ double2 lds[];
+  double2 *twl = lds + lds_slots - kTwSlots;
+  const RingPair rp = rt.pairs[rt.order[first + blockIdx.x]];
+  const int n = rp.n, h = n / 2, K = t.K, M = t.M, P = rp.P;
+  const bool south = rp.pix_s >= 0;
+  double2 *qe = t.qbuf + ((int64_t)rp.r * 2 + 0) * (M + 1) * K;
+  double2 *qo = t.qbuf + ((int64_t)rp.r * 2 + 1) * (M + 1) * K;
+  const double2 *chirp = rt.chirp + rp.chirp_off;
+  const double2 *vhat = rt.vhat + rp.vhat_off;
+  load_twiddles(twl, rt.twiddle);
+  const int nhemi = south ? 2 : 1;
+  for (int k = 0; k < K; ++k) {
+    for (int hemi = 0; hemi < nhemi; ++hemi) {
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      const double *g = map + (hemi ? rp.pix_s : rp.pix_n) * K;
+      // z_p = g_{2p} + i g_{2p+1}; Z = DFT_h(z), computed as conj(IDFT(conj z))
+      // for Bluestein lengths
+#pragma unroll
+      for (int i = 0; i < kHalf; ++i) {
+        const int p = tid + NT * i;
+        if (p < h) {
+          const double2 z = make_double2(g[(int64_t)(2 * p) * K + k], g[(int64_t)(2 * p + 1) * K + k]);
+          lds[pidx(p)] = (P == 0) ? z : cmul(cconj(z), chirp[p]);
+        }
+      }
+      if (P == 0) {
+        __syncthreads();
+        lds_fft<false>(lds, h, twl);
+      } else {
+#pragma unroll
+        for (int i = 0; i < kFull; ++i) {
+          const int a = h + tid + NT * i;
+          if (a < P) lds[pidx(a)] = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        lds_fft<false>(lds, P, twl);
+#pragma unroll
+        for (int i = 0; i < kFull; ++i) {
+          const int a = tid + NT * i;
+          if (a < P) lds[pidx(a)] = cmul(lds[pidx(a)], vhat[a]);
+        }
+        __syncthreads();
+        lds_fft<true>(lds, P, twl);
+#pragma unroll
+        for (int i = 0; i < kHalf; ++i) {
+          const int p = tid + NT * i;
+          if (p < h) lds[pidx(p)] = cconj(cmul(lds[pidx(p)], chirp[p]));
+        }
+        __syncthreads();
+      }
+      // real-DFT split: G^_k = (Z_k + conj Z_{h-k})/2 - (i/2) e^{-2 pi i k/n}
+      // (Z_k - conj Z_{h-k}), k in [0, h], Z_h = Z_0.  Each thread reads and
+      // rewrites only its own pair (a, h - a), so the split runs in place.
+      {
+        const double2 step = expipi(-2 * NT, n);
+        double2 om = expipi(-2 * (int64_t)tid, n);
+#pragma unroll
+        for (int i = 0; i < kHalf; ++i) {
+          const int a = tid + NT * i;
+          if (a <= h / 2) {
+            const int c = h - a;
+            const double2 za = lds[pidx(a)], zc = lds[pidx(c % h)];
+            const double2 e = cadd(za, cconj(zc)), d = csub(za, cconj(zc));
+            // -(i/2) om d
+            const double2 od = cmul(om, d);
+            const double2 ga = make_double2(0.5 * (e.x + od.y), 0.5 * (e.y - od.x));
+            // partner: e' = conj(e), d' = -conj(d), om' = e^{-2 pi i c/n} = -conj(om)
+            const double2 oc = make_double2(-om.x, om.y);
+            const double2 dc = make_double2(-d.x, d.y);
+            const double2 odc = cmul(oc, dc);
+            const double2 gc = make_double2(0.5 * (e.x + odc.y), 0.5 * (-e.y - odc.x));
+            lds[pidx(a)] = ga;
+            lds[pidx(c)] = gc;
+          }
+          om = cmul(om, step);
+        }
+      }
+      __syncthreads();
+      // F_m = e^{-i pi s m / n} G^[m mod n] (conjugate past n/2), into q
+      {
+        const double2 step = expipi(-(int64_t)rp.s * NT, n);
+        double2 w = expipi(-(int64_t)rp.s * tid, n);
+        for (int m0 = 0; m0 <= M; m0 += NT * kHalf) {
+#pragma unroll
+          for (int i = 0; i < kHalf; ++i) {
+            const int m = m0 + tid + NT * i;
+            if (m <= M) {
+              if (i == 0 && m0 > 0) w = expipi(-(int64_t)rp.s * m, n);
+              const int kk = m % n;
+              const double2 v = (kk <= h) ? lds[pidx(kk)] : cconj(lds[pidx(n - kk)]);
+              const double2 f = cmul(v, w);
+              const int64_t o = (int64_t)m * K + k;
+              if (hemi == 0) {
+                qe[o] = f;
+                qo[o] = f;
+              } else {
+                qe[o] = cadd(qe[o], f);
+                qo[o] = csub(qo[o], f);
+              }
+            }
+            w = cmul(w, step);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// b~ partial sums.  One 64-lane block per (m, ring group g); the group's
+// ring chunks c = g, g + G, ... (64 rings each) run one after the other,
+// and for each the l-range runs in 64-l windows: lanes evaluate their ring
+// (monic recurrence, 16-l chunk rescale), a padded 64 x 65 LDS tile
+// transposes, lanes then own l and sum over the 64 rings, accumulating into
+// partial[g][off_m + l - m][k] (this block's row only).
+__global__ __launch_bounds__(64) void k_legendre_adj(LegTables t, int G, double2 *partial) {
+  __shared__ double tile[64 * 65];
+  __shared__ double2 qs[2 * 64 * 8];
+  const int lane = threadIdx.x;
+  const int m = blockIdx.x / G, g = blockIdx.x - (blockIdx.x / G) * G;
+  const int L = t.L, R = t.R, M = t.M, K = t.K;
+  const int64_t offm = t.off[m];
+  const int len = L - m + 1;
+  const int lp = (len + kChunk - 1) / kChunk * kChunk;
+  const double *ct = t.ctil + (offm - m);
+  const double *cs = t.cscale + offm / kChunk;
+  double2 *prow = partial + ((int64_t)g * t.nlpad + offm) * K;
+  for (int j = lane; j < lp * K; j += 64) prow[j] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int nchunks = (R + 63) / 64;
+  for (int c = g; c < nchunks; c += G) {
+    const int r = c * 64 + lane;
+    const bool ok = r < R;
+    const double x = ok ? t.ring_x[r] : 0.0;
+    const int sl = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
+    int lmin = sl;
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int o = __shfl_xor(lmin, d);
+      lmin = o < lmin ? o : lmin;
+    }
+    lmin = __builtin_amdgcn_readfirstlane(lmin);
+    if (lmin > L) continue;
+    double prev = 0.0, cur = 0.0;
+    // windows of 64 l on the grid l = m + 64 w; seeds sit on the 16-grid
+    const int w0 = (lmin - m) / 64;
+    for (int wl = m + 64 * w0; wl <= L; wl += 64) {
+      for (int kb = 0; kb < K; kb += 8) {
+        const int kn = (K - kb) < 8 ? (K - kb) : 8;
+        // stage q for this window's chunk of rings: qs[par][ring][kk]
+        for (int j = lane; j < 2 * 64 * kn; j += 64) {
+          const int par = j / (64 * kn), rem = j - par * 64 * kn;
+          const int rr = rem / kn, kk = rem - rr * kn;
+          const int rg = c * 64 + rr;
+          qs[(par * 64 + rr) * 8 + kk] =
+              (rg < R) ? t.qbuf[(((int64_t)rg * 2 + par) * (M + 1) + m) * K + kb + kk]
+                       : make_double2(0.0, 0.0);
+        }
+        // lanes = rings: L~ for the window (only on the first map block)
+        if (kb == 0) {
+          for (int sub = 0; sub < 4; ++sub) {
+            const int l0 = wl + 16 * sub;
+            if (l0 > L) {
+              for (int j = 0; j < 16; ++j) tile[(16 * sub + j) * 65 + lane] = 0.0;
+              continue;
+            }
+            if (sl == l0) {
+              const double2 v = t.seed_v[(int64_t)m * R + r];
+              prev = v.x;
+              cur = v.y;
+            }
+            for (int j = 0; j < kChunk; ++j) {
+              tile[(16 * sub + j) * 65 + lane] = cur;
+              const double nx = fma(x, cur, -(ct[l0 + j] * prev));
+              prev = cur;
+              cur = nx;
+            }
+            const double S = cs[(l0 - m) / kChunk];
+            prev *= S;
+            cur *= S;
+          }
+        }
+        __syncthreads();
+        // lanes = l: sum over the 64 rings
+        const int l = wl + lane;
+        if (l <= L && l < m + lp) {
+          const int par = (l - m) & 1;
+          double2 acc[8];
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) acc[kk] = make_double2(0.0, 0.0);
+          for (int rr = 0; rr < 64; ++rr) {
+            const double v = tile[lane * 65 + rr];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+              if (kk < kn) {
+                const double2 q = qs[(par * 64 + rr) * 8 + kk];
+                acc[kk].x = fma(v, q.x, acc[kk].x);
+                acc[kk].y = fma(v, q.y, acc[kk].y);
+              }
+            }
+          }
+          double2 *dst = prow + (int64_t)(l - m) * K + kb;
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk)
+            if (kk < kn) dst[kk] = cadd(dst[kk], acc[kk]);
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// b_lm = c_m A_l sum_g partial[g], into the m-major complex layout.
+__global__ __launch_bounds__(256) void k_adj_finish(LegTables t, int G,
+                                                    const double2 *__restrict__ partial,
+                                                    double2 *__restrict__ alm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= t.nlpad) return;
+  const int m = t.chunk_m[j / kChunk];
+  const int64_t jj = j - t.off[m];
+  const int l = m + (int)jj;
+  if (l > t.L) return;
+  const int K = t.K;
+  const double s = t.ascale[j] * (m == 0 ? 1.0 : 2.0);
+  const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + jj;
+  for (int k = 0; k < K; ++k) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int g = 0; g < G; ++g) acc = cadd(acc, partial[((int64_t)g * t.nlpad + j) * K + k]);
+    acc.x *= s;
+    acc.y = (m == 0) ? 0.0 : acc.y * s;
+    alm[idx * K + k] = acc;
+  }
+}
+
+int adjoint_groups(const LegTables &t) {
+  // enough (m, group) blocks to fill the GPU, at most one group per 64-ring chunk
+  const int nchunks = (t.R + 63) / 64;
+  int G = 1;
+  while (G < nchunks && (int64_t)(t.M + 1) * G < 4096) G *= 2;
+  return G < nchunks ? G : nchunks;
+}
+
+hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double *map,
+                          double2 *alm, double2 *partial, int G, hipStream_t s) {
+  for (int i = 0; i < rt.nlaunch; ++i) {
+    if (rt.count[i] == 0) continue;
+    const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
+    hipLaunchKernelGGL(k_ring_adjoint, dim3(rt.count[i]), dim3(NT), lds, s, t, rt, rt.first[i],
+                       rt.lds[i], map);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_legendre_adj, dim3((unsigned)((t.M + 1) * G)), dim3(64), 0, s, t, G,
+                     partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_adj_finish, dim3((unsigned)((t.nlpad + 255) / 256)), dim3(256), 0, s, t,
+                     G, partial, alm);
+  return hipGetLastError();
+}
+
+hipError_t adjoint_set_lds_limit(int slots) {
+  return hipFuncSetAttribute((const void *)k_ring_adjoint,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)((size_t)slots * sizeof(double2)));
+}
+
+}  // namespace wm

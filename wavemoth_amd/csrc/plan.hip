@@ -54,6 +54,9 @@ struct _wavemoth_plan {
   std::vector<int64_t> elems;  // per (m, odd): (l, ring) elements evaluated
   std::vector<void *> allocs;
   size_t dev_bytes = 0;
+  // adjoint (extension): partial sums over ring groups, allocated on first use
+  double2 *adj_partial = nullptr;
+  int adj_groups = 0;
   // stage timing (extension): events recorded around prep / sweep / rings
   bool profiling = false;
   std::vector<hipEvent_t> ev_free;
@@ -336,6 +339,7 @@ static bool build_ring_tables(wavemoth_plan p) {
     max_need = std::max(max_need, need[order[i]]);
   }
   WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");
+  WM_TRY(adjoint_set_lds_limit(max_need), "ring adjoint kernel LDS limit");
   std::vector<double2> tw(kMaxFFT);
   for (int j = 0; j < kMaxFFT; ++j) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * j / kMaxFFT;
@@ -626,6 +630,51 @@ int wavemoth_execute_device(wavemoth_plan p, const double *d_input, double *d_ou
   if (!run_legendre(p, (const double2 *)d_input, s)) return -1;
   if (!run_rings(p, d_output, s)) return -1;
   return 0;
+}
+
+static bool run_adjoint(wavemoth_plan p, const double *d_map, double2 *d_alm, hipStream_t s) {
+  if (!p->adj_partial) {
+    p->adj_groups = adjoint_groups(p->lt);
+    if (!dalloc(p, &p->adj_partial, (size_t)p->adj_groups * p->lt.nlpad * p->K,
+                "adjoint partial sums"))
+      return false;
+  }
+  WM_TRY(launch_adjoint(p->lt, p->rt, d_map, d_alm, p->adj_partial, p->adj_groups, s),
+         "adjoint kernels");
+  return true;
+}
+
+int wavemoth_execute_adjoint_device(wavemoth_plan p, const double *d_map, double *d_alm,
+                                    void *stream) {
+  g_err.clear();
+  if (!p || !d_map || !d_alm) {
+    set_err("NULL argument");
+    return -1;
+  }
+  if (((uintptr_t)d_map | (uintptr_t)d_alm) & 15) {
+    set_err("device buffers must be 16-byte aligned");
+    return -1;
+  }
+  return run_adjoint(p, d_map, (double2 *)d_alm, (hipStream_t)stream) ? 0 : -1;
+}
+
+void wavemoth_execute_adjoint(wavemoth_plan p) {
+  g_err.clear();
+  if (!p || !p->h_in || !p->h_out) {
+    set_err("plan has no host buffers");
+    return;
+  }
+  hipStream_t s = p->stream;
+  if (hipMemcpyAsync(p->d_map, p->h_out, (size_t)p->npix * p->K * sizeof(double),
+                     hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_err("map upload failed");
+    return;
+  }
+  if (!run_adjoint(p, p->d_map, p->d_alm, s)) return;
+  if (hipMemcpyAsync(p->h_in, p->d_alm, (size_t)p->nalm * p->K * sizeof(double2),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    set_err("adjoint download failed");
 }
 
 int wavemoth_set_profiling(wavemoth_plan p, int on) {

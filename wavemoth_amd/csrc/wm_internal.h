@@ -88,6 +88,10 @@ hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const i
                                  hipStream_t s);
 int ring_lds_slots(int n, int P, int M);
 hipError_t ring_set_lds_limit(int slots);
+int adjoint_groups(const LegTables &t);
+hipError_t adjoint_set_lds_limit(int slots);
+hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double *map,
+                          double2 *alm, double2 *partial, int G, hipStream_t s);
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
                              hipStream_t s);
 

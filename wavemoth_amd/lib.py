@@ -100,6 +100,13 @@ class ShtPlan:
             self._check()
 
     # ---- extensions
+    def execute_adjoint(self):
+        """input <- Y^T output (the transpose of execute, in place on the
+        plan's arrays); returns input."""
+        lib().wavemoth_execute_adjoint(self._plan)
+        self._check()
+        return self.input
+
     def execute_device(self, d_input, d_output, stream=None):
         """Asynchronous execute on device pointers (ints)."""
         rc = lib().wavemoth_execute_device(self._plan, ctypes.c_void_p(int(d_input)),

@@ -39,6 +39,17 @@ class DevicePlan:
             raise RuntimeError(last_error())
         return out
 
+    def adjoint_into(self, gmap, out, stream=None):
+        """out (nalm, nmaps) complex128 <- Y^T gmap (npix, nmaps) float64."""
+        if stream is None:
+            stream = torch.cuda.current_stream(gmap.device).cuda_stream
+        rc = lib().wavemoth_execute_adjoint_device(self._plan, ctypes.c_void_p(gmap.data_ptr()),
+                                                   ctypes.c_void_p(out.data_ptr()),
+                                                   ctypes.c_void_p(int(stream)))
+        if rc != 0:
+            raise RuntimeError(last_error())
+        return out
+
     def set_profiling(self, on):
         if lib().wavemoth_set_profiling(self._plan, int(bool(on))) != 0:
             raise RuntimeError(last_error())
@@ -69,18 +80,54 @@ def get_plan(Nside, lmax, mmax, nmaps, device, flags=0):
     return p
 
 
-def synthesis(alm, Nside, lmax, mmax=None, flags=0):
-    """alm: complex128 cuda tensor (nalm, nmaps) in the m-major layout.
-    Returns the float64 map (12 Nside^2, nmaps) on the same device."""
-    if mmax is None:
-        mmax = lmax
+class _Synthesis(torch.autograd.Function):
+    """Synthesis with the hand-written adjoint as its backward: for a real
+    loss, grad_alm = dL/dRe(a) + i dL/dIm(a) = Y^T (dL/dmap)."""
+
+    @staticmethod
+    def forward(ctx, alm, plan):
+        out = torch.empty((plan.npix, plan.nmaps), dtype=torch.float64, device=alm.device)
+        with torch.cuda.device(alm.device):
+            plan.synthesize_into(alm, out)
+        ctx.plan = plan
+        return out
+
+    @staticmethod
+    def backward(ctx, gmap):
+        plan = ctx.plan
+        gmap = gmap.contiguous().to(torch.float64)
+        out = torch.empty((plan.nalm, plan.nmaps), dtype=torch.complex128, device=gmap.device)
+        with torch.cuda.device(gmap.device):
+            plan.adjoint_into(gmap, out)
+        return out, None
+
+
+def _check_alm(alm, lmax, mmax):
     if not (alm.is_cuda and alm.dtype == torch.complex128 and alm.dim() == 2):
         raise ValueError("alm must be a 2-D complex128 CUDA tensor")
     if alm.shape[0] != nalm(lmax, mmax):
         raise ValueError("alm has %d rows, expected %d" % (alm.shape[0], nalm(lmax, mmax)))
-    alm = alm.contiguous()
+
+
+def synthesis(alm, Nside, lmax, mmax=None, flags=0):
+    """alm: complex128 cuda tensor (nalm, nmaps) in the m-major layout.
+    Returns the float64 map (12 Nside^2, nmaps) on the same device;
+    differentiable with respect to alm (backward = the adjoint kernels)."""
+    if mmax is None:
+        mmax = lmax
+    _check_alm(alm, lmax, mmax)
     p = get_plan(Nside, lmax, mmax, alm.shape[1], alm.device, flags)
-    out = torch.empty((p.npix, alm.shape[1]), dtype=torch.float64, device=alm.device)
-    with torch.cuda.device(alm.device):
-        p.synthesize_into(alm, out)
+    return _Synthesis.apply(alm.contiguous(), p)
+
+
+def adjoint(gmap, Nside, lmax, mmax=None, flags=0):
+    """Y^T gmap: gmap float64 cuda (12 Nside^2, nmaps) -> complex128 (nalm, nmaps)."""
+    if mmax is None:
+        mmax = lmax
+    if not (gmap.is_cuda and gmap.dtype == torch.float64 and gmap.dim() == 2):
+        raise ValueError("gmap must be a 2-D float64 CUDA tensor")
+    p = get_plan(Nside, lmax, mmax, gmap.shape[1], gmap.device, flags)
+    out = torch.empty((p.nalm, gmap.shape[1]), dtype=torch.complex128, device=gmap.device)
+    with torch.cuda.device(gmap.device):
+        p.adjoint_into(gmap.contiguous(), out)
     return out
