@@ -97,17 +97,28 @@ def test_autograd_finite_differences_config4():
     part = rng.integers(0, 2, 64)  # 0: real part, 1: imaginary part
     h = 1e-6
     worst = 0.0
+    gscale = float(np.abs(grad).max())
+    errs = []
     with torch.no_grad():
         for i in range(64):
             e = torch.zeros_like(a0)
             e[idx[i], kk[i]] = h if part[i] == 0 else 1j * h
-            fd = (loss(a0 + e) - loss(a0 - e)).item() / (2 * h)
+            rp = synthesis(a0 + e, N, L) - d
+            rm = synthesis(a0 - e, N, L) - d
+            # central difference [L(a+e) - L(a-e)] / 2h, with the difference of
+            # the two ~1e7 losses formed pixel by pixel as
+            # 1/2 w (r+ - r-)(r+ + r-) so it does not cancel catastrophically
+            fd = torch.sum(0.5 * wt * (rp - rm) * (rp + rm)).item() / (2 * h)
             an = grad[idx[i], kk[i]].real if part[i] == 0 else grad[idx[i], kk[i]].imag
-            m_is0 = idx[i] <= L  # m = 0 rows: the imaginary part has no effect
-            if part[i] == 1 and m_is0:
-                assert abs(an) == 0.0 and abs(fd) < 1e-3
+            if part[i] == 1 and idx[i] <= L:
+                # m = 0 rows: the synthesis ignores Im(a_l0) (c2r drops Im X_0)
+                assert an == 0.0 and fd == 0.0
                 continue
-            scale = max(abs(fd), 1e-2)
-            worst = max(worst, abs(fd - an) / scale)
-    print("worst relative FD mismatch", worst)
-    assert worst < 1e-4
+            # relative error, floored at 1e-6 of the gradient's scale: the
+            # FD sum over 9.4M pixels carries ~1e-12 of rounding per step
+            err = abs(fd - an) / (abs(an) + 1e-6 * gscale)
+            errs.append(err)
+            worst = max(worst, err)
+    print("FD relative mismatch: median %.3e worst %.3e (grad scale %.3e, loss %.6e)"
+          % (float(np.median(errs)), worst, gscale, L0.item()))
+    assert worst < 1e-5
