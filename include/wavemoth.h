@@ -127,6 +127,52 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
    including its wave/chunk padding.  Returns -1 for an unknown key. */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 
+/* ---- batched synthesis (extension) -------------------------------------
+   Many independent syntheses of uneven (Nside, lmax, mmax, nmaps) in one
+   execute.  The reference has no batch call; its nearest equivalent is one
+   wavemoth_plan_to_healpix + wavemoth_execute per request, and every
+   request's map equals (bit for bit) what a single plan of the same
+   request produces here.  The prescale, Legendre sweep and ring stage each
+   run as one persistent work-queue kernel over all requests' work items,
+   costliest request first; requests of equal (Nside, lmax, mmax) share
+   their seed and ring tables.  The three launches are captured once into a
+   HIP graph (unless WAVEMOTH_BATCH_NO_GRAPH) and replayed by every execute.
+   With WAVEMOTH_MEASURE the sweep shape of each distinct (Nside, lmax,
+   mmax, nmaps) -- rings per lane and maps per pass -- is chosen by timing
+   the candidates on the device at plan time (results are kept for the
+   process); otherwise a fixed heuristic chooses.  Executes of one batch
+   must not overlap. */
+typedef struct _wavemoth_batch *wavemoth_batch;
+#define WAVEMOTH_BATCH_NO_GRAPH 0x200
+
+/* Returns NULL on error (wavemoth_last_error). */
+wavemoth_batch wavemoth_plan_batch(int nreq, const int *Nside, const int *lmax,
+                                   const int *mmax, const int *nmaps, unsigned flags);
+void wavemoth_destroy_batch(wavemoth_batch batch);
+/* d_inputs[r] / d_outputs[r]: device buffers of request r in the layouts
+   of wavemoth_execute_device, 16-byte aligned; asynchronous on `stream`
+   (a hipStream_t, NULL = default).  Returns 0 on success. */
+int wavemoth_execute_batch_device(wavemoth_batch batch, const double *const *d_inputs,
+                                  double *const *d_outputs, void *stream);
+/* Synchronous host-buffer variant (stages through batch-owned device
+   buffers allocated on first use). */
+int wavemoth_execute_batch(wavemoth_batch batch, const double *const *inputs,
+                           double *const *outputs);
+/* Stage timing as wavemoth_set_profiling / wavemoth_stage_times, for
+   (0) prescale, (1) Legendre sweep, (2) ring stage.  While profiling is on
+   executes run the launches directly instead of replaying the graph. */
+int wavemoth_batch_set_profiling(wavemoth_batch batch, int on);
+int wavemoth_batch_stage_times(wavemoth_batch batch, double *avg_ms, int *count);
+/* req = -1, batch facts: 0 requests, 1 total maps, 2 sum of T_eps, 3 sum of
+   dense T, 4 algorithmic flops per execute (sum of (5 + 4 nmaps) T_eps),
+   5 device bytes held, 6 sweep work items, 7 ring work items, 8 graph in
+   use, 9 sum of executed (l, ring-pair, map-pass) slots, 10 persistent
+   sweep workgroups.  req >= 0, request facts: 0 Nside, 1 lmax, 2 mmax,
+   3 nmaps, 4 rings per lane, 5 maps per pass, 6 T_eps, 7 dense T, 8 npix,
+   9 executed slots, 10 shape chosen by measurement (1) or heuristic (0).
+   Returns -1 for an unknown key or request. */
+int64_t wavemoth_batch_info(wavemoth_batch batch, int req, int key);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,3 +104,32 @@ def test_alm_generator_layout():
     ls, ms = l_of_index(600, 600)
     v = np.mean(np.abs(flat[ms > 0]) ** 2)
     assert 0.97 < v < 1.03
+
+
+@pytest.mark.parametrize("reqs,msg", [([], "nreq"),
+                                      ([(4, 8, 8, 1), (0, 8, 8, 1)], "Nside"),
+                                      ([(4096, 8, 8, 1)], "Nside"),
+                                      ([(4, 8, 9, 1)], "mmax"),
+                                      ([(4, 8, 8, 1), (4, 8, 8, 0)], "nmaps")])
+def test_batch_argument_errors_return_null(reqs, msg):
+    """wavemoth_plan_batch validates every request before touching a device
+    and names the offending request."""
+    from wavemoth_amd import _native
+    L = _native.lib()
+    n = len(reqs)
+    cols = [(ctypes.c_int * max(n, 1))(*[r[i] for r in reqs]) for i in range(4)]
+    assert not L.wavemoth_plan_batch(n, *cols, 0)
+    err = _native.last_error()
+    assert msg in err
+    if n > 1:
+        assert "request %d" % (n - 1) in err
+
+
+def test_batch_python_checks_requests():
+    from wavemoth_amd import random_requests
+    reqs = random_requests(96, seed=3)
+    assert len(reqs) == 96
+    assert {r[0] for r in reqs} <= {128, 256, 512, 1024, 2048}
+    assert all(r[1] == r[2] == 2 * r[0] for r in reqs)
+    assert {r[3] for r in reqs} <= {1, 3, 8}
+    assert reqs == random_requests(96, seed=3)

@@ -33,6 +33,14 @@ SIGNATURES = {
     "wavemoth_set_profiling": (_c_int, [_vp, _c_int]),
     "wavemoth_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int)]),
     "wavemoth_plan_info": (_c_i64, [_vp, _c_int]),
+    "wavemoth_plan_batch": (_vp, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                                  ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_u]),
+    "wavemoth_destroy_batch": (None, [_vp]),
+    "wavemoth_execute_batch_device": (_c_int, [_vp, _vp, _vp, _vp]),
+    "wavemoth_execute_batch": (_c_int, [_vp, _vp, _vp]),
+    "wavemoth_batch_set_profiling": (_c_int, [_vp, _c_int]),
+    "wavemoth_batch_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int)]),
+    "wavemoth_batch_info": (_c_i64, [_vp, _c_int, _c_int]),
 }
 
 WAVEMOTH_MMAJOR = 0x0
@@ -40,6 +48,7 @@ WAVEMOTH_ESTIMATE = 0x0
 WAVEMOTH_MEASURE = 0x1
 WAVEMOTH_NO_RESOURCE_COPY = 0x10
 WAVEMOTH_SEEDS_ON_THE_FLY = 0x100
+WAVEMOTH_BATCH_NO_GRAPH = 0x200
 
 
 def _bind_torch_runtime():

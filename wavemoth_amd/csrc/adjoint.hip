@@ -20,6 +20,8 @@
 //     same monic recurrence and seeds as the synthesis; lanes evaluate rings,
 //     a 64 x 64 LDS tile transposes them so lanes then own l.
 //  3. k_adj_finish: b_lm = c_m A_l sum_groups b~ into the m-major layout.
+#include <mutex>
+
 #include "wm_fft.h"
 
 namespace wm {
@@ -291,9 +293,16 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
 }
 
 hipError_t adjoint_set_lds_limit(int slots) {
-  return hipFuncSetAttribute((const void *)k_ring_adjoint,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)((size_t)slots * sizeof(double2)));
+  // The limit only grows, so plans made earlier keep a valid launch size.
+  static std::mutex mu;
+  static int cur = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (slots <= cur) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute((const void *)k_ring_adjoint,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)((size_t)slots * sizeof(double2)));
+  if (e == hipSuccess) cur = slots;
+  return e;
 }
 
 }  // namespace wm

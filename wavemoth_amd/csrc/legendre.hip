@@ -118,10 +118,8 @@ __global__ __launch_bounds__(256) void k_wave_bounds(LegTables t) {
 // atil[kb][j][k'] = A_l * a_{l m}[kb*KB + k'] on the padded l-grid, zero
 // past lmax and for map slots past nmaps: map-block-major so that a wave
 // of map block kb reads KB consecutive complex values per l.
-__global__ __launch_bounds__(256) void k_prep_alm(LegTables t, const double2 *__restrict__ alm,
-                                                  int KB, int nkb) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= t.nlpad) return;
+__device__ __forceinline__ void prep_item(const LegTables &t, const double2 *__restrict__ alm,
+                                          int64_t j, int KB, int nkb) {
   const int m = t.chunk_m[j / kChunk];
   const int64_t jj = j - t.off[m];
   const int l = m + (int)jj;
@@ -144,35 +142,34 @@ __global__ __launch_bounds__(256) void k_prep_alm(LegTables t, const double2 *__
   }
 }
 
-// ------------------------------------------------------------ main sweep
+__global__ __launch_bounds__(256) void k_prep_alm(LegTables t, const double2 *__restrict__ alm,
+                                                  int KB, int nkb) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= t.nlpad) return;
+  prep_item(t, alm, j, KB, nkb);
+}
 
-// Block = 4 waves.  Block b runs on XCD b % 8; every m's blocks are kept
-// on one XCD (m = xcd + 8 * i) so the per-m coefficient rows are fetched
-// into one L2 only, and each XCD walks its m's from cheap-to-start (small
-// m, long l-range) to short ones.
-// Wave (m, ring chunk w, map block kb): lane handles rings
-// w*64*RL + lane + 64*i, i < RL; KB maps per pass.
+// One wave's share of the sweep: ring chunk w (rings w*64*RL + lane + 64*i,
+// i < RL) of m, maps kb*KB .. kb*KB+KB-1.  Used by the grid kernel below
+// and by the batched work-queue kernel (batch.hip).
+// Tables the sweep only reads are accessed through constant-address-space
+// views: their loads are then known not to be clobbered by the q stores
+// (which matters inside the persistent loop of the batched kernel), and
+// wave-uniform ones go through the scalar unit.
+#define WM_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const WM_CONST T *const_view(const T *p) {
+  return (const WM_CONST T *)p;
+}
+
+// The sweep proper.
 template <int RL, int KB>
-__global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm) {
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int b = blockIdx.x;
-  const int s = b >> 3;
-  const int m = (b & 7) + 8 * (s / bpm);
-  if (m > t.M) return;
-  const int item = (s % bpm) * 4 + wv;
-  if (item >= t.W * nkb) return;
-  const int kb = item / t.W, w = item - kb * t.W;
-  const int L = t.L, R = t.R, M = t.M, K = t.K;
-  const int kbase = kb * KB;
-
-  int l0 = __builtin_amdgcn_readfirstlane(t.wstart[(int64_t)m * t.W + w]);
-  const int lsmax = __builtin_amdgcn_readfirstlane(t.wseedmax[(int64_t)m * t.W + w]);
-  const int64_t offm = t.off[m];
-  const double *__restrict__ ct = t.ctil + (offm - m);
-  const double2 *__restrict__ at = t.atil + ((int64_t)kb * t.nlpad + offm - m) * KB;
-  const double *__restrict__ cs = t.cscale + offm / kChunk;
-
+__device__ __forceinline__ void legendre_sweep(
+    int L, int R, int Mp1, int K, int m, int w, int kbase, int lane, int l0, int lsmax,
+    const WM_CONST double *ct, const WM_CONST double *at,
+    const WM_CONST double *cs, const double *__restrict__ ring_x,
+    const int *__restrict__ seed_l, const double2 *__restrict__ seed_v,
+    double2 *__restrict__ qbuf) {
   double x[RL], prev[RL], cur[RL];
   int sl[RL], rr[RL];
   double acc[RL][2][KB][2];
@@ -180,8 +177,8 @@ __global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm)
   for (int i = 0; i < RL; ++i) {
     rr[i] = w * 64 * RL + lane + 64 * i;
     const bool ok = rr[i] < R;
-    x[i] = ok ? t.ring_x[rr[i]] : 0.0;
-    sl[i] = ok ? t.seed_l[(int64_t)m * R + rr[i]] : kSeedNone;
+    x[i] = ok ? ring_x[rr[i]] : 0.0;
+    sl[i] = ok ? seed_l[(int64_t)m * R + rr[i]] : kSeedNone;
     prev[i] = 0.0;
     cur[i] = 0.0;
 #pragma unroll
@@ -195,7 +192,7 @@ __global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm)
 #pragma unroll
       for (int i = 0; i < RL; ++i) {
         if (sl[i] == l0) {
-          const double2 v = t.seed_v[(int64_t)m * R + rr[i]];
+          const double2 v = seed_v[(int64_t)m * R + rr[i]];
           prev[i] = v.x;
           cur[i] = v.y;
         }
@@ -208,8 +205,9 @@ __global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm)
       double2 a0[KB], a1[KB];
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
-        a0[k] = at[(int64_t)l * KB + k];
-        a1[k] = at[(int64_t)(l + 1) * KB + k];
+        const int64_t i0 = 2 * ((int64_t)l * KB + k), i1 = i0 + 2 * KB;
+        a0[k] = make_double2(at[i0], at[i0 + 1]);
+        a1[k] = make_double2(at[i1], at[i1 + 1]);
       }
 #pragma unroll
       for (int i = 0; i < RL; ++i) {
@@ -244,13 +242,46 @@ __global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm)
     if (rr[i] < R) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        double2 *dst = t.qbuf + (((int64_t)rr[i] * 2 + p) * (M + 1) + m) * K + kbase;
+        double2 *dst = qbuf + (((int64_t)rr[i] * 2 + p) * Mp1 + m) * K + kbase;
 #pragma unroll
         for (int k = 0; k < KB; ++k)
           if (kbase + k < K) dst[k] = make_double2(acc[i][p][k][0], acc[i][p][k][1]);
       }
     }
   }
+}
+
+// One wave's share of the sweep: ring chunk w (rings w*64*RL + lane + 64*i,
+// i < RL) of m, maps kb*KB .. kb*KB+KB-1.  Used by the grid kernel below
+// and by the batched work-queue kernel.
+template <int RL, int KB>
+__device__ __forceinline__ void legendre_item(const LegTables &t, int m, int w, int kb,
+                                              int lane) {
+  const int l0 = __builtin_amdgcn_readfirstlane(t.wstart[(int64_t)m * t.W + w]);
+  const int lsmax = __builtin_amdgcn_readfirstlane(t.wseedmax[(int64_t)m * t.W + w]);
+  const int64_t offm = t.off[m];
+  legendre_sweep<RL, KB>(
+      t.L, t.R, t.M + 1, t.K, m, w, kb * KB, lane, l0, lsmax, const_view(t.ctil + (offm - m)),
+      const_view((const double *)(t.atil + ((int64_t)kb * t.nlpad + offm - m) * KB)),
+      const_view(t.cscale + offm / kChunk), t.ring_x, t.seed_l, t.seed_v, t.qbuf);
+}
+
+// Block = 4 waves.  Block b runs on XCD b % 8; every m's blocks are kept
+// on one XCD (m = xcd + 8 * i) so the per-m coefficient rows are fetched
+// into one L2 only, and each XCD walks its m's from cheap-to-start (small
+// m, long l-range) to short ones.
+template <int RL, int KB>
+__global__ __launch_bounds__(256) void k_legendre(LegTables t, int nkb, int bpm) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x;
+  const int s = b >> 3;
+  const int m = (b & 7) + 8 * (s / bpm);
+  if (m > t.M) return;
+  const int item = (s % bpm) * 4 + wv;
+  if (item >= t.W * nkb) return;
+  const int kb = item / t.W, w = item - kb * t.W;
+  legendre_item<RL, KB>(t, m, w, kb, lane);
 }
 
 // ------------------------------------------------------------ launchers
@@ -269,26 +300,17 @@ hipError_t launch_wave_bounds(const LegTables &t, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Map-block size and rings per lane chosen from nmaps.
-struct LegShape {
-  int RL, KB;
-};
-static LegShape leg_shape(int K) {
-  if (K == 1) return {2, 1};
-  if (K == 2) return {2, 2};
-  if (K == 3) return {1, 3};
-  if (K == 4) return {1, 4};
-  return {1, 8};
-}
-
-int legendre_rings_per_lane(int K) { return leg_shape(K).RL; }
+// Maps per pass (KB) from nmaps; rings per lane (RL) is a plan choice
+// (heuristic below, or measured by the autotuner).
+int legendre_map_block(int K) { return K <= 4 ? K : 8; }
+int legendre_default_rl(int K) { return K <= 2 ? 2 : 1; }
 int legendre_padded_maps(int K) {
-  const LegShape sh = leg_shape(K);
-  return ((K + sh.KB - 1) / sh.KB) * sh.KB;
+  const int KB = legendre_map_block(K);
+  return ((K + KB - 1) / KB) * KB;
 }
 
 hipError_t launch_prep_alm(const LegTables &t, const double2 *alm, hipStream_t s) {
-  const int KB = leg_shape(t.K).KB;
+  const int KB = t.KB;
   hipLaunchKernelGGL(k_prep_alm, dim3(nblocks(t.nlpad, 256)), dim3(256), 0, s, t, alm, KB,
                      (t.K + KB - 1) / KB);
   return hipGetLastError();
@@ -304,15 +326,112 @@ static hipError_t launch_leg_t(const LegTables &t, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int RL>
+static hipError_t launch_leg_rl(const LegTables &t, hipStream_t s) {
+  switch (t.KB) {
+    case 1: return launch_leg_t<RL, 1>(t, s);
+    case 2: return launch_leg_t<RL, 2>(t, s);
+    case 3: return launch_leg_t<RL, 3>(t, s);
+    case 4: return launch_leg_t<RL, 4>(t, s);
+    default: return launch_leg_t<RL, 8>(t, s);
+  }
+}
+
+// ---------------------------------------------------------------- batch
+
+__global__ void k_batch_reset(int *counters) {
+  if (threadIdx.x < kBatchCounters) counters[threadIdx.x] = 0;
+}
+
+// Prescale of every request's a_lm: thread g owns padded l-entry
+// g - prep_off[r] of request r.
+__global__ __launch_bounds__(256) void k_batch_prep(BatchDev b) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= b.prep_total) return;
+  int lo = 0, hi = b.nreq - 1;  // prep_off[lo] <= g < prep_off[lo + 1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (b.prep_off[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  const LegTables &t = b.tabs[lo];
+  prep_item(t, b.alm[lo], g - b.prep_off[lo], t.KB, (t.K + t.KB - 1) / t.KB);
+}
+
+#define WM_BATCH_SHAPES(X) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 3) X(4, 1)
+
+bool batch_shape_supported(int RL, int KB) {
+#define WM_SUP(R_, K_) if (RL == R_ && KB == K_) return true;
+  WM_BATCH_SHAPES(WM_SUP)
+#undef WM_SUP
+  return false;
+}
+
+// Persistent sweep: each wave takes items (request, m, ring chunk, map
+// block) from one global queue until it runs dry; the host orders the
+// queue costliest request first.
+__global__ __launch_bounds__(256, 6) void k_batch_legendre(BatchDev b) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(&b.counters[0], 1);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= b.nlitems) break;
+    const WM_CONST BatchLegItem *ip = const_view(b.litems + it);
+    const int req = ip->req, m = ip->m, w = ip->w, kb = ip->kb;
+    const WM_CONST LegTables *tp = const_view(b.tabs + req);
+    LegTables t;
+    t.L = tp->L;
+    t.M = tp->M;
+    t.K = tp->K;
+    t.R = tp->R;
+    t.W = tp->W;
+    t.nlpad = tp->nlpad;
+    t.off = tp->off;
+    t.ctil = tp->ctil;
+    t.cscale = tp->cscale;
+    t.ring_x = tp->ring_x;
+    t.seed_l = tp->seed_l;
+    t.seed_v = tp->seed_v;
+    t.wstart = tp->wstart;
+    t.wseedmax = tp->wseedmax;
+    t.atil = tp->atil;
+    t.qbuf = tp->qbuf;
+    switch (tp->RL * 16 + tp->KB) {
+#define WM_CASE(R_, K_) \
+  case 16 * R_ + K_: legendre_item<R_, K_>(t, m, w, kb, lane); break;
+      WM_BATCH_SHAPES(WM_CASE)
+#undef WM_CASE
+      default: break;
+    }
+  }
+}
+
+hipError_t launch_batch_reset(const BatchDev &b, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b.counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_prep(const BatchDev &b, hipStream_t s) {
+  if (b.prep_total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_prep, dim3(nblocks(b.prep_total, 256)), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t batch_legendre_occupancy(int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_legendre, 256, 0);
+}
+
+hipError_t launch_batch_legendre(const BatchDev &b, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_legendre, dim3(nb), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_legendre(const LegTables &t, hipStream_t s) {
-  const LegShape sh = leg_shape(t.K);
-  if (sh.RL != t.RL) return hipErrorInvalidValue;
-  switch (sh.KB) {
-    case 1: return launch_leg_t<2, 1>(t, s);
-    case 2: return launch_leg_t<2, 2>(t, s);
-    case 3: return launch_leg_t<1, 3>(t, s);
-    case 4: return launch_leg_t<1, 4>(t, s);
-    default: return launch_leg_t<1, 8>(t, s);
+  switch (t.RL) {
+    case 1: return launch_leg_rl<1>(t, s);
+    case 2: return launch_leg_rl<2>(t, s);
+    case 4: return launch_leg_rl<4>(t, s);
+    default: return hipErrorInvalidValue;
   }
 }
 

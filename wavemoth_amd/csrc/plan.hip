@@ -19,23 +19,18 @@
 #include <vector>
 
 #include "../../include/wavemoth.h"
-#include "wm_internal.h"
+#include "wm_host.h"
 
 using namespace wm;
 
 static thread_local std::string g_err;
 static std::string g_resource_dir;
 
-static void set_err(const std::string &s) { g_err = s; }
-
-#define WM_TRY(expr, what)                                                        \
-  do {                                                                            \
-    hipError_t e_ = (expr);                                                       \
-    if (e_ != hipSuccess) {                                                       \
-      set_err(std::string(what) + ": " + hipGetErrorString(e_));                  \
-      return false;                                                               \
-    }                                                                             \
-  } while (0)
+namespace wm {
+void set_err(const std::string &s) { g_err = s; }
+void clear_err() { g_err.clear(); }
+const char *last_err() { return g_err.c_str(); }
+}  // namespace wm
 
 struct _wavemoth_plan {
   int N = 0, L = 0, M = 0, K = 0, nthreads = 0;
@@ -46,14 +41,15 @@ struct _wavemoth_plan {
   hipStream_t stream = nullptr;
   LegTables lt;
   RingTables rt;
+  RingHost rhost;
   double2 *d_alm = nullptr;
   double *d_map = nullptr;
   bool phase_shifts = true;
   bool fly = false;
-  int64_t T_dense = 0, T_eps = 0, T_exec = 0;
-  std::vector<int64_t> elems;  // per (m, odd): (l, ring) elements evaluated
-  std::vector<void *> allocs;
-  size_t dev_bytes = 0;
+  int rl_choice = 0;  // rings per lane of the Legendre sweep (0 = heuristic)
+  LegCounts cnt;
+  int64_t T_exec = 0;
+  DevArena arena;
   // adjoint (extension): partial sums over ring groups, allocated on first use
   double2 *adj_partial = nullptr;
   int adj_groups = 0;
@@ -62,30 +58,6 @@ struct _wavemoth_plan {
   std::vector<hipEvent_t> ev_free;
   std::vector<std::array<hipEvent_t, 4>> ev_used;
 };
-
-template <typename T>
-static bool dalloc(wavemoth_plan p, T **ptr, size_t count, const char *what) {
-  void *v = nullptr;
-  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&v, bytes);
-  if (e != hipSuccess) {
-    set_err(std::string("hipMalloc ") + what + " (" + std::to_string(bytes) +
-            " bytes): " + hipGetErrorString(e));
-    return false;
-  }
-  p->allocs.push_back(v);
-  p->dev_bytes += bytes;
-  *ptr = (T *)v;
-  return true;
-}
-
-template <typename T>
-static bool upload(wavemoth_plan p, T **ptr, const std::vector<T> &h, const char *what) {
-  if (!dalloc(p, ptr, h.size(), what)) return false;
-  if (!h.empty())
-    WM_TRY(hipMemcpy(*ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), what);
-  return true;
-}
 
 static void release_events(wavemoth_plan p) {
   for (auto &a : p->ev_used)
@@ -98,7 +70,7 @@ static void free_plan(wavemoth_plan p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   release_events(p);
   for (hipEvent_t e : p->ev_free) (void)hipEventDestroy(e);
-  for (void *v : p->allocs) (void)hipFree(v);
+  p->arena.release();
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -117,16 +89,16 @@ static double ring_z(int N, int r) {
   return 4.0 / 3.0 - 2.0 * (double)i / (3.0 * (double)N);
 }
 
-static bool build_legendre_tables(wavemoth_plan p) {
-  LegTables &t = p->lt;
-  const int N = p->N, L = p->L, M = p->M, K = p->K, R = 2 * N;
+namespace wm {
+
+bool build_leg_geometry(DevArena &a, LegTables &t, int N, int L, int M, hipStream_t stream,
+                        LegCounts *cnt) {
+  const int R = 2 * N;
   t.N = N;
   t.L = L;
   t.M = M;
-  t.K = K;
+  t.K = 0;
   t.R = R;
-  t.RL = legendre_rings_per_lane(K);
-  t.W = (R + 64 * t.RL - 1) / (64 * t.RL);
 
   std::vector<int64_t> off(M + 1);
   int64_t tot = 0;
@@ -175,61 +147,81 @@ static bool build_legendre_tables(wavemoth_plan p) {
     rls[r] = dd(log2l((long double)std::sin(th)));
   }
 
-  if (!upload(p, &t.off, off, "off")) return false;
-  if (!upload(p, &t.chunk_m, chunk_m, "chunk_m")) return false;
-  if (!upload(p, &t.ctil, ctil, "ctil")) return false;
-  if (!upload(p, &t.ascale, ascale, "ascale")) return false;
-  if (!upload(p, &t.cscale, cscale, "cscale")) return false;
-  if (!upload(p, &t.mfac, mfac, "mfac")) return false;
-  if (!upload(p, &t.ring_x, rx, "ring_x")) return false;
-  if (!upload(p, &t.ring_ls, rls, "ring_ls")) return false;
+  if (!upload(a, &t.off, off, "off")) return false;
+  if (!upload(a, &t.chunk_m, chunk_m, "chunk_m")) return false;
+  if (!upload(a, &t.ctil, ctil, "ctil")) return false;
+  if (!upload(a, &t.ascale, ascale, "ascale")) return false;
+  if (!upload(a, &t.cscale, cscale, "cscale")) return false;
+  if (!upload(a, &t.mfac, mfac, "mfac")) return false;
+  if (!upload(a, &t.ring_x, rx, "ring_x")) return false;
+  if (!upload(a, &t.ring_ls, rls, "ring_ls")) return false;
   const size_t nseed = (size_t)(M + 1) * R;
-  if (!dalloc(p, &t.seed_l, nseed, "seed_l")) return false;
-  if (!dalloc(p, &t.seed_v, nseed, "seed_v")) return false;
-  if (!dalloc(p, &t.wstart, (size_t)(M + 1) * t.W, "wstart")) return false;
-  if (!dalloc(p, &t.wseedmax, (size_t)(M + 1) * t.W, "wseedmax")) return false;
-  if (!dalloc(p, &t.atil, (size_t)t.nlpad * legendre_padded_maps(K), "atil")) return false;
-  if (!dalloc(p, &t.qbuf, (size_t)R * 2 * (M + 1) * K, "qbuf")) return false;
-
-  WM_TRY(launch_seeds(t, p->stream), "seed kernel");
-  WM_TRY(launch_wave_bounds(t, p->stream), "wave-bounds kernel");
-  WM_TRY(hipStreamSynchronize(p->stream), "seed stage");
+  if (!dalloc(a, &t.seed_l, nseed, "seed_l")) return false;
+  if (!dalloc(a, &t.seed_v, nseed, "seed_v")) return false;
+  WM_TRY(launch_seeds(t, stream), "seed kernel");
+  WM_TRY(hipStreamSynchronize(stream), "seed stage");
 
   // Work accounting from the seeds: T_eps and per-(m, odd) element counts.
   std::vector<int> sl(nseed);
   WM_TRY(hipMemcpy(sl.data(), t.seed_l, nseed * sizeof(int), hipMemcpyDeviceToHost),
          "seed_l download");
-  // (l, ring) pairs the sweep actually evaluates: every wave runs its
-  // 64*RL ring slots from its first seed to the end of its last l-chunk.
-  {
-    std::vector<int> ws((size_t)(M + 1) * t.W);
-    WM_TRY(hipMemcpy(ws.data(), t.wstart, ws.size() * sizeof(int), hipMemcpyDeviceToHost),
-           "wstart download");
-    p->T_exec = 0;
-    for (int m = 0; m <= M; ++m) {
-      const int64_t lend = m + (L - m + kChunk) / kChunk * kChunk;  // exclusive
-      for (int w = 0; w < t.W; ++w) {
-        const int s0 = ws[(size_t)m * t.W + w];
-        if (s0 <= L) p->T_exec += (lend - s0) * 64 * t.RL;
-      }
-    }
-  }
-  p->elems.assign((size_t)(M + 1) * 2, 0);
-  p->T_dense = 0;
-  p->T_eps = 0;
+  cnt->elems.assign((size_t)(M + 1) * 2, 0);
+  cnt->T_dense = 0;
+  cnt->T_eps = 0;
   for (int m = 0; m <= M; ++m) {
-    p->T_dense += (int64_t)(L - m + 1) * R;
+    cnt->T_dense += (int64_t)(L - m + 1) * R;
     for (int r = 0; r < R; ++r) {
       const int s = sl[(size_t)m * R + r];
       if (s > L) continue;
       const int64_t n = L - s + 1;
-      p->T_eps += n;
+      cnt->T_eps += n;
       // l in [s, L] with l - m even / odd
       const int64_t ne = (L - m) / 2 - (s - m + 1) / 2 + 1;
-      p->elems[2 * m] += std::max<int64_t>(ne, 0);
-      p->elems[2 * m + 1] += n - std::max<int64_t>(ne, 0);
+      cnt->elems[2 * m] += std::max<int64_t>(ne, 0);
+      cnt->elems[2 * m + 1] += n - std::max<int64_t>(ne, 0);
     }
   }
+  return true;
+}
+
+bool build_leg_waves(DevArena &a, LegTables &t, int RL, hipStream_t stream, int64_t *T_exec) {
+  const int L = t.L, M = t.M;
+  t.RL = RL;
+  t.W = (t.R + 64 * RL - 1) / (64 * RL);
+  if (!dalloc(a, &t.wstart, (size_t)(M + 1) * t.W, "wstart")) return false;
+  if (!dalloc(a, &t.wseedmax, (size_t)(M + 1) * t.W, "wseedmax")) return false;
+  WM_TRY(launch_wave_bounds(t, stream), "wave-bounds kernel");
+  // (l, ring) slots the sweep evaluates: every wave runs its 64*RL ring
+  // slots from its first seed to the end of its last l-chunk.
+  std::vector<int> ws((size_t)(M + 1) * t.W);
+  WM_TRY(hipMemcpyAsync(ws.data(), t.wstart, ws.size() * sizeof(int), hipMemcpyDeviceToHost,
+                        stream),
+         "wstart download");
+  WM_TRY(hipStreamSynchronize(stream), "wave bounds");
+  int64_t tx = 0;
+  for (int m = 0; m <= M; ++m) {
+    const int64_t lend = m + (L - m + kChunk) / kChunk * kChunk;  // exclusive
+    for (int w = 0; w < t.W; ++w) {
+      const int s0 = ws[(size_t)m * t.W + w];
+      if (s0 <= L) tx += (lend - s0) * 64 * RL;
+    }
+  }
+  *T_exec = tx;
+  return true;
+}
+
+}  // namespace wm
+
+static bool build_legendre_tables(wavemoth_plan p) {
+  LegTables &t = p->lt;
+  if (!build_leg_geometry(p->arena, t, p->N, p->L, p->M, p->stream, &p->cnt)) return false;
+  const int RL = p->rl_choice > 0 ? p->rl_choice : legendre_default_rl(p->K);
+  if (!build_leg_waves(p->arena, t, RL, p->stream, &p->T_exec)) return false;
+  t.K = p->K;
+  t.KB = legendre_map_block(p->K);
+  if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil"))
+    return false;
+  if (!dalloc(p->arena, &t.qbuf, (size_t)t.R * 2 * (p->M + 1) * p->K, "qbuf")) return false;
   return true;
 }
 
@@ -241,9 +233,10 @@ static int next_pow2(int v) {
 
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
-static bool build_ring_tables(wavemoth_plan p) {
-  RingTables &rt = p->rt;
-  const int N = p->N;
+namespace wm {
+
+bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int M,
+                         bool phase_shifts, hipStream_t stream) {
   const int nrings = 4 * N - 1, mid = 2 * N - 1, R = 2 * N;
   // Ring length of iring and phi0 = pi * num / den of grid entry iring
   // (src/wavemoth.c:1199-1232); den always equals the ring length.
@@ -265,7 +258,7 @@ static bool build_ring_tables(wavemoth_plan p) {
     geom(i, n, num);
     offs[i + 1] = offs[i] + n;
   }
-  p->npix = offs[nrings];
+  host.npix = offs[nrings];
   rt.npairs = R;
   std::vector<RingPair> pairs(R);
   std::vector<double> cost(R);
@@ -284,13 +277,13 @@ static bool build_ring_tables(wavemoth_plan p) {
     rp.pix_n = offs[mid - r];
     rp.pix_s = (r == 0) ? -1 : offs[mid + r];
     rp.n = n;
-    rp.s = p->phase_shifts ? num : 0;
+    rp.s = phase_shifts ? num : 0;
     rp.r = r;
     const int h = n / 2;
     if (is_pow2(h)) {
       rp.P = 0;
       rp.chirp_off = rp.vhat_off = 0;
-      cost[r] = (double)h * std::log2((double)h + 1.0) + (double)(p->M + 1);
+      cost[r] = (double)h * std::log2((double)h + 1.0) + (double)(M + 1);
     } else {
       const int P = next_pow2(2 * h - 1);
       if (h_index[h] < 0) {
@@ -306,9 +299,9 @@ static bool build_ring_tables(wavemoth_plan p) {
       rp.P = P;
       rp.chirp_off = choff[hi];
       rp.vhat_off = vhoff[hi];
-      cost[r] = 2.0 * (double)P * std::log2((double)P) + (double)(p->M + 1);
+      cost[r] = 2.0 * (double)P * std::log2((double)P) + (double)(M + 1);
     }
-    need[r] = ring_lds_slots(n, rp.P, p->M);
+    need[r] = ring_lds_slots(n, rp.P, M);
     if (rp.P > kMaxFFT || (rp.P == 0 && h > kMaxFFT)) {
       set_err("ring stage: Nside too large for the LDS-resident ring FFT (Nside <= 2048)");
       return false;
@@ -345,22 +338,34 @@ static bool build_ring_tables(wavemoth_plan p) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * j / kMaxFFT;
     tw[j] = make_double2((double)cosl(a), (double)sinl(a));
   }
-  if (!upload(p, &rt.pairs, pairs, "ring pairs")) return false;
-  if (!upload(p, &rt.order, order, "ring order")) return false;
-  if (!upload(p, &rt.twiddle, tw, "twiddle")) return false;
-  if (!dalloc(p, &rt.chirp, (size_t)chtot, "chirp")) return false;
-  if (!dalloc(p, &rt.vhat, (size_t)vhtot, "vhat")) return false;
+  if (!upload(a, &rt.pairs, pairs, "ring pairs")) return false;
+  if (!upload(a, &rt.order, order, "ring order")) return false;
+  if (!upload(a, &rt.twiddle, tw, "twiddle")) return false;
+  if (!dalloc(a, &rt.chirp, (size_t)chtot, "chirp")) return false;
+  if (!dalloc(a, &rt.vhat, (size_t)vhtot, "vhat")) return false;
   if (!hlist.empty()) {
     int *d_h = nullptr, *d_p = nullptr;
     int64_t *d_co = nullptr, *d_vo = nullptr;
-    if (!upload(p, &d_h, hlist, "hlist")) return false;
-    if (!upload(p, &d_p, plist, "plist")) return false;
-    if (!upload(p, &d_co, choff, "choff")) return false;
-    if (!upload(p, &d_vo, vhoff, "vhoff")) return false;
-    WM_TRY(launch_bluestein_prep(rt, d_h, d_co, d_vo, d_p, (int)hlist.size(), p->stream),
+    if (!upload(a, &d_h, hlist, "hlist")) return false;
+    if (!upload(a, &d_p, plist, "plist")) return false;
+    if (!upload(a, &d_co, choff, "choff")) return false;
+    if (!upload(a, &d_vo, vhoff, "vhoff")) return false;
+    WM_TRY(launch_bluestein_prep(rt, d_h, d_co, d_vo, d_p, (int)hlist.size(), stream),
            "bluestein prep");
-    WM_TRY(hipStreamSynchronize(p->stream), "bluestein prep");
+    WM_TRY(hipStreamSynchronize(stream), "bluestein prep");
   }
+  host.pairs = std::move(pairs);
+  host.need = std::move(need);
+  host.cost = std::move(cost);
+  return true;
+}
+
+}  // namespace wm
+
+static bool build_ring_tables(wavemoth_plan p) {
+  if (!build_ring_geometry(p->arena, p->rt, p->rhost, p->N, p->M, p->phase_shifts, p->stream))
+    return false;
+  p->npix = p->rhost.npix;
   return true;
 }
 
@@ -521,8 +526,8 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     return nullptr;
   }
   if (!build_legendre_tables(p) || !build_ring_tables(p) ||
-      !dalloc(p, &p->d_alm, (size_t)p->nalm * nmaps, "alm") ||
-      !dalloc(p, &p->d_map, (size_t)p->npix * nmaps, "map")) {
+      !dalloc(p->arena, &p->d_alm, (size_t)p->nalm * nmaps, "alm") ||
+      !dalloc(p->arena, &p->d_map, (size_t)p->npix * nmaps, "map")) {
     std::string e = g_err;
     free_plan(p);
     g_err = e;
@@ -587,10 +592,10 @@ void wavemoth_disable_phase_shifting(wavemoth_plan plan) {
 
 int64_t wavemoth_get_legendre_flops(wavemoth_plan plan, int m, int odd) {
   if (!plan || m < 0 || m > plan->M || odd < 0 || odd > 1) return -1;
-  return 4 * plan->elems[(size_t)2 * m + odd];
+  return 4 * plan->cnt.elems[(size_t)2 * m + odd];
 }
 
-const char *wavemoth_last_error(void) { return g_err.c_str(); }
+const char *wavemoth_last_error(void) { return last_err(); }
 
 int wavemoth_get_work(wavemoth_plan p, double *out) {
   g_err.clear();
@@ -635,7 +640,7 @@ int wavemoth_execute_device(wavemoth_plan p, const double *d_input, double *d_ou
 static bool run_adjoint(wavemoth_plan p, const double *d_map, double2 *d_alm, hipStream_t s) {
   if (!p->adj_partial) {
     p->adj_groups = adjoint_groups(p->lt);
-    if (!dalloc(p, &p->adj_partial, (size_t)p->adj_groups * p->lt.nlpad * p->K,
+    if (!dalloc(p->arena, &p->adj_partial, (size_t)p->adj_groups * p->lt.nlpad * p->K,
                 "adjoint partial sums"))
       return false;
   }
@@ -721,11 +726,11 @@ int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
     case 2: return p->M;
     case 3: return p->K;
     case 4: return p->npix;
-    case 5: return p->T_dense;
-    case 6: return p->T_eps;
+    case 5: return p->cnt.T_dense;
+    case 6: return p->cnt.T_eps;
     case 7: return p->flags;
     case 8: return 1;
-    case 9: return (int64_t)p->dev_bytes;
+    case 9: return (int64_t)p->arena.bytes;
     case 10: return p->T_exec;
     default: return -1;
   }

@@ -23,17 +23,21 @@
 // conjugate tabulated per h at plan time.  LDS arrays are padded by one
 // slot per 8 (pidx) so the Stockham writes of the first passes do not pile
 // onto one bank.
+#include <mutex>
+
 #include "wm_fft.h"
 
 namespace wm {
 
 using namespace fft;
 
-__global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, int first,
-                                                   int lds_slots, double *__restrict__ map) {
-  extern __shared__ double2 lds[];
-  double2 *twl = lds + lds_slots - kTwSlots;  // twiddle table at the end of LDS
-  const RingPair rp = rt.pairs[rt.order[first + blockIdx.x]];
+// One ring pair (index into rt.pairs) by the whole block; `lds` holds
+// lds_slots complex slots, the last kTwSlots of them the twiddle table
+// (already loaded).  Used by k_ring_pairs and the batched kernel.
+__device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair,
+                               double2 *lds, int lds_slots, double *__restrict__ map) {
+  double2 *twl = lds + lds_slots - kTwSlots;
+  const RingPair rp = rt.pairs[pair];
   const int n = rp.n, h = n / 2, B = h + 1, K = t.K, M = t.M, P = rp.P;
   const bool south = rp.pix_s >= 0;
   const double2 *qe = t.qbuf + ((int64_t)rp.r * 2 + 0) * (M + 1) * K;
@@ -45,7 +49,6 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
   const int G = ring_groups(B);
   const int J = (M + h) / n + 2;   // j runs over [0, J)
   const int JG = (J + G - 1) / G;  // j-iterations per group
-  load_twiddles(twl, rt.twiddle);
 
   const int nhemi = south ? 2 : 1;
   for (int k = 0; k < K; ++k) {
@@ -208,6 +211,58 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
   }
 }
 
+__global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, int first,
+                                                   int lds_slots, double *__restrict__ map) {
+  extern __shared__ double2 lds[];
+  load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
+  ring_pair_body(t, rt, rt.order[first + blockIdx.x], lds, lds_slots, map);
+}
+
+// Persistent ring stage of a batch: each block takes ring pairs of any
+// request from the queue ritems[first .. first + count) via head `ctr`.
+__global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int count, int ctr,
+                                                     int slots) {
+  extern __shared__ double2 lds[];
+  __shared__ int s_item;
+  load_twiddles(lds + slots - kTwSlots, b.twiddle);
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&b.counters[ctr], 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= count) break;
+    const BatchRingItem item = b.ritems[first + it];
+    ring_pair_body(b.tabs[item.req], b.rtabs[item.req], item.pair, lds, slots,
+                   b.map[item.req]);
+  }
+}
+
+hipError_t batch_rings_set_lds_limit(int slots) {
+  // The limit only grows, so plans made earlier keep a valid launch size.
+  static std::mutex mu;
+  static int cur = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (slots <= cur) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute((const void *)k_batch_rings,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)((size_t)slots * sizeof(double2)));
+  if (e == hipSuccess) cur = slots;
+  return e;
+}
+
+hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_rings, NT,
+                                                       (size_t)slots * sizeof(double2));
+}
+
+hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
+                              int nb, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_rings, dim3(nb), dim3(NT), (size_t)slots * sizeof(double2), s, b,
+                     first, count, ctr, slots);
+  return hipGetLastError();
+}
+
 // Plan-time Bluestein tables, one block per distinct h:
 // chirp[a] = e^{i pi a^2 / h} (a < h), and vhat = FFT_P(v) / P with
 // v[d] = conj(chirp[|d|]) placed cyclically (d in (-h, h)).
@@ -261,9 +316,16 @@ hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const i
 }
 
 hipError_t ring_set_lds_limit(int slots) {
-  return hipFuncSetAttribute((const void *)k_ring_pairs,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)((size_t)slots * sizeof(double2)));
+  // The limit only grows, so plans made earlier keep a valid launch size.
+  static std::mutex mu;
+  static int cur = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (slots <= cur) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)((size_t)slots * sizeof(double2)));
+  if (e == hipSuccess) cur = slots;
+  return e;
 }
 
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,

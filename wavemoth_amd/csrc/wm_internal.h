@@ -26,6 +26,7 @@ constexpr int kRingThreads = 512;
 struct LegTables {
   int N = 0, L = 0, M = 0, K = 0, R = 0;
   int RL = 0;        // rings per lane
+  int KB = 0;        // maps per sweep pass (atil holds ceil(K/KB)*KB maps)
   int W = 0;         // ring chunks (waves) per m
   int64_t nlpad = 0; // padded l entries over all m
   int64_t *off = nullptr;     // [M+1] start of m's padded l-range
@@ -40,7 +41,7 @@ struct LegTables {
   double2 *seed_v = nullptr;  // [(M+1)*R] (Lambda_{l-1} a_{l-1}, Lambda_l)
   int *wstart = nullptr;      // [(M+1)*W] min seed_l over a wave's rings
   int *wseedmax = nullptr;    // [(M+1)*W] max seed_l (<= lmax) over a wave
-  double2 *atil = nullptr;    // [nlpad*K] prescaled a_lm (per execute)
+  double2 *atil = nullptr;    // [ceil(K/KB)][nlpad][KB] prescaled a_lm (per execute)
   double2 *qbuf = nullptr;    // [R][2][M+1][K] even/odd Legendre sums
 };
 
@@ -74,8 +75,50 @@ struct RingTables {
   int lds[kMaxLaunches] = {0};
 };
 
+// Batched execution of many independent transforms: prescale, Legendre
+// sweep and ring stage each run as one persistent work-queue kernel over
+// the union of the requests' work items, in the order the host put them.
+struct BatchLegItem {
+  int req, m, w, kb;  // request, m, ring chunk, map block
+};
+struct BatchRingItem {
+  int req, pair;
+};
+constexpr int kBatchCounters = 4;  // queue heads: sweep, ring class 0, ring class 1
+struct BatchDev {
+  int nreq = 0;
+  const LegTables *tabs = nullptr;      // [nreq] per-request tables (K, KB, RL, buffers)
+  const RingTables *rtabs = nullptr;    // [nreq] ring tables of the request's geometry
+  const double2 *const *alm = nullptr;  // [nreq] device input of each request
+  double *const *map = nullptr;         // [nreq] device output of each request
+  const int64_t *prep_off = nullptr;    // [nreq + 1] prefix sums of nlpad
+  int64_t prep_total = 0;
+  const BatchLegItem *litems = nullptr;
+  int nlitems = 0;
+  const BatchRingItem *ritems = nullptr;
+  const double2 *twiddle = nullptr;     // [kMaxFFT] shared twiddle table
+  int *counters = nullptr;              // [kBatchCounters] work-queue heads
+};
+
+// (RL, KB) sweep shapes the batched kernel is compiled for.  The set is
+// kept to shapes of at most ~80 VGPRs, since the kernel holds the register
+// budget of its largest case.
+bool batch_shape_supported(int RL, int KB);
+
+hipError_t launch_batch_reset(const BatchDev &b, hipStream_t s);
+hipError_t launch_batch_prep(const BatchDev &b, hipStream_t s);
+hipError_t launch_batch_legendre(const BatchDev &b, int nblocks, hipStream_t s);
+// Ring items [first, first + count) of b.ritems through queue head `ctr`,
+// with `slots` complex slots of dynamic LDS per block.
+hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
+                              int nblocks, hipStream_t s);
+hipError_t batch_rings_set_lds_limit(int slots);
+hipError_t batch_legendre_occupancy(int *blocks_per_cu);
+hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu);
+
 // Shape of the Legendre sweep for nmaps = K.
-int legendre_rings_per_lane(int K);
+int legendre_map_block(int K);
+int legendre_default_rl(int K);
 int legendre_padded_maps(int K);
 
 // Launchers (return hipError_t of the launch).
