@@ -1,17 +1,30 @@
 #!/usr/bin/env python3
-"""Throughput of FP64 a_lm -> HEALPix synthesis on one MI355X.
+"""Throughput of a_lm -> HEALPix synthesis on one MI355X.
 
-One step = one full synthesis (Legendre sweep + ring stage) of an a_lm set
-already resident in HBM into a map in HBM, through the library's
-device-pointer execute.  Default workload: Nside = 2048, lmax = mmax = 4096,
-nmaps = 1, seeded power-law a_lm (C_l = (l+1)^-2), FP64 throughout.
+Workloads (--workload):
+  fp64   (default) one step = one FP64 synthesis (Legendre sweep + ring
+         stage) of an a_lm set already resident in HBM into a map in HBM,
+         through the library's device-pointer execute.  Nside = 2048,
+         lmax = mmax = 4096, nmaps = 1, seeded power-law a_lm
+         (C_l = (l+1)^-2).
+  batch  one step = one execute of a batch of 96 uneven requests (Nside
+         uniform over {128 .. 2048}, lmax = mmax = 2 Nside, nmaps over
+         {1, 3, 8}, seeded), FP64, replayed from the batch's HIP graph.
+  mixed  Nside = 2048, lmax = 4096, nmaps = 64: the FP64 path and each
+         mixed-precision contraction scheme (FP32 MFMA, BF16x3, BF16x6) on
+         the same a_lm; one step = one synthesis of all 64 maps.  Reports
+         each scheme's maps/s and its rms / max error against the FP64 map
+         for C_l = (l+1)^-2 and for flat C_l = 1; `value` is the FP32-MFMA
+         scheme's maps/s.
 
 Prints one JSON line.  `roofline` prices the Legendre sweep (the dominant
 kernel) against the FP64 peak with the reference's flop convention
 (5 + 4 nmaps flops per evaluated (l, m, ring-pair) triple); its duration is
-measured with HIP events on the execute stream over the timed steps.
-`cpu_baseline` times the CPU oracle (a C restatement of the reference
-algorithm, OpenMP) on a bounded sample of the same workload.
+measured with HIP events recorded on the execute stream around the kernel.
+`cpu_baseline` (fp64 workload) times the CPU oracle (a C restatement of the
+reference algorithm, OpenMP) on a bounded sample of the same workload.
+--calibrate-copy MIB runs a device copy of that size first (untimed), as
+the known-size kernel against which HBM byte counters are scaled.
 """
 import argparse
 import json
@@ -31,14 +44,23 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["fp64", "batch", "mixed"], default="fp64")
+    ap.add_argument("--mixed-nmaps", type=int, default=64)
+    ap.add_argument("--schemes", default="f32,bf16x3,bf16x6")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="mixed workload: timed steps of the FP64 reference path")
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=None)
     ap.add_argument("--nmaps", type=int, default=1)
+    ap.add_argument("--nreq", type=int, default=96, help="batch workload: requests")
+    ap.add_argument("--measure", action="store_true",
+                    help="batch workload: autotune sweep shapes (WAVEMOTH_MEASURE)")
     ap.add_argument("--fly", action="store_true",
                     help="generate the Lambda seeds inside every execute")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-m-stride", type=int, default=32,
                     help="CPU baseline samples every k-th m")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--seed", type=int, default=20482048)
     return ap.parse_args()
 
@@ -46,7 +68,6 @@ def parse():
 def cpu_baseline(N, L, K, alm, stride):
     """Time the oracle: Legendre stage on m = 0, stride, 2 stride, ...
     (extrapolated by (mmax+1)/#sampled), ring stage on the full map."""
-    import numpy as np
     from oracle import oracle as O
     M = L
     op = O.OraclePlan(N, L, M, K, m_start=0, m_stop=M + 1, m_step=stride)
@@ -76,14 +97,82 @@ def cpu_baseline(N, L, K, alm, stride):
     }
 
 
-def main():
-    args = parse()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        sys.stderr.write("bench.py runs on exactly one GPU: this build has no multi-GPU "
-                         "path (--gpus %d, WORLD_SIZE %d requested)\n" % (args.gpus, world))
-        return 2
+def cpu_baseline_mixed(N, L, K, alm, stride):
+    """Oracle on a bounded sample of the nmaps = K workload: Legendre stage
+    at nmaps = K on every `stride`-th m (extrapolated), ring stage timed at
+    nmaps = 1 on the full map and multiplied by K (it is independent per map)."""
     import numpy as np
+    from oracle import oracle as O
+    M = L
+    op = O.OraclePlan(N, L, M, K, m_start=0, m_stop=M + 1, m_step=stride)
+    nsamp = len(range(0, M + 1, stride))
+    t0 = time.perf_counter()
+    op.legendre_stage(alm, compact=True)
+    t_leg = time.perf_counter() - t0
+    del op
+    op1 = O.OraclePlan(N, L, M, 1, m_start=0, m_stop=1, m_step=1)
+    work = np.zeros((M + 1, 2, 2 * N, 1), np.complex128)
+    t0 = time.perf_counter()
+    op1.c2r(op1.ring_spectra(work))
+    t_ring1 = time.perf_counter() - t0
+    total = t_leg * (M + 1) / nsamp + t_ring1 * K
+    return {
+        "value": K / total,
+        "unit": "maps/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": ("oracle (C restatement of the reference's seeds + x^2 recurrence, "
+                   "OpenMP): Legendre stage at nmaps=%d timed on %d of %d m values (every "
+                   "%d-th), %.3f s, scaled by %d/%d; ring stage (fold, phase, numpy c2r) "
+                   "timed at nmaps=1 on the full map (zero work array; its cost does not "
+                   "depend on the data), %.3f s, times %d maps" %
+                   (K, nsamp, M + 1, stride, t_leg, M + 1, nsamp, t_ring1, K)),
+        "legendre_s": t_leg * (M + 1) / nsamp,
+        "ring_s": t_ring1 * K,
+    }
+
+
+def calibrate_copy(mib, dev):
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"bytes_read": n * 8, "bytes_written": n * 8, "launches": 3}
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)[key]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(flops, kernel_ms, kernel, extra):
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    r = {
+        "bound": "mfma",
+        "pipe": "fp64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+        "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "traffic": None,
+        "kernel": kernel,
+        "flops_per_launch": flops,
+        "flop_convention": "(5 + 4*nmaps) per evaluated (l, m, ring-pair) triple",
+        "kernel_ms": kernel_ms,
+    }
+    r.update(extra)
+    return r
+
+
+def run_fp64(args, dev, res):
     import torch
 
     from wavemoth_amd import WAVEMOTH_SEEDS_ON_THE_FLY
@@ -93,8 +182,6 @@ def main():
     N = args.nside
     L = args.lmax if args.lmax is not None else 2 * N
     K = args.nmaps
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     alm_h = power_law_alm(L, L, K, args.seed)
     alm = torch.from_numpy(alm_h).to(dev)
     npix = 12 * N * N
@@ -117,49 +204,27 @@ def main():
     t1 = time.perf_counter()
     stage_ms, nrec = plan.stage_times()
     plan.set_profiling(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    maps_per_s = K * args.steps / (t1 - t0)
-    finite = bool(torch.isfinite(out).all().item())
 
     T_eps = info["triples_eps"]
     T_dense = info["triples_dense"]
     flops = (5 + 4 * K) * T_eps
-    leg_s = stage_ms[1] / 1e3
-    achieved = flops / leg_s / 1e12
     ring_bytes = 2 * (L + 1) * 2 * N * 2 * K * 16 + npix * K * 8
-    res = {
+    res.update({
         "metric": "maps/s synthesis Nside=%d lmax=%d: FP64" % (N, L),
-        "value": maps_per_s,
-        "unit": "maps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "value": K * args.steps / (t1 - t0),
+        "ms_per_step": (t1 - t0) * 1e3 / args.steps,
         "dtype": "f64",
         "data": "synthetic (seeded power-law a_lm, C_l=(l+1)^-2)",
         "config": {"workload": "a_lm->HEALPix synthesis, FP64, seeds %s" %
                                ("per execute" if args.fly else "cached in plan"),
                    "nside": N, "lmax": L, "mmax": L, "nmaps": K, "npix": npix},
-        "roofline": {
-            "bound": "mfma",
-            "pipe": "fp64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "k_legendre",
-            "flops_per_launch": flops,
-            "flop_convention": "(5 + 4*nmaps) per evaluated (l, m, ring-pair) triple",
-            "triples_eps": T_eps,
-            "triples_dense": T_dense,
+        "roofline": roofline(flops, stage_ms[1], "k_legendre", {
+            "traffic": measured_traffic("k_legendre nside=%d lmax=%d nmaps=%d" % (N, L, K)),
+            "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                              "x2.0 / x1.0 from a 1 GiB copy)",
+            "triples_eps": T_eps, "triples_dense": T_dense,
             "triples_executed": info["triples_executed"],
-            "achieved_dense_equiv": (5 + 4 * K) * T_dense / leg_s / 1e12,
-            "kernel_ms": stage_ms[1],
-        },
+            "achieved_dense_equiv": (5 + 4 * K) * T_dense / (stage_ms[1] / 1e3) / 1e12}),
         "stages_ms": {"prescale": stage_ms[0], "legendre": stage_ms[1], "rings": stage_ms[2],
                       "events": nrec},
         "ring_stage": {"bytes": ring_bytes,
@@ -167,11 +232,210 @@ def main():
                        "frac_hbm": (ring_bytes / (stage_ms[2] / 1e3) / 1e9 / HBM_PEAK_GBS
                                     if stage_ms[2] else None)},
         "plan_s": t_plan,
-        "output_finite": finite,
-        "cpu_baseline": None,
-    }
+        "output_finite": bool(torch.isfinite(out).all().item()),
+    })
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(N, L, K, alm_h, args.cpu_m_stride)
+
+
+def run_batch(args, dev, res):
+    import torch
+
+    from wavemoth_amd import WAVEMOTH_MEASURE, BatchPlan, random_requests
+    from wavemoth_amd.torch_sht import random_alm
+
+    reqs = random_requests(args.nreq, seed=args.seed % (1 << 31))
+    t0 = time.perf_counter()
+    b = BatchPlan(reqs, WAVEMOTH_MEASURE if args.measure else 0)
+    t_plan = time.perf_counter() - t0
+    info = b.info()
+    alms = [random_alm(L, M, K, args.seed + i, dev) for i, (N, L, M, K) in enumerate(reqs)]
+    outs = [torch.empty((12 * N * N, K), dtype=torch.float64, device=dev)
+            for (N, L, M, K) in reqs]
+    for _ in range(args.warmup):
+        b.synthesize_into(alms, outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.synthesize_into(alms, outs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # stage split from direct-launch executes (profiling bypasses the graph)
+    b.set_profiling(True)
+    for _ in range(2):
+        b.synthesize_into(alms, outs)
+    stage_ms, nrec = b.stage_times()
+    b.set_profiling(False)
+    maps = info["maps"]
+    shapes = {}
+    for r in range(len(reqs)):
+        ri = b.request_info(r)
+        shapes["%d/%d" % (ri["Nside"], ri["nmaps"])] = [ri["rings_per_lane"],
+                                                       ri["maps_per_pass"]]
+    res.update({
+        "metric": "maps/s batched uneven synthesis (%d requests): FP64" % len(reqs),
+        "value": maps * args.steps / (t1 - t0),
+        "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+        "dtype": "f64",
+        "data": "synthetic (seeded power-law a_lm, C_l=(l+1)^-2, drawn on the device)",
+        "config": {"workload": "batched uneven a_lm->HEALPix synthesis, FP64, hipGraph replay",
+                   "requests": len(reqs), "maps": maps,
+                   "nside_choices": [128, 256, 512, 1024, 2048], "nmaps_choices": [1, 3, 8],
+                   "lmax": "2*Nside", "autotuned": bool(args.measure)},
+        "roofline": roofline(info["flops"], stage_ms[1], "k_batch_legendre", {
+            "triples_eps": info["triples_eps"], "triples_dense": info["triples_dense"],
+            "slots_executed": info["slots_executed"],
+            "sweep_items": info["sweep_items"], "sweep_workgroups": info["sweep_workgroups"]}),
+        "stages_ms": {"prescale": stage_ms[0], "legendre": stage_ms[1], "rings": stage_ms[2],
+                      "events": nrec, "note": "direct launches; value uses graph replays"},
+        "sweep_shapes": {"Nside/nmaps -> [rings_per_lane, maps_per_pass]":
+                         dict(sorted(shapes.items()))},
+        "plan_s": t_plan,
+        "device_bytes": info["device_bytes"],
+        "output_finite": all(bool(torch.isfinite(o).all().item()) for o in outs),
+    })
+
+
+SCHEMES = {"f32": (0x1000, "FP32 MFMA", 157.3, 1),
+           "bf16x3": (0x2000, "BF16x3", 2500.0, 3),
+           "bf16x6": (0x4000, "BF16x6", 2500.0, 6)}
+
+
+def _time_plan(plan, alm, out, warmup, steps):
+    import torch
+    for _ in range(warmup):
+        plan.synthesize_into(alm, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.synthesize_into(alm, out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    plan.set_profiling(True)
+    for _ in range(2):
+        plan.synthesize_into(alm, out)
+    stage_ms, _ = plan.stage_times()
+    plan.set_profiling(False)
+    return dt, stage_ms
+
+
+def _err(x, ref):
+    import torch
+    d = x - ref
+    return {"rms": float(torch.linalg.norm(d) / torch.linalg.norm(ref)),
+            "max": float(d.abs().max() / torch.sqrt(torch.mean(ref * ref)))}
+
+
+def run_mixed(args, dev, res):
+    import torch
+
+    from wavemoth_amd.torch_sht import DevicePlan, random_alm
+
+    N = args.nside
+    L = args.lmax if args.lmax is not None else 2 * N
+    K = args.mixed_nmaps
+    npix = 12 * N * N
+    alms = {"powerlaw": random_alm(L, L, K, args.seed, dev),
+            "flat": random_alm(L, L, K, args.seed + 1, dev, flat=True)}
+    out = torch.empty((npix, K), dtype=torch.float64, device=dev)
+
+    # FP64 reference maps and FP64 throughput at the same nmaps
+    p64 = DevicePlan(N, L, L, K)
+    info = p64.info()
+    T_eps = info["triples_eps"]
+    dt64, st64 = _time_plan(p64, alms["powerlaw"], out, 1, args.fp64_steps)
+    refs = {}
+    for name, a in alms.items():
+        p64.synthesize_into(a, out)
+        torch.cuda.synchronize()
+        refs[name] = out.clone()
+    del p64
+    # FP64 at nmaps = 1 (the FP64 headline shape)
+    p1 = DevicePlan(N, L, L, 1)
+    a1 = alms["powerlaw"][:, :1].contiguous()
+    o1 = torch.empty((npix, 1), dtype=torch.float64, device=dev)
+    dt1, st1 = _time_plan(p1, a1, o1, 2, 10)
+    del p1, a1, o1
+
+    schemes = {}
+    for key in [k.strip() for k in args.schemes.split(",") if k.strip()]:
+        flag, label, peak, nprod = SCHEMES[key]
+        p = DevicePlan(N, L, L, K, flag)
+        dt, st = _time_plan(p, alms["powerlaw"], out, args.warmup, args.steps)
+        errs = {}
+        for name, a in alms.items():
+            p.synthesize_into(a, out)
+            torch.cuda.synchronize()
+            errs[name] = _err(out, refs[name])
+        del p
+        flops = 4.0 * K * T_eps
+        schemes[key] = {
+            "label": label, "maps_per_s": K / dt, "ms_per_step": dt * 1e3,
+            "stages_ms": {"prescale": st[0], "contraction": st[1], "rings": st[2]},
+            "contraction_tflops": flops / (st[1] / 1e3) / 1e12,
+            "mfma_peak_tflops": peak,
+            "mfma_frac": nprod * flops / (st[1] / 1e3) / 1e12 / peak,
+            "errors_vs_fp64": errs,
+            "meets_1e-6": all(e["rms"] <= 1e-6 for e in errs.values()),
+        }
+    main_key = "f32" if "f32" in schemes else next(iter(schemes))
+    m = schemes[main_key]
+    res.update({
+        "metric": "maps/s synthesis Nside=%d lmax=%d: FP64 vs batched FP32-MFMA @<=1e-6 err"
+                  % (N, L),
+        "value": m["maps_per_s"],
+        "ms_per_step": m["ms_per_step"],
+        "dtype": "f32 MFMA contraction (FP64 recurrence, FP64 l-chunk reduction)",
+        "data": "synthetic (seeded a_lm drawn on the device; C_l=(l+1)^-2 timed, flat C_l=1 "
+                "also checked)",
+        "config": {"workload": "a_lm->HEALPix synthesis of %d maps, mixed-precision "
+                               "contraction vs FP64" % K,
+                   "nside": N, "lmax": L, "mmax": L, "nmaps": K, "npix": npix},
+        "roofline": {
+            "bound": "mfma", "achieved": m["contraction_tflops"],
+            "peak": SCHEMES[main_key][2], "unit": "TFLOP/s",
+            "frac": m["mfma_frac"], "traffic": None, "kernel": "k_mix<%d>" %
+            ["f32", "bf16x3", "bf16x6"].index(main_key),
+            "flops_per_launch": 4.0 * K * T_eps,
+            "flop_convention": "4*nmaps per (l, m, ring-pair) triple at or after first l "
+                               "(the contraction; the FP64 recurrence is not counted)",
+            "kernel_ms": m["stages_ms"]["contraction"]},
+        "fp64": {"nmaps": K, "maps_per_s": K / dt64, "ms_per_step": dt64 * 1e3,
+                 "stages_ms": {"prescale": st64[0], "legendre": st64[1], "rings": st64[2]},
+                 "nmaps1_maps_per_s": 1.0 / dt1, "nmaps1_stages_ms": st1},
+        "speedup_vs_fp64_same_nmaps": m["maps_per_s"] / (K / dt64),
+        "speedup_vs_fp64_nmaps1": m["maps_per_s"] * dt1,
+        "schemes": schemes,
+        "triples_eps": T_eps,
+    })
+    if not args.no_cpu_baseline:
+        a_h = alms["powerlaw"].cpu().numpy()
+        res["cpu_baseline"] = cpu_baseline_mixed(N, L, K, a_h, args.cpu_m_stride * 2)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        sys.stderr.write("bench.py runs on exactly one GPU: this build has no multi-GPU "
+                         "path (--gpus %d, WORLD_SIZE %d requested)\n" % (args.gpus, world))
+        return 2
+    import torch
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    res = {"metric": None, "value": None, "unit": "maps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None}
+    if args.calibrate_copy:
+        res["copy_calibration"] = calibrate_copy(args.calibrate_copy, dev)
+    if args.workload == "fp64":
+        run_fp64(args, dev, res)
+    elif args.workload == "mixed":
+        run_mixed(args, dev, res)
+    else:
+        run_batch(args, dev, res)
+    res.setdefault("cpu_baseline", None)
     print(json.dumps(res), flush=True)
     return 0
 

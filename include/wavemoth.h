@@ -43,6 +43,17 @@ typedef struct _wavemoth_plan *wavemoth_plan;
 /* Extension flag: generate the per-(m, ring) Lambda seeds inside every
    execute instead of once at plan time. */
 #define WAVEMOTH_SEEDS_ON_THE_FLY 0x100
+/* Extension flags (at most one): the Legendre contraction on the matrix
+   cores, the Lambda recurrence and its seeds still in FP64.  Sums are
+   accumulated in FP32 within l-chunks of 128 and reduced across chunks in
+   FP64.  Meant for nmaps >= 16 (any nmaps works).
+     WAVEMOTH_MIXED_F32     FP32 inputs, FP32 MFMA;
+     WAVEMOTH_MIXED_BF16X3  Lambda and a split into two bf16 parts each,
+                            three products (hi*hi + hi*lo + lo*hi);
+     WAVEMOTH_MIXED_BF16X6  three bf16 parts each, six products. */
+#define WAVEMOTH_MIXED_F32 0x1000
+#define WAVEMOTH_MIXED_BF16X3 0x2000
+#define WAVEMOTH_MIXED_BF16X6 0x4000
 
 /* ---- stable driver API (replaces src/wavemoth.h:33-46) ---------------- */
 
@@ -124,7 +135,9 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
    (sum over m of (lmax-m+1) * 2 Nside), 6 trimmed triples T_eps evaluated by
    the plan, 7 flags, 8 legendre-kernel launches per execute, 9 device bytes
    held by the plan, 10 (l, ring-pair) slots the Legendre sweep evaluates
-   including its wave/chunk padding.  Returns -1 for an unknown key. */
+   including its wave/chunk padding, 11 contraction scheme (-1 FP64,
+   0 MIXED_F32, 1 MIXED_BF16X3, 2 MIXED_BF16X6).  Returns -1 for an
+   unknown key (key 11 of an FP64 plan is also -1). */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 
 /* ---- batched synthesis (extension) -------------------------------------

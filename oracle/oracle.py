@@ -59,7 +59,7 @@ def lib():
         L.or_plan_new.restype = ctypes.c_void_p
         L.or_plan_free.argtypes = [ctypes.c_void_p]
         L.or_legendre_stage.argtypes = [ctypes.c_void_p, _d, _d, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_ring_spectra.argtypes = [ctypes.c_void_p, _d, ctypes.c_int, _i64, _d]
         L.or_plan_trimmed_triples.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int]
@@ -216,12 +216,15 @@ class OraclePlan:
         """T_eps counted per (m, odd, ring) column, both parities."""
         return lib().or_plan_trimmed_triples(self._p, *self.m_range)
 
-    def legendre_stage(self, alm):
+    def legendre_stage(self, alm, compact=False):
+        """work[m, odd, ring, map]; with compact=True only the plan's sampled
+        m values, in order (for timing samples of large transforms)."""
         alm = np.ascontiguousarray(alm, np.complex128)
         assert alm.shape == (nalm(self.lmax, self.mmax), self.nmaps)
-        work = np.zeros((self.mmax + 1, 2, self.R, self.nmaps), np.complex128)
+        nm = len(range(*self.m_range)) if compact else self.mmax + 1
+        work = np.zeros((nm, 2, self.R, self.nmaps), np.complex128)
         lib().or_legendre_stage(self._p, _ptr(alm.view(np.float64)),
-                                _ptr(work.view(np.float64)), *self.m_range)
+                                _ptr(work.view(np.float64)), *self.m_range, int(bool(compact)))
         return work
 
     def ring_spectra(self, work, phase_shifts=True):

@@ -394,7 +394,7 @@ void or_plan_free(or_plan *p) {
    work: complex128 [M+1][2][R][nmaps] (the reference's get_work layout,
    wavemoth/lib.pyx:158-179).  Only m in [m_start, m_stop) step m_step. */
 void or_legendre_stage(const or_plan *p, const double *alm, double *work, int m_start,
-                       int m_stop, int m_step) {
+                       int m_stop, int m_step, int compact) {
   int L = p->lmax, R = p->R, K = p->nmaps;
   int64_t nvecs = 2 * K;
 #pragma omp parallel
@@ -410,7 +410,9 @@ void or_legendre_stage(const or_plan *p, const double *alm, double *work, int m_
         const double *aux = p->aux + p->aux_off[2 * m + odd];
         for (int r = 0; r < R; ++r) {
           size_t slot = ((size_t)m * 2 + odd) * R + r;
-          double *out = work + 2 * (((size_t)m * 2 + odd) * R + r) * K;
+          /* compact: work holds only the sampled m, in sampling order */
+          size_t mo = compact ? (size_t)((m - m_start) / m_step) : (size_t)m;
+          double *out = work + 2 * ((mo * 2 + odd) * R + r) * K;
           int64_t k0 = p->k0[slot];
           const double *sd = p->seed + slot * 4;
           int64_t rows = nk - k0;

@@ -133,3 +133,12 @@ def test_batch_python_checks_requests():
     assert all(r[1] == r[2] == 2 * r[0] for r in reqs)
     assert {r[3] for r in reqs} <= {1, 3, 8}
     assert reqs == random_requests(96, seed=3)
+
+
+def test_two_mixed_flags_rejected():
+    """At most one WAVEMOTH_MIXED_* scheme; checked before any device work."""
+    from wavemoth_amd import WAVEMOTH_MIXED_BF16X3, WAVEMOTH_MIXED_F32, _native
+    L = _native.lib()
+    flags = WAVEMOTH_MIXED_F32 | WAVEMOTH_MIXED_BF16X3
+    assert not L.wavemoth_plan_to_healpix(4, 8, 8, 1, 1, None, None, 0, flags, None)
+    assert "MIXED" in _native.last_error()

@@ -1,0 +1,80 @@
+"""GPU tests of the mixed-precision contraction (WAVEMOTH_MIXED_*): the
+map of each scheme against the FP64 path on the same a_lm, for the power
+law and flat C_l, with the error bound of each scheme written here:
+  FP32 MFMA  rel rms <= 1e-6
+  BF16x6     rel rms <= 1e-6
+  BF16x3     rel rms <= 5e-5 (two bf16 parts carry ~16 bits)
+plus the oracle at a small size, determinism, mmax < lmax and nmaps not a
+multiple of the column block."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32, BF3, BF6 = 0x1000, 0x2000, 0x4000
+BOUND = {F32: 1e-6, BF6: 1e-6, BF3: 5e-5}
+NAME = {F32: "FP32", BF3: "BF16x3", BF6: "BF16x6"}
+
+
+def _maps(N, L, M, K, flags, alm):
+    import torch
+    from wavemoth_amd.torch_sht import DevicePlan
+    p = DevicePlan(N, L, M, K, flags)
+    out = torch.empty((12 * N * N, K), dtype=torch.float64, device=alm.device)
+    p.synthesize_into(alm, out)
+    torch.cuda.synchronize()
+    return out, p
+
+
+def _err(x, ref):
+    import torch
+    d = x - ref
+    rms = float(torch.linalg.norm(d) / torch.linalg.norm(ref))
+    mx = float(d.abs().max() / torch.sqrt(torch.mean(ref * ref)))
+    return rms, mx
+
+
+@pytest.mark.parametrize("scheme", [F32, BF3, BF6])
+@pytest.mark.parametrize("N,L,M,K,flat", [(32, 64, 64, 16, False), (64, 128, 128, 32, False),
+                                          (64, 128, 128, 32, True), (16, 40, 20, 3, False),
+                                          (128, 256, 256, 17, False)])
+def test_scheme_vs_fp64(scheme, N, L, M, K, flat):
+    import torch
+    from wavemoth_amd.torch_sht import random_alm
+    dev = torch.device("cuda:0")
+    print("case %s N=%d L=%d M=%d K=%d flat=%s" % (NAME[scheme], N, L, M, K, flat), flush=True)
+    alm = random_alm(L, M, K, 123 + N + K, dev, flat=flat)
+    ref, _ = _maps(N, L, M, K, 0, alm)
+    got, p = _maps(N, L, M, K, scheme, alm)
+    assert p.info()["scheme"] == {F32: 0, BF3: 1, BF6: 2}[scheme]
+    rms, mx = _err(got, ref)
+    print("rel rms %.3e  max/rms %.3e" % (rms, mx), flush=True)
+    assert rms <= BOUND[scheme]
+
+
+def test_fp32_scheme_vs_oracle(oracle):
+    import torch
+    from wavemoth_amd import power_law_alm
+    N, L, K = 16, 32, 16
+    alm_h = power_law_alm(L, L, K, 5)
+    got, _ = _maps(N, L, L, K, F32, torch.from_numpy(alm_h).to("cuda:0"))
+    ref = oracle.OraclePlan(N, L, L, K).synthesize(alm_h)
+    err = np.linalg.norm(got.cpu().numpy() - ref) / np.linalg.norm(ref)
+    print("rel rms vs oracle", err)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("scheme", [F32, BF6])
+def test_scheme_deterministic(scheme):
+    import torch
+    from wavemoth_amd.torch_sht import DevicePlan, random_alm
+    dev = torch.device("cuda:0")
+    N, L, K = 64, 128, 32
+    alm = random_alm(L, L, K, 9, dev)
+    p = DevicePlan(N, L, L, K, scheme)
+    a = torch.empty((12 * N * N, K), dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    p.synthesize_into(alm, a)
+    p.synthesize_into(alm, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)

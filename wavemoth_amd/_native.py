@@ -49,6 +49,9 @@ WAVEMOTH_MEASURE = 0x1
 WAVEMOTH_NO_RESOURCE_COPY = 0x10
 WAVEMOTH_SEEDS_ON_THE_FLY = 0x100
 WAVEMOTH_BATCH_NO_GRAPH = 0x200
+WAVEMOTH_MIXED_F32 = 0x1000
+WAVEMOTH_MIXED_BF16X3 = 0x2000
+WAVEMOTH_MIXED_BF16X6 = 0x4000
 
 
 def _bind_torch_runtime():

@@ -47,6 +47,12 @@ struct _wavemoth_plan {
   bool phase_shifts = true;
   bool fly = false;
   int rl_choice = 0;  // rings per lane of the Legendre sweep (0 = heuristic)
+  // mixed-precision contraction (extension): -1 = FP64 sweep, else scheme
+  // 0 FP32 MFMA, 1 BF16x3, 2 BF16x6
+  int mix = -1;
+  int mix_ncp = 0;          // padded real columns
+  int64_t mix_pstride = 0;  // elements per B part
+  void *mix_b = nullptr;
   LegCounts cnt;
   int64_t T_exec = 0;
   DevArena arena;
@@ -215,12 +221,29 @@ bool build_leg_waves(DevArena &a, LegTables &t, int RL, hipStream_t stream, int6
 static bool build_legendre_tables(wavemoth_plan p) {
   LegTables &t = p->lt;
   if (!build_leg_geometry(p->arena, t, p->N, p->L, p->M, p->stream, &p->cnt)) return false;
-  const int RL = p->rl_choice > 0 ? p->rl_choice : legendre_default_rl(p->K);
+  int RL = p->rl_choice > 0 ? p->rl_choice : legendre_default_rl(p->K);
+  if (p->mix >= 0) RL = mix_rings_per_lane_table();  // one wave-table entry per workgroup
   if (!build_leg_waves(p->arena, t, RL, p->stream, &p->T_exec)) return false;
   t.K = p->K;
   t.KB = legendre_map_block(p->K);
-  if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil"))
+  if (p->mix >= 0) {
+    // the mixed path reads the pre-split B operand instead of atil
+    const int cpb = mix_cols_per_block();
+    p->mix_ncp = (2 * p->K + cpb - 1) / cpb * cpb;
+    p->mix_pstride = mix_bmat_elems(t, p->mix_ncp);
+    const size_t esz = p->mix == 0 ? 4 : 2;
+    const size_t bytes = (size_t)p->mix_pstride * mix_parts(p->mix) * esz;
+    unsigned char *b = nullptr;
+    if (!dalloc(p->arena, &b, bytes, "mixed B operand")) return false;
+    // the trailing padding chunks are never written by the prescale
+    WM_TRY(hipMemsetAsync(b, 0, bytes, p->stream), "mixed B clear");
+    WM_TRY(hipStreamSynchronize(p->stream), "mixed B clear");
+    p->mix_b = b;
+    WM_TRY(mix_set_lds_limits(), "mixed kernel LDS limit");
+    if (!dalloc(p->arena, &t.atil, 1, "atil")) return false;
+  } else if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil")) {
     return false;
+  }
   if (!dalloc(p->arena, &t.qbuf, (size_t)t.R * 2 * (p->M + 1) * p->K, "qbuf")) return false;
   return true;
 }
@@ -413,9 +436,17 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
     WM_TRY(launch_seeds(p->lt, s), "seed kernel");
     WM_TRY(launch_wave_bounds(p->lt, s), "wave-bounds kernel");
   }
-  WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");
-  if (!mark(p, 1, s)) return false;
-  WM_TRY(launch_legendre(p->lt, s), "legendre kernel");
+  if (p->mix >= 0) {
+    WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, s),
+           "mixed prescale kernel");
+    if (!mark(p, 1, s)) return false;
+    WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, s),
+           "mixed contraction kernel");
+  } else {
+    WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");
+    if (!mark(p, 1, s)) return false;
+    WM_TRY(launch_legendre(p->lt, s), "legendre kernel");
+  }
   if (!mark(p, 2, s)) return false;
   return true;
 }
@@ -513,6 +544,17 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
   p->h_in = input;
   p->h_out = output;
   p->fly = (flags & WAVEMOTH_SEEDS_ON_THE_FLY) != 0;
+  {
+    const unsigned mf = flags & (WAVEMOTH_MIXED_F32 | WAVEMOTH_MIXED_BF16X3 | WAVEMOTH_MIXED_BF16X6);
+    if (mf == WAVEMOTH_MIXED_F32) p->mix = 0;
+    else if (mf == WAVEMOTH_MIXED_BF16X3) p->mix = 1;
+    else if (mf == WAVEMOTH_MIXED_BF16X6) p->mix = 2;
+    else if (mf != 0) {
+      set_err("at most one WAVEMOTH_MIXED_* flag");
+      delete p;
+      return nullptr;
+    }
+  }
   p->nalm = 0;
   for (int m = 0; m <= mmax; ++m) p->nalm += lmax - m + 1;
   if (hipGetDevice(&p->device) != hipSuccess) {
@@ -525,9 +567,7 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     delete p;
     return nullptr;
   }
-  if (!build_legendre_tables(p) || !build_ring_tables(p) ||
-      !dalloc(p->arena, &p->d_alm, (size_t)p->nalm * nmaps, "alm") ||
-      !dalloc(p->arena, &p->d_map, (size_t)p->npix * nmaps, "map")) {
+  if (!build_legendre_tables(p) || !build_ring_tables(p)) {
     std::string e = g_err;
     free_plan(p);
     g_err = e;
@@ -538,11 +578,22 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
 
 void wavemoth_destroy_plan(wavemoth_plan plan) { free_plan(plan); }
 
+// Device staging of the host-buffer executes, allocated on first use (a
+// plan driven only through device pointers never holds them).
+static bool ensure_staging(wavemoth_plan p) {
+  if (!p->d_alm && !dalloc(p->arena, &p->d_alm, (size_t)p->nalm * p->K, "alm staging"))
+    return false;
+  if (!p->d_map && !dalloc(p->arena, &p->d_map, (size_t)p->npix * p->K, "map staging"))
+    return false;
+  return true;
+}
+
 static bool execute_host(wavemoth_plan p, bool leg, bool rings) {
   if (!p) {
     set_err("NULL plan");
     return false;
   }
+  if (!ensure_staging(p)) return false;
   hipStream_t s = p->stream;
   if (leg) {
     if (!p->h_in) {
@@ -669,6 +720,7 @@ void wavemoth_execute_adjoint(wavemoth_plan p) {
     set_err("plan has no host buffers");
     return;
   }
+  if (!ensure_staging(p)) return;
   hipStream_t s = p->stream;
   if (hipMemcpyAsync(p->d_map, p->h_out, (size_t)p->npix * p->K * sizeof(double),
                      hipMemcpyHostToDevice, s) != hipSuccess) {
@@ -732,6 +784,7 @@ int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
     case 8: return 1;
     case 9: return (int64_t)p->arena.bytes;
     case 10: return p->T_exec;
+    case 11: return p->mix;
     default: return -1;
   }
 }

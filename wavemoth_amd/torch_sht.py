@@ -65,8 +65,28 @@ class DevicePlan:
 
     def info(self):
         keys = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
-                "flags", "legendre_launches", "device_bytes", "triples_executed"]
+                "flags", "legendre_launches", "device_bytes", "triples_executed", "scheme"]
         return {k: int(lib().wavemoth_plan_info(self._plan, i)) for i, k in enumerate(keys)}
+
+
+def random_alm(lmax, mmax, nmaps, seed, device, flat=False):
+    """Seeded a_lm generated on the device: Re, Im ~ N(0, C_l/2) for m > 0,
+    a_l0 real ~ N(0, C_l), C_l = (l+1)^-2 or 1 (flat).  Same law as
+    alm.power_law_alm but drawn by torch's device generator (fast at large
+    sizes; the values differ from the numpy stream)."""
+    from .alm import l_of_index
+    ls, ms = l_of_index(lmax, mmax)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    re = torch.randn((len(ls), nmaps), generator=g, device=device, dtype=torch.float64)
+    im = torch.randn((len(ls), nmaps), generator=g, device=device, dtype=torch.float64)
+    l_t = torch.from_numpy(ls).to(device=device, dtype=torch.float64)
+    m0 = torch.from_numpy(ms == 0).to(device)[:, None]
+    cl = torch.ones_like(l_t) if flat else (l_t + 1.0) ** -2
+    sd = torch.sqrt(cl / 2.0)[:, None]
+    sd = torch.where(m0, sd * (2.0 ** 0.5), sd)
+    im = torch.where(m0, torch.zeros_like(im), im)
+    return torch.complex(re * sd, im * sd).contiguous()
 
 
 def get_plan(Nside, lmax, mmax, nmaps, device, flags=0):
