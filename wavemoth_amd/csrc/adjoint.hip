@@ -36,13 +36,13 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
   const RingPair rp = rt.pairs[rt.order[first + blockIdx.x]];
   const int n = rp.n, h = n / 2, K = t.K, M = t.M, P = rp.P;
   const bool south = rp.pix_s >= 0;
-  double2 *qe = t.qbuf + ((int64_t)rp.r * 2 + 0) * (M + 1) * K;
-  double2 *qo = t.qbuf + ((int64_t)rp.r * 2 + 1) * (M + 1) * K;
   const double2 *chirp = rt.chirp + rp.chirp_off;
   const double2 *vhat = rt.vhat + rp.vhat_off;
   load_twiddles(twl, rt.twiddle);
   const int nhemi = south ? 2 : 1;
   for (int k = 0; k < K; ++k) {
+    double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);  // element m at [m * G]
+    double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
     for (int hemi = 0; hemi < nhemi; ++hemi) {
       int tid = threadIdx.x;
       asm volatile("" : "+v"(tid));
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
               const int kk = m % n;
               const double2 v = (kk <= h) ? lds[pidx(kk)] : cconj(lds[pidx(n - kk)]);
               const double2 f = cmul(v, w);
-              const int64_t o = (int64_t)m * K + k;
+              const int64_t o = (int64_t)m * q_group(K);
               if (hemi == 0) {
                 qe[o] = f;
                 qo[o] = f;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) void k_legendre_adj(LegTables t, int G, double2
           const int rr = rem / kn, kk = rem - rr * kn;
           const int rg = c * 64 + rr;
           qs[(par * 64 + rr) * 8 + kk] =
-              (rg < R) ? t.qbuf[(((int64_t)rg * 2 + par) * (M + 1) + m) * K + kb + kk]
+              (rg < R) ? t.qbuf[q_index(rg, par, kb + kk, m, K, M + 1)]
                        : make_double2(0.0, 0.0);
         }
         // lanes = rings: L~ for the window (only on the first map block)

@@ -147,7 +147,10 @@ bool request_tables(DevArena &keep, DevArena &a, Geometry *g, int K, Shape sh, h
   t.KB = sh.KB;
   const int nkb = (K + sh.KB - 1) / sh.KB;
   if (!dalloc(a, &t.atil, (size_t)t.nlpad * nkb * sh.KB, "batch atil")) return false;
-  if (!dalloc(a, &t.qbuf, (size_t)t.R * 2 * (t.M + 1) * K, "batch qbuf")) return false;
+  if (!dalloc(a, &t.qbuf, (size_t)q_elems(t.R, K, t.M + 1), "batch qbuf")) return false;
+  t.npix = g->rhost.npix;
+  t.mtmp = nullptr;
+  if (K > 1 && !dalloc(a, &t.mtmp, (size_t)t.npix * K, "batch map-major output")) return false;
   *out = t;
   return true;
 }
@@ -179,6 +182,13 @@ bool make_dev(DevArena &a, const std::vector<LegTables> &tabs,
   BatchRingItem *dri = nullptr;
   if (!upload(a, &dri, ritems, "ring items")) return false;
   d->ritems = dri;
+  std::vector<int64_t> toff(nreq + 1, 0);
+  for (int r = 0; r < nreq; ++r)
+    toff[r + 1] = toff[r] + (tabs[r].K > 1 ? map_transpose_tiles(tabs[r].npix, tabs[r].K) : 0);
+  int64_t *dto = nullptr;
+  if (!upload(a, &dto, toff, "transpose tile offsets")) return false;
+  d->tile_off = dto;
+  d->ntiles = toff[nreq];
   d->twiddle = twiddle;
   if (!dalloc(a, &d->counters, kBatchCounters, "queue heads")) return false;
   void **p = nullptr;
@@ -321,6 +331,7 @@ bool enqueue(wavemoth_batch b, hipStream_t s) {
     WM_TRY(launch_batch_rings(d, b->ring_first[c], b->ring_count[c], 1 + c, b->ring_slots[c],
                               b->ring_blocks[c], s),
            "batch ring stage");
+  WM_TRY(launch_batch_transpose(d, s), "batch map transpose");
   if (!mark(b, 3, s)) return false;
   return true;
 }
@@ -401,7 +412,8 @@ bool build_batch(wavemoth_batch b) {
   for (int r = 0; r < nreq; ++r) {
     const RingHost &h = b->geoms[b->geom[r]]->rhost;
     for (int p = 0; p < (int)h.pairs.size(); ++p)
-      cls[h.need[p] > big_slots ? 0 : 1].push_back({{r, p}, h.cost[p] * b->K[r], h.need[p]});
+      for (int k = 0; k < b->K[r]; ++k)
+        cls[h.need[p] > big_slots ? 0 : 1].push_back({{r, p, k}, h.cost[p], h.need[p]});
   }
   std::vector<BatchRingItem> ritems;
   for (int c = 0; c < 2; ++c) {

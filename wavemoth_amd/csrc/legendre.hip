@@ -242,10 +242,11 @@ __device__ __forceinline__ void legendre_sweep(
     if (rr[i] < R) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        double2 *dst = qbuf + (((int64_t)rr[i] * 2 + p) * Mp1 + m) * K + kbase;
 #pragma unroll
         for (int k = 0; k < KB; ++k)
-          if (kbase + k < K) dst[k] = make_double2(acc[i][p][k][0], acc[i][p][k][1]);
+          if (kbase + k < K)
+            qbuf[q_index(rr[i], p, kbase + k, m, K, Mp1)] =
+                make_double2(acc[i][p][k][0], acc[i][p][k][1]);
       }
     }
   }

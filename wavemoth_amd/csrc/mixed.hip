@@ -92,48 +92,50 @@ __device__ __forceinline__ double bf16_value(unsigned short b) {
 __global__ __launch_bounds__(256) void k_mix_prep(LegTables t, const double2 *__restrict__ alm,
                                                   void *bmat, int NCp, int scheme,
                                                   int64_t part_stride) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= t.nlpad * NCp) return;
-  // thread order follows the output layout (innermost index fastest)
-  int64_t c;
-  int p, i, col;
-  if (scheme == 0) {
-    col = (int)(g % NCp);
-    const int64_t r = g / NCp;  // (c * 2 + p) * 8 + i
-    i = (int)(r & 7);
-    p = (int)((r >> 3) & 1);
-    c = r >> 4;
-  } else {
-    i = (int)(g & 7);
-    const int64_t r = g >> 3;  // (c * 2 + p) * NCp + col
-    col = (int)(r % NCp);
-    p = (int)((r / NCp) & 1);
-    c = r / NCp >> 1;
-  }
-  const int64_t j = c * kChunk + 2 * i + p;
-  const int m = t.chunk_m[c];
-  const int l = m + (int)(j - t.off[m]);
-  const double A = t.ascale[j];
-  const int k = col >> 1;
-  double v = 0.0;
-  if (k < t.K && A != 0.0) {
-    const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + (l - m);
-    const double2 a = alm[idx * t.K + k];
-    v = A * ((col & 1) ? a.y : a.x);
-  }
-  if (scheme == 0) {
-    ((float *)bmat)[g] = (float)v;
-    return;
-  }
-  unsigned short *bq = (unsigned short *)bmat;
-  const unsigned short h = bf16_bits(v);
-  const double r1 = v - bf16_value(h);
-  const unsigned short mid = bf16_bits(r1);
-  bq[g] = h;
-  bq[part_stride + g] = mid;
-  if (scheme == 2) {
-    const double r2 = r1 - bf16_value(mid);
-    bq[2 * part_stride + g] = bf16_bits(r2);
+  const int64_t total = t.nlpad * NCp;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    // thread order follows the output layout (innermost index fastest)
+    int64_t c;
+    int p, i, col;
+    if (scheme == 0) {
+      col = (int)(g % NCp);
+      const int64_t r = g / NCp;  // (c * 2 + p) * 8 + i
+      i = (int)(r & 7);
+      p = (int)((r >> 3) & 1);
+      c = r >> 4;
+    } else {
+      i = (int)(g & 7);
+      const int64_t r = g >> 3;  // (c * 2 + p) * NCp + col
+      col = (int)(r % NCp);
+      p = (int)((r / NCp) & 1);
+      c = r / NCp >> 1;
+    }
+    const int64_t j = c * kChunk + 2 * i + p;
+    const int m = t.chunk_m[c];
+    const int l = m + (int)(j - t.off[m]);
+    const double A = t.ascale[j];
+    const int k = col >> 1;
+    double v = 0.0;
+    if (k < t.K && A != 0.0) {
+      const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + (l - m);
+      const double2 a = alm[idx * t.K + k];
+      v = A * ((col & 1) ? a.y : a.x);
+    }
+    if (scheme == 0) {
+      ((float *)bmat)[g] = (float)v;
+      continue;
+    }
+    unsigned short *bq = (unsigned short *)bmat;
+    const unsigned short h = bf16_bits(v);
+    const double r1 = v - bf16_value(h);
+    const unsigned short mid = bf16_bits(r1);
+    bq[g] = h;
+    bq[part_stride + g] = mid;
+    if (scheme == 2) {
+      const double r2 = r1 - bf16_value(mid);
+      bq[2 * part_stride + g] = bf16_bits(r2);
+    }
   }
 }
 
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
 #undef WM_STORE_B
   }
 
-  // ---- store: q[(ring * 2 + p) * (M + 1) + m][map], Re/Im interleaved
+  // ---- store: column col = 2 map + (Re, Im) of q(ring, p, map, m)
   double *qd_out = (double *)t.qbuf;
 #pragma unroll
   for (int p = 0; p < 2; ++p)
@@ -400,11 +402,11 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
       for (int r = 0; r < 4; ++r) {
         const int ring = ring0 + 16 * rt + 4 * g + r;
         if (ring < R) {
-          double *row = qd_out + (((int64_t)ring * 2 + p) * Mp1 + m) * (2 * (int64_t)K);
 #pragma unroll
           for (int ct = 0; ct < 2; ++ct) {
             const int col = col0 + 16 * ct + i16;
-            if (col < 2 * K) row[col] = qd[p][rt][ct][r];
+            if (col < 2 * K)
+              qd_out[2 * q_index(ring, p, col >> 1, m, K, Mp1) + (col & 1)] = qd[p][rt][ct][r];
           }
         }
       }
@@ -419,8 +421,9 @@ int64_t mix_bmat_elems(const LegTables &t, int NCp) {
 hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, int NCp,
                            int scheme, int64_t part_stride, hipStream_t s) {
   const int64_t total = t.nlpad * NCp;
-  hipLaunchKernelGGL(k_mix_prep, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, t, alm,
-                     bmat, NCp, scheme, part_stride);
+  const int64_t nb = (total + 255) / 256;
+  hipLaunchKernelGGL(k_mix_prep, dim3((unsigned)(nb < 1048576 ? nb : 1048576)), dim3(256), 0,
+                     s, t, alm, bmat, NCp, scheme, part_stride);
   return hipGetLastError();
 }
 

@@ -244,7 +244,10 @@ static bool build_legendre_tables(wavemoth_plan p) {
   } else if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil")) {
     return false;
   }
-  if (!dalloc(p->arena, &t.qbuf, (size_t)t.R * 2 * (p->M + 1) * p->K, "qbuf")) return false;
+  if (!dalloc(p->arena, &t.qbuf, (size_t)q_elems(t.R, p->K, p->M + 1), "qbuf")) return false;
+  t.npix = 12 * (int64_t)p->N * p->N;
+  if (p->K > 1 && !dalloc(p->arena, &t.mtmp, (size_t)t.npix * p->K, "map-major ring output"))
+    return false;
   return true;
 }
 
@@ -655,7 +658,7 @@ int wavemoth_get_work(wavemoth_plan p, double *out) {
     return -1;
   }
   const int R = p->lt.R, M = p->M, K = p->K;
-  std::vector<double2> q((size_t)R * 2 * (M + 1) * K);
+  std::vector<double2> q((size_t)q_elems(R, K, M + 1));
   if (hipMemcpy(q.data(), p->lt.qbuf, q.size() * sizeof(double2), hipMemcpyDeviceToHost) !=
       hipSuccess) {
     set_err("work download failed");
@@ -666,8 +669,7 @@ int wavemoth_get_work(wavemoth_plan p, double *out) {
     for (int odd = 0; odd < 2; ++odd)
       for (int m = 0; m <= M; ++m)
         for (int k = 0; k < K; ++k)
-          o[(((size_t)m * 2 + odd) * R + r) * K + k] =
-              q[(((size_t)r * 2 + odd) * (M + 1) + m) * K + k];
+          o[(((size_t)m * 2 + odd) * R + r) * K + k] = q[q_index(r, odd, k, m, K, M + 1)];
   return 0;
 }
 

@@ -31,17 +31,21 @@ namespace wm {
 
 using namespace fft;
 
-// One ring pair (index into rt.pairs) by the whole block; `lds` holds
-// lds_slots complex slots, the last kTwSlots of them the twiddle table
-// (already loaded).  Used by k_ring_pairs and the batched kernel.
-__device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair,
-                               double2 *lds, int lds_slots, double *__restrict__ map) {
+// Map k of one ring pair (index into rt.pairs) by the whole block; `lds`
+// holds lds_slots complex slots, the last kTwSlots of them the twiddle
+// table (already loaded).  The ring's pixels go to out[pixel] (contiguous:
+// the map itself for one map, else map k's row of the map-major buffer).
+// Used by k_ring_pairs and the batched kernel.
+__device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair, int k,
+                               double2 *lds, int lds_slots, double *__restrict__ outk) {
   double2 *twl = lds + lds_slots - kTwSlots;
   const RingPair rp = rt.pairs[pair];
   const int n = rp.n, h = n / 2, B = h + 1, K = t.K, M = t.M, P = rp.P;
   const bool south = rp.pix_s >= 0;
-  const double2 *qe = t.qbuf + ((int64_t)rp.r * 2 + 0) * (M + 1) * K;
-  const double2 *qo = t.qbuf + ((int64_t)rp.r * 2 + 1) * (M + 1) * K;
+  // q of map k: element m at qe[m * qs] (maps of a group interleaved)
+  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
   const double2 *chirp = rt.chirp + rp.chirp_off;
   const double2 *vhat = rt.vhat + rp.vhat_off;
   const int xoff = (P == 0) ? 0 : (P - h - 1);
@@ -51,7 +55,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
   const int JG = (J + G - 1) / G;  // j-iterations per group
 
   const int nhemi = south ? 2 : 1;
-  for (int k = 0; k < K; ++k) {
+  {
     for (int hemi = 0; hemi < nhemi; ++hemi) {
       // Opaque copy of the thread index: keeps the per-bin LDS/global
       // addresses from being hoisted out of the map/hemisphere loops (they
@@ -59,7 +63,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
       int tid = threadIdx.x;
       asm volatile("" : "+v"(tid));
       const double hs = hemi ? -1.0 : 1.0;  // E + O north, E - O south
-      double *out = map + (hemi ? rp.pix_s : rp.pix_n) * K;
+      double *out = outk + (hemi ? rp.pix_s : rp.pix_n);
       // 1.-2. g_m = E_m +/- O_m staged in LDS, kStage*NT values of m at a
       // time (one round of independent loads), then the signed alias sums
       // per bin: slot (i) -> bin b = slot % B, group g = slot / B; direct
@@ -73,7 +77,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         for (int i = 0; i < kStage; ++i) {
           const int m = mb + tid + NT * i;
           if (m < me) {
-            const double2 e = qe[(int64_t)m * K + k], o = qo[(int64_t)m * K + k];
+            const double2 e = qe[m * qs], o = qo[m * qs];
             lds[pidx(m - mb)] = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
           }
         }
@@ -168,13 +172,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         for (int i = 0; i < kOut; ++i) {
           const int p = tid + NT * i;
           if (p < h) {
-            const double2 z = lds[pidx(p)];
-            if (K == 1) {
-              *reinterpret_cast<double2 *>(out + 2 * p) = z;
-            } else {
-              out[(int64_t)(2 * p) * K + k] = z.x;
-              out[(int64_t)(2 * p + 1) * K + k] = z.y;
-            }
+            *reinterpret_cast<double2 *>(out + 2 * p) = lds[pidx(p)];
           }
         }
       } else {
@@ -196,13 +194,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         for (int i = 0; i < kOut; ++i) {
           const int p = tid + NT * i;
           if (p < h) {
-            const double2 z = cmul(lds[pidx(p)], chirp[p]);
-            if (K == 1) {
-              *reinterpret_cast<double2 *>(out + 2 * p) = z;
-            } else {
-              out[(int64_t)(2 * p) * K + k] = z.x;
-              out[(int64_t)(2 * p + 1) * K + k] = z.y;
-            }
+            *reinterpret_cast<double2 *>(out + 2 * p) = cmul(lds[pidx(p)], chirp[p]);
           }
         }
       }
@@ -211,11 +203,87 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
   }
 }
 
+// Block -> (ring pair, map).  The G = q_group(K) maps of one q group share
+// its 128-byte lines, so their G workgroups are put on one XCD (hardware
+// dispatches block b to XCD b % 8) at consecutive dispatch slots: block
+// b = 8 (sigma G + j) + x runs map j of group gamma = 8 sigma + x, where
+// gamma enumerates (ring pair i, map group).
 __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, int first,
-                                                   int lds_slots, double *__restrict__ map) {
+                                                   int count, int lds_slots,
+                                                   double *__restrict__ map) {
   extern __shared__ double2 lds[];
+  const int K = t.K, G = q_group(K), nG = q_groups(K);
+  const int b = blockIdx.x, x = b & 7, s = b >> 3;
+  const int sigma = s / G, j = s - sigma * G;
+  const int gamma = sigma * 8 + x;
+  const int i = gamma / nG, k = (gamma - i * nG) * G + j;
+  if (i >= count || k >= K) return;
   load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
-  ring_pair_body(t, rt, rt.order[first + blockIdx.x], lds, lds_slots, map);
+  ring_pair_body(t, rt, rt.order[first + i], k, lds, lds_slots,
+                 K == 1 ? map : t.mtmp + (int64_t)k * t.npix);
+}
+
+// (npix, K) map from the map-major [K][npix] ring output: 32 x 32 tiles
+// through LDS, reads coalesced along pixels, writes along maps.
+constexpr int kTT = 32;
+__device__ __forceinline__ void transpose_tile(const double *__restrict__ src,
+                                               double *__restrict__ dst, int64_t npix, int K,
+                                               int64_t tile, double (*sh)[kTT + 1]) {
+  const int64_t tp = (npix + kTT - 1) / kTT;
+  const int64_t p0 = (tile % tp) * kTT;
+  const int k0 = (int)(tile / tp) * kTT;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+  for (int j = 0; j < kTT; j += 8) {
+    const int kk = ty + j;
+    const int64_t p = p0 + tx;
+    if (k0 + kk < K && p < npix) sh[kk][tx] = src[(int64_t)(k0 + kk) * npix + p];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kTT; j += 8) {
+    const int pp = ty + j;
+    if (p0 + pp < npix && k0 + tx < K) dst[(p0 + pp) * K + k0 + tx] = sh[tx][pp];
+  }
+  __syncthreads();
+}
+
+int64_t map_transpose_tiles(int64_t npix, int K) {
+  return ((npix + kTT - 1) / kTT) * ((K + kTT - 1) / kTT);
+}
+
+// Grid-stride over tiles (a grid is limited to 2^32 threads).
+constexpr int kTransposeBlocks = 8192;
+
+__global__ __launch_bounds__(256) void k_map_transpose(const double *__restrict__ src,
+                                                       double *__restrict__ dst, int64_t npix,
+                                                       int K, int64_t ntiles) {
+  __shared__ double sh[kTT][kTT + 1];
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+    transpose_tile(src, dst, npix, K, tile, sh);
+}
+
+__global__ __launch_bounds__(256) void k_batch_transpose(BatchDev b) {
+  __shared__ double sh[kTT][kTT + 1];
+  for (int64_t g = blockIdx.x; g < b.ntiles; g += gridDim.x) {
+    int lo = 0, hi = b.nreq - 1;  // tile_off[lo] <= g < tile_off[lo + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (b.tile_off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const LegTables &t = b.tabs[lo];
+    transpose_tile(t.mtmp, b.map[lo], t.npix, t.K, g - b.tile_off[lo], sh);
+  }
+}
+
+static unsigned transpose_grid(int64_t ntiles) {
+  return (unsigned)(ntiles < kTransposeBlocks ? ntiles : kTransposeBlocks);
+}
+
+hipError_t launch_batch_transpose(const BatchDev &b, hipStream_t s) {
+  if (b.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_transpose, dim3(transpose_grid(b.ntiles)), dim3(256), 0, s, b);
+  return hipGetLastError();
 }
 
 // Persistent ring stage of a batch: each block takes ring pairs of any
@@ -232,8 +300,9 @@ __global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int c
     __syncthreads();
     if (it >= count) break;
     const BatchRingItem item = b.ritems[first + it];
-    ring_pair_body(b.tabs[item.req], b.rtabs[item.req], item.pair, lds, slots,
-                   b.map[item.req]);
+    const LegTables &t = b.tabs[item.req];
+    ring_pair_body(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
+                   t.K == 1 ? b.map[item.req] : t.mtmp + (int64_t)item.k * t.npix);
   }
 }
 
@@ -333,10 +402,18 @@ hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *m
   for (int i = 0; i < rt.nlaunch; ++i) {
     if (rt.count[i] == 0) continue;
     const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
-    hipLaunchKernelGGL(k_ring_pairs, dim3(rt.count[i]), dim3(NT), lds, s, t, rt, rt.first[i],
-                       rt.lds[i], map);
+    const int64_t groups = (int64_t)rt.count[i] * q_groups(t.K);
+    const int64_t nb = (groups + 7) / 8 * 8 * q_group(t.K);
+    hipLaunchKernelGGL(k_ring_pairs, dim3((unsigned)nb), dim3(NT), lds, s, t, rt, rt.first[i],
+                       rt.count[i], rt.lds[i], map);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+  }
+  if (t.K > 1) {
+    const int64_t nt = map_transpose_tiles(t.npix, t.K);
+    hipLaunchKernelGGL(k_map_transpose, dim3(transpose_grid(nt)), dim3(256), 0, s, t.mtmp, map,
+                       t.npix, t.K, nt);
+    return hipGetLastError();
   }
   return hipSuccess;
 }

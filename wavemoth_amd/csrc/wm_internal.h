@@ -42,8 +42,27 @@ struct LegTables {
   int *wstart = nullptr;      // [(M+1)*W] min seed_l over a wave's rings
   int *wseedmax = nullptr;    // [(M+1)*W] max seed_l (<= lmax) over a wave
   double2 *atil = nullptr;    // [ceil(K/KB)][nlpad][KB] prescaled a_lm (per execute)
-  double2 *qbuf = nullptr;    // [R][2][M+1][K] even/odd Legendre sums
+  double2 *qbuf = nullptr;    // [R][2][K][M+1] even/odd Legendre sums (q_index)
+  double *mtmp = nullptr;     // [K][npix] map-major ring-stage output (K > 1 only)
+  int64_t npix = 0;
 };
+
+// q(ring pair r, parity p, map k, m) lives at
+//   [r][p][k / G][m][k % G],  G = q_group(K) = min(K, 8),
+// with K padded to a multiple of G.  A group of G maps of one m is one
+// 16G-byte row (a full 128-byte line for G = 8), which the Legendre kernels
+// write whole; the ring stage reads map k of a group with stride G, and the
+// G ring workgroups of a group run side by side on one XCD so they share
+// the lines in its L2.  For K = 1 the layout is [R][2][M+1].
+__host__ __device__ inline int q_group(int K) { return K < 8 ? K : 8; }
+__host__ __device__ inline int q_groups(int K) { return (K + q_group(K) - 1) / q_group(K); }
+__host__ __device__ inline int64_t q_index(int r, int p, int k, int m, int K, int Mp1) {
+  const int G = q_group(K);
+  return ((((int64_t)r * 2 + p) * q_groups(K) + k / G) * Mp1 + m) * G + k % G;
+}
+__host__ __device__ inline int64_t q_elems(int R, int K, int Mp1) {
+  return (int64_t)R * 2 * q_groups(K) * q_group(K) * Mp1;
+}
 
 // One ring pair of the output map (northern ring + its southern mirror),
 // for the ring (phase + alias + c2r) stage.
@@ -82,7 +101,7 @@ struct BatchLegItem {
   int req, m, w, kb;  // request, m, ring chunk, map block
 };
 struct BatchRingItem {
-  int req, pair;
+  int req, pair, k;  // request, ring pair, map
 };
 constexpr int kBatchCounters = 4;  // queue heads: sweep, ring class 0, ring class 1
 struct BatchDev {
@@ -97,6 +116,8 @@ struct BatchDev {
   int nlitems = 0;
   const BatchRingItem *ritems = nullptr;
   const double2 *twiddle = nullptr;     // [kMaxFFT] shared twiddle table
+  const int64_t *tile_off = nullptr;    // [nreq + 1] prefix of transpose tiles (K > 1)
+  int64_t ntiles = 0;
   int *counters = nullptr;              // [kBatchCounters] work-queue heads
 };
 
@@ -113,6 +134,9 @@ hipError_t launch_batch_legendre(const BatchDev &b, int nblocks, hipStream_t s);
 hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
                               int nblocks, hipStream_t s);
 hipError_t batch_rings_set_lds_limit(int slots);
+// Map-major ring output -> (npix, K) map of every request with K > 1.
+hipError_t launch_batch_transpose(const BatchDev &b, hipStream_t s);
+int64_t map_transpose_tiles(int64_t npix, int K);
 hipError_t batch_legendre_occupancy(int *blocks_per_cu);
 hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu);
 
