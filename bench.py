@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["fp64", "batch", "mixed"], default="fp64")
     ap.add_argument("--mixed-nmaps", type=int, default=64)
-    ap.add_argument("--schemes", default="f32,bf16x3,bf16x6")
+    ap.add_argument("--schemes", default="f32,f16x3,bf16x3,bf16x6")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="mixed workload: timed steps of the FP64 reference path")
     ap.add_argument("--nside", type=int, default=2048)
@@ -298,7 +298,8 @@ def run_batch(args, dev, res):
 
 SCHEMES = {"f32": (0x1000, "FP32 MFMA", 157.3, 1),
            "bf16x3": (0x2000, "BF16x3", 2500.0, 3),
-           "bf16x6": (0x4000, "BF16x6", 2500.0, 6)}
+           "bf16x6": (0x4000, "BF16x6", 2500.0, 6),
+           "f16x3": (0x8000, "FP16x3", 2500.0, 3)}
 
 
 def _time_plan(plan, alm, out, warmup, steps):
@@ -395,7 +396,7 @@ def run_mixed(args, dev, res):
             "bound": "mfma", "achieved": m["contraction_tflops"],
             "peak": SCHEMES[main_key][2], "unit": "TFLOP/s",
             "frac": m["mfma_frac"], "traffic": None, "kernel": "k_mix<%d>" %
-            ["f32", "bf16x3", "bf16x6"].index(main_key),
+            ["f32", "bf16x3", "bf16x6", "f16x3"].index(main_key),
             "flops_per_launch": 4.0 * K * T_eps,
             "flop_convention": "4*nmaps per (l, m, ring-pair) triple at or after first l "
                                "(the contraction; the FP64 recurrence is not counted)",

@@ -50,10 +50,13 @@ typedef struct _wavemoth_plan *wavemoth_plan;
      WAVEMOTH_MIXED_F32     FP32 inputs, FP32 MFMA;
      WAVEMOTH_MIXED_BF16X3  Lambda and a split into two bf16 parts each,
                             three products (hi*hi + hi*lo + lo*hi);
-     WAVEMOTH_MIXED_BF16X6  three bf16 parts each, six products. */
+     WAVEMOTH_MIXED_BF16X6  three bf16 parts each, six products;
+     WAVEMOTH_MIXED_F16X3   two fp16 parts each, three products, with
+                            power-of-two scales keeping fp16's range. */
 #define WAVEMOTH_MIXED_F32 0x1000
 #define WAVEMOTH_MIXED_BF16X3 0x2000
 #define WAVEMOTH_MIXED_BF16X6 0x4000
+#define WAVEMOTH_MIXED_F16X3 0x8000
 
 /* ---- stable driver API (replaces src/wavemoth.h:33-46) ---------------- */
 
@@ -136,7 +139,7 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
    the plan, 7 flags, 8 legendre-kernel launches per execute, 9 device bytes
    held by the plan, 10 (l, ring-pair) slots the Legendre sweep evaluates
    including its wave/chunk padding, 11 contraction scheme (-1 FP64,
-   0 MIXED_F32, 1 MIXED_BF16X3, 2 MIXED_BF16X6).  Returns -1 for an
+   0 MIXED_F32, 1 MIXED_BF16X3, 2 MIXED_BF16X6, 3 MIXED_F16X3).  Returns -1 for an
    unknown key (key 11 of an FP64 plan is also -1). */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 

@@ -4,6 +4,7 @@ law and flat C_l, with the error bound of each scheme written here:
   FP32 MFMA  rel rms <= 1e-6
   BF16x6     rel rms <= 1e-6
   BF16x3     rel rms <= 5e-5 (two bf16 parts carry ~16 bits)
+  FP16x3     rel rms <= 1e-6 (two fp16 parts carry ~22 bits; scaled)
 plus the oracle at a small size, determinism, mmax < lmax and nmaps not a
 multiple of the column block."""
 import numpy as np
@@ -11,9 +12,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-F32, BF3, BF6 = 0x1000, 0x2000, 0x4000
-BOUND = {F32: 1e-6, BF6: 1e-6, BF3: 5e-5}
-NAME = {F32: "FP32", BF3: "BF16x3", BF6: "BF16x6"}
+F32, BF3, BF6, F16 = 0x1000, 0x2000, 0x4000, 0x8000
+BOUND = {F32: 1e-6, BF6: 1e-6, BF3: 5e-5, F16: 1e-6}
+NAME = {F32: "FP32", BF3: "BF16x3", BF6: "BF16x6", F16: "FP16x3"}
 
 
 def _maps(N, L, M, K, flags, alm):
@@ -34,7 +35,7 @@ def _err(x, ref):
     return rms, mx
 
 
-@pytest.mark.parametrize("scheme", [F32, BF3, BF6])
+@pytest.mark.parametrize("scheme", [F32, BF3, BF6, F16])
 @pytest.mark.parametrize("N,L,M,K,flat", [(32, 64, 64, 16, False), (64, 128, 128, 32, False),
                                           (64, 128, 128, 32, True), (16, 40, 20, 3, False),
                                           (128, 256, 256, 17, False)])
@@ -46,7 +47,7 @@ def test_scheme_vs_fp64(scheme, N, L, M, K, flat):
     alm = random_alm(L, M, K, 123 + N + K, dev, flat=flat)
     ref, _ = _maps(N, L, M, K, 0, alm)
     got, p = _maps(N, L, M, K, scheme, alm)
-    assert p.info()["scheme"] == {F32: 0, BF3: 1, BF6: 2}[scheme]
+    assert p.info()["scheme"] == {F32: 0, BF3: 1, BF6: 2, F16: 3}[scheme]
     rms, mx = _err(got, ref)
     print("rel rms %.3e  max/rms %.3e" % (rms, mx), flush=True)
     assert rms <= BOUND[scheme]
@@ -64,7 +65,7 @@ def test_fp32_scheme_vs_oracle(oracle):
     assert err <= 1e-6
 
 
-@pytest.mark.parametrize("scheme", [F32, BF6])
+@pytest.mark.parametrize("scheme", [F32, BF6, F16])
 def test_scheme_deterministic(scheme):
     import torch
     from wavemoth_amd.torch_sht import DevicePlan, random_alm
@@ -78,3 +79,19 @@ def test_scheme_deterministic(scheme):
     p.synthesize_into(alm, b)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("amp", [1e-12, 1.0, 1e9])
+def test_fp16_scheme_scale_invariant(amp):
+    """The FP16 scheme's column scales follow the data: the relative error
+    does not depend on the amplitude of a_lm (fp16 alone spans 6e-8..65504)."""
+    import torch
+    from wavemoth_amd.torch_sht import random_alm
+    dev = torch.device("cuda:0")
+    N, L, K = 32, 64, 16
+    alm = random_alm(L, L, K, 31, dev) * amp
+    ref, _ = _maps(N, L, L, K, 0, alm)
+    got, _ = _maps(N, L, L, K, F16, alm)
+    rms, mx = _err(got, ref)
+    print("amp %g rel rms %.3e" % (amp, rms), flush=True)
+    assert rms <= 1e-6

@@ -52,6 +52,7 @@ WAVEMOTH_BATCH_NO_GRAPH = 0x200
 WAVEMOTH_MIXED_F32 = 0x1000
 WAVEMOTH_MIXED_BF16X3 = 0x2000
 WAVEMOTH_MIXED_BF16X6 = 0x4000
+WAVEMOTH_MIXED_F16X3 = 0x8000
 
 
 def _bind_torch_runtime():

@@ -23,13 +23,22 @@
 //   0 FP32   v_mfma_f32_16x16x4_f32 on fp32 Lambda and a.
 //   1 BF16x3 Lambda = h + l, a = h + l (bf16 parts): hh + hl + lh.
 //   2 BF16x6 three bf16 parts each: hh + hm + mh + hl + mm + lh.
-// BF16 schemes use v_mfma_f32_16x16x32_bf16.
+//   3 FP16x3 Lambda and a split into two fp16 parts each (~22 bits):
+//            hh + hl + lh.  fp16's range is kept by power-of-two scales:
+//            Lambda (the true value, A_l times the monic one) by 2^e with
+//            |Lambda| <= sqrt((2 lmax + 1) / 4 pi), each a_lm column by a
+//            power of two from its maximum (found per execute), both undone
+//            when the FP32 sums are added into FP64.
+// BF16 schemes use v_mfma_f32_16x16x32_bf16, FP16x3 v_mfma_f32_16x16x32_f16.
+#include <cmath>
+
 #include "wm_internal.h"
 
 namespace wm {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 #define WM_CONST __attribute__((address_space(4)))
 template <typename T>
@@ -70,7 +79,54 @@ struct MixScheme<2> {
   __device__ static constexpr int pb(int i) { return i == 0 ? 0 : (i == 1 ? 1 : (i == 2 ? 0 : (i == 3 ? 2 : (i == 4 ? 1 : 0)))); }
 };
 
-int mix_parts(int scheme) { return scheme == 0 ? 1 : (scheme == 1 ? 2 : 3); }
+template <>
+struct MixScheme<3> {
+  static constexpr int parts = 2;
+  static constexpr int nprod = 3;
+  __device__ static constexpr int pa(int i) { return i == 2 ? 1 : 0; }
+  __device__ static constexpr int pb(int i) { return i == 1 ? 1 : 0; }
+};
+
+int mix_parts(int scheme) { return scheme == 0 ? 1 : (scheme == 2 ? 3 : 2); }
+
+// Scale of a_lm column c for the FP16 scheme: 2^(13 - floor(log2 max|a|)),
+// so the column's largest value lands in [2^13, 2^14).
+__device__ __forceinline__ double col_scale(const unsigned long long *colmax, int c) {
+  const double mx = __builtin_bit_cast(double, colmax[c]);
+  return mx > 0.0 ? ldexp(1.0, 13 - ilogb(mx)) : 1.0;
+}
+
+// Per-column max |a| (bit patterns of non-negative doubles order as
+// integers): block-level max in LDS, then one global atomic per column.
+__global__ __launch_bounds__(256) void k_mix_colmax_clear(unsigned long long *colmax, int NCp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < NCp) colmax[c] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_mix_colmax(const double *__restrict__ alm,
+                                                    int64_t nrows, int C2,
+                                                    unsigned long long *colmax) {
+  extern __shared__ unsigned long long smax[];
+  for (int c = threadIdx.x; c < C2; c += blockDim.x) smax[c] = 0ull;
+  __syncthreads();
+  const int64_t per = (nrows + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = blockIdx.x * per;
+  const int64_t r1 = r0 + per < nrows ? r0 + per : nrows;
+  for (int64_t e = r0 * C2 + threadIdx.x; e < r1 * C2; e += blockDim.x) {
+    const int c = (int)(e % C2);
+    atomicMax(&smax[c], __builtin_bit_cast(unsigned long long, fabs(alm[e])));
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C2; c += blockDim.x)
+    if (smax[c]) atomicMax(&colmax[c], smax[c]);
+}
+
+__device__ __forceinline__ unsigned short f16_bits(double v) {
+  return __builtin_bit_cast(unsigned short, (_Float16)(float)v);
+}
+__device__ __forceinline__ double f16_value(unsigned short b) {
+  return (double)(float)__builtin_bit_cast(_Float16, b);
+}
 
 // ------------------------------------------------------------- prescale
 
@@ -91,7 +147,8 @@ __device__ __forceinline__ double bf16_value(unsigned short b) {
 
 __global__ __launch_bounds__(256) void k_mix_prep(LegTables t, const double2 *__restrict__ alm,
                                                   void *bmat, int NCp, int scheme,
-                                                  int64_t part_stride) {
+                                                  int64_t part_stride,
+                                                  const unsigned long long *colmax) {
   const int64_t total = t.nlpad * NCp;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
        g += (int64_t)gridDim.x * blockDim.x) {
@@ -120,7 +177,15 @@ __global__ __launch_bounds__(256) void k_mix_prep(LegTables t, const double2 *__
     if (k < t.K && A != 0.0) {
       const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + (l - m);
       const double2 a = alm[idx * t.K + k];
-      v = A * ((col & 1) ? a.y : a.x);
+      // FP16: the true a, column-scaled (Lambda carries A_l); else A_l a
+      v = (scheme == 3 ? col_scale(colmax, col) : A) * ((col & 1) ? a.y : a.x);
+    }
+    if (scheme == 3) {
+      unsigned short *bq = (unsigned short *)bmat;
+      const unsigned short h = f16_bits(v);
+      bq[g] = h;
+      bq[part_stride + g] = f16_bits(v - f16_value(h));
+      continue;
     }
     if (scheme == 0) {
       ((float *)bmat)[g] = (float)v;
@@ -154,7 +219,8 @@ struct MixLds {
 
 template <int S>
 __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, int NCp, int ncb,
-                                                int bpm, int64_t part_stride) {
+                                                int bpm, int64_t part_stride, double lam_scale,
+                                                const unsigned long long *colmax) {
   using Sc = MixScheme<S>;
   using Ly = MixLds<S>;
   constexpr int P = Sc::parts;
@@ -192,6 +258,7 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
     const int64_t offm = t.off[m];
     const WM_CONST double *ct_ = cview(t.ctil + (offm - m));
     const WM_CONST double *cs_ = cview(t.cscale + offm / kChunk);
+    const WM_CONST double *as_ = cview(t.ascale + (offm - m));  // A_l, 0 past lmax
     const int nch = (L - m + kChunk) / kChunk;  // chunks of m's padded l-range
 
     // producer lane: ring ring0 + (lane & 31), parity lane >> 5
@@ -239,6 +306,15 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
     if constexpr (S == 2) *(uint4 *)(bdst + 2 * Ly::b_part) = b2;                   \
   } while (0)
 
+    // undoes the FP16 scheme's scales of this lane's two columns (exact)
+    double unscale[2] = {1.0, 1.0};
+    if constexpr (S == 3) {
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const int col = col0 + 16 * ct + i16;
+        unscale[ct] = 1.0 / (lam_scale * col_scale(colmax, col < NCp ? col : 0));
+      }
+    }
     f32x4 acc[2][2][2];
     auto zero_acc = [&]() {
 #pragma unroll
@@ -256,7 +332,8 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
 #pragma unroll
           for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) qd[p][rt][ct][r] += (double)acc[p][rt][ct][r];
+            for (int r = 0; r < 4; ++r)
+              qd[p][rt][ct][r] += unscale[ct] * (double)acc[p][rt][ct][r];
       zero_acc();
     };
     zero_acc();
@@ -290,7 +367,10 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
             const double n2 = fma(x, cur, -(ct_[lc + j + 1] * prev));
             prev = cur;
             cur = n2;
-            f[j >> 1] = (float)(pr ? d1 : d0);
+            if constexpr (S == 3)
+              f[j >> 1] = (float)(lam_scale * (pr ? d1 * as_[lc + j + 1] : d0 * as_[lc + j]));
+            else
+              f[j >> 1] = (float)(pr ? d1 : d0);
           }
           const double sc = cs_[(lc - m) / kChunk];
           prev *= sc;
@@ -305,8 +385,21 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
           *(f32x4 *)dst = (f32x4){f[0], f[1], f[2], f[3]};
           *(f32x4 *)(dst + 4) = (f32x4){f[4], f[5], f[6], f[7]};
         } else {
-          // parts packed two bf16 per dword: wh (high), wm (middle), wl (low)
+          // parts packed two 16-bit values per dword: wh (high), wm (middle),
+          // wl (low)
           unsigned wh[4], wm[4], wl[4];
+          if constexpr (S == 3) {
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+              const _Float16 h0 = (_Float16)f[i], h1 = (_Float16)f[i + 1];
+              const _Float16 m0 = (_Float16)(f[i] - (float)h0);
+              const _Float16 m1 = (_Float16)(f[i + 1] - (float)h1);
+              wh[i >> 1] = __builtin_bit_cast(unsigned short, h0) |
+                           ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
+              wm[i >> 1] = __builtin_bit_cast(unsigned short, m0) |
+                           ((unsigned)__builtin_bit_cast(unsigned short, m1) << 16);
+            }
+          } else
 #pragma unroll
           for (int i = 0; i < 8; i += 2) {
             const __bf16 h0 = (__bf16)f[i], h1 = (__bf16)f[i + 1];
@@ -377,9 +470,15 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-              for (int ct = 0; ct < 2; ++ct)
-                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    a[Sc::pa(pi)][rt], bv[Sc::pb(pi)][ct], acc[p][rt][ct], 0, 0, 0);
+              for (int ct = 0; ct < 2; ++ct) {
+                if constexpr (S == 3)
+                  acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                      __builtin_bit_cast(f16x8, a[Sc::pa(pi)][rt]),
+                      __builtin_bit_cast(f16x8, bv[Sc::pb(pi)][ct]), acc[p][rt][ct], 0, 0, 0);
+                else
+                  acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                      a[Sc::pa(pi)][rt], bv[Sc::pb(pi)][ct], acc[p][rt][ct], 0, 0, 0);
+              }
         }
       }
       if (++nsc == kFlush) {
@@ -419,23 +518,33 @@ int64_t mix_bmat_elems(const LegTables &t, int NCp) {
 }
 
 hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, int NCp,
-                           int scheme, int64_t part_stride, hipStream_t s) {
+                           int scheme, int64_t part_stride, unsigned long long *colmax,
+                           hipStream_t s) {
+  if (scheme == 3) {
+    hipLaunchKernelGGL(k_mix_colmax_clear, dim3((NCp + 255) / 256), dim3(256), 0, s, colmax,
+                       NCp);
+    const int64_t nalm = (int64_t)(t.M + 1) * (t.L + 1) - (int64_t)t.M * (t.M + 1) / 2;
+    const int C2 = 2 * t.K;
+    hipLaunchKernelGGL(k_mix_colmax, dim3(1024), dim3(256), (size_t)C2 * 8, s,
+                       (const double *)alm, nalm, C2, colmax);
+  }
   const int64_t total = t.nlpad * NCp;
   const int64_t nb = (total + 255) / 256;
   hipLaunchKernelGGL(k_mix_prep, dim3((unsigned)(nb < 1048576 ? nb : 1048576)), dim3(256), 0,
-                     s, t, alm, bmat, NCp, scheme, part_stride);
+                     s, t, alm, bmat, NCp, scheme, part_stride, (const unsigned long long *)colmax);
   return hipGetLastError();
 }
 
 template <int S>
 static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
-                               int64_t part_stride, hipStream_t s) {
+                               int64_t part_stride, double lam_scale,
+                               const unsigned long long *colmax, hipStream_t s) {
   const int ncb = NCp / kMixCols;
   const int bpm = t.W * ncb;
   const int mgroups = (t.M + 1 + 7) / 8;
   const unsigned grid = (unsigned)(8 * mgroups * bpm);
   hipLaunchKernelGGL(k_mix<S>, dim3(grid), dim3(256), (size_t)MixLds<S>::bytes, s, t, bmat, NCp,
-                     ncb, bpm, part_stride);
+                     ncb, bpm, part_stride, lam_scale, colmax);
   return hipGetLastError();
 }
 
@@ -448,16 +557,28 @@ hipError_t mix_set_lds_limits() {
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void *)k_mix<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             MixLds<2>::bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_mix<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            MixLds<3>::bytes);
   return e;
 }
 
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
-                      int64_t part_stride, hipStream_t s) {
+                      int64_t part_stride, double lam_scale, const unsigned long long *colmax,
+                      hipStream_t s) {
   switch (scheme) {
-    case 0: return launch_mix_t<0>(t, bmat, NCp, part_stride, s);
-    case 1: return launch_mix_t<1>(t, bmat, NCp, part_stride, s);
-    default: return launch_mix_t<2>(t, bmat, NCp, part_stride, s);
+    case 0: return launch_mix_t<0>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
+    case 1: return launch_mix_t<1>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
+    case 2: return launch_mix_t<2>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
+    default: return launch_mix_t<3>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
   }
+}
+
+// Power-of-two scale of Lambda for the FP16 scheme: |Lambda_lm| is at most
+// sqrt((2 lmax + 1) / (4 pi)); the scaled maximum stays below 2^15.
+double mix_lambda_scale(int lmax) {
+  const double bound = std::sqrt((2.0 * lmax + 1.0) / (4.0 * 3.14159265358979323846));
+  return std::ldexp(1.0, (int)std::floor(std::log2(32768.0 / bound)));
 }
 
 int mix_cols_per_block() { return kMixCols; }

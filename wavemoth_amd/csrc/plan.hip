@@ -53,6 +53,8 @@ struct _wavemoth_plan {
   int mix_ncp = 0;          // padded real columns
   int64_t mix_pstride = 0;  // elements per B part
   void *mix_b = nullptr;
+  unsigned long long *mix_colmax = nullptr;  // FP16 scheme: per-column max |a|
+  double mix_lam_scale = 1.0;
   LegCounts cnt;
   int64_t T_exec = 0;
   DevArena arena;
@@ -239,6 +241,8 @@ static bool build_legendre_tables(wavemoth_plan p) {
     WM_TRY(hipMemsetAsync(b, 0, bytes, p->stream), "mixed B clear");
     WM_TRY(hipStreamSynchronize(p->stream), "mixed B clear");
     p->mix_b = b;
+    if (!dalloc(p->arena, &p->mix_colmax, (size_t)p->mix_ncp, "column maxima")) return false;
+    p->mix_lam_scale = mix_lambda_scale(p->L);
     WM_TRY(mix_set_lds_limits(), "mixed kernel LDS limit");
     if (!dalloc(p->arena, &t.atil, 1, "atil")) return false;
   } else if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil")) {
@@ -440,10 +444,12 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
     WM_TRY(launch_wave_bounds(p->lt, s), "wave-bounds kernel");
   }
   if (p->mix >= 0) {
-    WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, s),
+    WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride,
+                           p->mix_colmax, s),
            "mixed prescale kernel");
     if (!mark(p, 1, s)) return false;
-    WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, s),
+    WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, p->mix_lam_scale,
+                      p->mix_colmax, s),
            "mixed contraction kernel");
   } else {
     WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");
@@ -548,12 +554,19 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
   p->h_out = output;
   p->fly = (flags & WAVEMOTH_SEEDS_ON_THE_FLY) != 0;
   {
-    const unsigned mf = flags & (WAVEMOTH_MIXED_F32 | WAVEMOTH_MIXED_BF16X3 | WAVEMOTH_MIXED_BF16X6);
+    const unsigned mf = flags & (WAVEMOTH_MIXED_F32 | WAVEMOTH_MIXED_BF16X3 |
+                                 WAVEMOTH_MIXED_BF16X6 | WAVEMOTH_MIXED_F16X3);
     if (mf == WAVEMOTH_MIXED_F32) p->mix = 0;
     else if (mf == WAVEMOTH_MIXED_BF16X3) p->mix = 1;
     else if (mf == WAVEMOTH_MIXED_BF16X6) p->mix = 2;
+    else if (mf == WAVEMOTH_MIXED_F16X3) p->mix = 3;
     else if (mf != 0) {
       set_err("at most one WAVEMOTH_MIXED_* flag");
+      delete p;
+      return nullptr;
+    }
+    if (p->mix == 3 && nmaps > 4096) {
+      set_err("WAVEMOTH_MIXED_F16X3 supports nmaps <= 4096");
       delete p;
       return nullptr;
     }

@@ -141,7 +141,7 @@ hipError_t batch_legendre_occupancy(int *blocks_per_cu);
 hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu);
 
 // Mixed-precision contraction (matrix cores) of the Legendre stage.
-// scheme 0 FP32 MFMA, 1 BF16x3, 2 BF16x6; B operand `bmat` of
+// scheme 0 FP32 MFMA, 1 BF16x3, 2 BF16x6, 3 FP16x3; B operand `bmat` of
 // mix_bmat_elems(t, NCp) elements per part (NCp = padded real columns, a
 // multiple of mix_cols_per_block()); needs t.RL == mix_rings_per_lane_table().
 int mix_parts(int scheme);
@@ -149,10 +149,14 @@ int mix_cols_per_block();
 int mix_rings_per_lane_table();
 int64_t mix_bmat_elems(const LegTables &t, int NCp);
 hipError_t mix_set_lds_limits();
+// colmax: [NCp] scratch of the FP16 scheme (scheme 3), unused otherwise.
 hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, int NCp,
-                           int scheme, int64_t part_stride, hipStream_t s);
+                           int scheme, int64_t part_stride, unsigned long long *colmax,
+                           hipStream_t s);
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
-                      int64_t part_stride, hipStream_t s);
+                      int64_t part_stride, double lam_scale, const unsigned long long *colmax,
+                      hipStream_t s);
+double mix_lambda_scale(int lmax);
 
 // Shape of the Legendre sweep for nmaps = K.
 int legendre_map_block(int K);
