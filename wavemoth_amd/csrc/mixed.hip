@@ -7,12 +7,12 @@
 // (src/legendre_transform.c.in:389-451, 453-618) for many maps.
 //
 // Workgroup = 4 waves = 128 ring pairs x 32 columns (16 maps) of one m.
-// Wave w owns rings 32w .. 32w+31: lanes 0-31 and 32-63 both run the FP64
-// three-term recurrence of ring (lane & 31) (the monic form of the FP64
-// kernel, same seeds and chunk rescaling) and keep the even-l (lanes 0-31)
-// or odd-l (lanes 32-63) values.  Every 64 l (a super-chunk) each lane
-// writes its 32 values, converted to the scheme's input format, into a
-// wave-private LDS tile; the wave then reads MFMA A-fragments from it.  The
+// Waves 2j and 2j+1 take rings 64j .. 64j+63 (one per lane) and the even /
+// odd l, stepping the FP64 recurrence two l at a time (the x^2 form; same
+// monic scaling, seeds and chunk rescale as the FP64 kernel).  Every 64 l
+// (a super-chunk) each lane writes its 32 values, converted to the scheme's
+// input format, into a wave-private LDS tile; the wave then reads MFMA
+// A-fragments from it.  The
 // B operand (prescaled a_lm, pre-split at prescale time) is staged in LDS
 // per workgroup, prefetched one super-chunk ahead.  MFMA accumulates in
 // FP32; every kFlush super-chunks the FP32 sums are added into FP64
@@ -208,19 +208,27 @@ __global__ __launch_bounds__(256) void k_mix_prep(LegTables t, const double2 *__
 
 template <int S>
 struct MixLds {
-  // A tiles: per wave, part, parity: 32 rings x (32 values + pad)
+  // A tile per wave and part: 64 rings x (32 values of the wave's parity + pad)
   static constexpr int a_row = (S == 0) ? (kKK + 4) : (kKK + 8);  // elements
   static constexpr int a_elem = (S == 0) ? 4 : 2;                   // bytes
-  static constexpr int a_wave = MixScheme<S>::parts * 2 * kMixRW * a_row * a_elem;
+  static constexpr int a_part = 64 * a_row;                         // elements
+  static constexpr int a_wave = MixScheme<S>::parts * a_part * a_elem;
   // B tile: per part, 4 chunks x 2 parities x 8 l x 32 cols
   static constexpr int b_part = 4 * 2 * 8 * kMixCols * ((S == 0) ? 4 : 2);
   static constexpr int bytes = kMixWaves * a_wave + MixScheme<S>::parts * b_part;
 };
 
+// Workgroup = 4 waves over 128 ring pairs x 32 columns of one m: wave w
+// takes rings 64 (w >> 1) .. +63 (one per lane) and the l of parity w & 1,
+// advanced two at a time by L~_{l+2} = (x^2 - d_l) L~_l - e_l L~_{l-2}
+// (d_l = c~_l + c~_{l+1}, e_l = c~_l c~_{l-1}; the x^2 form of
+// src/legendre_transform.c.in:680-716), so every coefficient is
+// wave-uniform and no lane repeats another's work.
 template <int S>
 __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, int NCp, int ncb,
                                                 int bpm, int64_t part_stride, double lam_scale,
-                                                const unsigned long long *colmax) {
+                                                const unsigned long long *colmax,
+                                                const double *dtil, const double *etil) {
   using Sc = MixScheme<S>;
   using Ly = MixLds<S>;
   constexpr int P = Sc::parts;
@@ -235,47 +243,47 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
   const int item = s8 % bpm;
   const int cb = item / t.W, wblk = item - cb * t.W;
   const int L = t.L, R = t.R, Mp1 = t.M + 1, K = t.K;
+  const int pw = wv & 1;                            // parity of this wave
+  const int ring0 = wblk * kMixRB + (wv >> 1) * 64;  // first ring of this wave
 
   // output mapping of the 16x16 MFMA tiles: row 4 g + r, column i16
   const int i16 = lane & 15, g = lane >> 4;
-  const int ring0 = wblk * kMixRB + wv * kMixRW;
   const int col0 = cb * kMixCols;
 
   int l0 = __builtin_amdgcn_readfirstlane(t.wstart[(int64_t)m * t.W + wblk]);
   const int lsmax = __builtin_amdgcn_readfirstlane(t.wseedmax[(int64_t)m * t.W + wblk]);
 
-  double qd[2][2][2][4];  // [parity][row tile][col tile][r]
+  double qd[4][2][4];  // [row tile][col tile][r]
 #pragma unroll
-  for (int p = 0; p < 2; ++p)
+  for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) qd[p][rt][ct][r] = 0.0;
+      for (int r = 0; r < 4; ++r) qd[rt][ct][r] = 0.0;
 
   if (l0 <= L) {
     const int64_t offm = t.off[m];
     const WM_CONST double *ct_ = cview(t.ctil + (offm - m));
+    const WM_CONST double *dt_ = cview(dtil + (offm - m));
+    const WM_CONST double *et_ = cview(etil + (offm - m));
     const WM_CONST double *cs_ = cview(t.cscale + offm / kChunk);
     const WM_CONST double *as_ = cview(t.ascale + (offm - m));  // A_l, 0 past lmax
     const int nch = (L - m + kChunk) / kChunk;  // chunks of m's padded l-range
 
-    // producer lane: ring ring0 + (lane & 31), parity lane >> 5
-    const int pr = lane >> 5;
-    const int rr = ring0 + (lane & 31);
+    const int rr = ring0 + lane;
     const bool rok = rr < R;
     const double x = rok ? t.ring_x[rr] : 0.0;
+    const double x2 = x * x;
     const int sl = rok ? t.seed_l[(int64_t)m * R + rr] : kSeedNone;
-    double prev = 0.0, cur = 0.0;
+    double u = 0.0, v = 0.0;  // L~ at the next two l of parity pw
 
     unsigned char *abase = lds + wv * Ly::a_wave;
     unsigned char *bbase = lds + kMixWaves * Ly::a_wave;
 
-    // B staging: thread -> 16 (BF16, per part) or 32 (FP32) bytes per
+    // B staging: thread -> 16 (BF16/FP16, per part) or 32 (FP32) bytes per
     // super-chunk, prefetched into b0..b2 one super-chunk ahead
     const int tid = threadIdx.x;
-    const int brow = (S == 0) ? (tid >> 2) : (tid >> 5);  // FP32: (cc, p, i); BF16: (cc, p)
+    const int brow = (S == 0) ? (tid >> 2) : (tid >> 5);  // FP32: (cc, p, i); 16-bit: (cc, p)
     const int bcol = (S == 0) ? 8 * (tid & 3) : (tid & 31);
     const char *bsrc0;
     if constexpr (S == 0)
@@ -315,25 +323,20 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
         unscale[ct] = 1.0 / (lam_scale * col_scale(colmax, col < NCp ? col : 0));
       }
     }
-    f32x4 acc[2][2][2];
+    f32x4 acc[4][2];
     auto zero_acc = [&]() {
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-          for (int ct = 0; ct < 2; ++ct) acc[p][rt][ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
     };
     auto flush = [&]() {
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              qd[p][rt][ct][r] += unscale[ct] * (double)acc[p][rt][ct][r];
+          for (int r = 0; r < 4; ++r) qd[rt][ct][r] += unscale[ct] * (double)acc[rt][ct][r];
       zero_acc();
     };
     zero_acc();
@@ -346,51 +349,52 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
       __syncthreads();
       if (l0 + kSC <= L) WM_LOAD_B(l0 + kSC);
 
-      // ---- producer: 4 chunks of 16 l, this lane's parity into LDS
+      // ---- producer: 4 chunks of 16 l; this lane's ring, the wave's parity
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
         const int lc = l0 + kChunk * cc;
         float f[8];
         if ((lc - m) / kChunk < nch) {
           if (lc <= lsmax && sl == lc) {
-            const double2 v = t.seed_v[(int64_t)m * R + rr];
-            prev = v.x;
-            cur = v.y;
+            // three-term steps from the seed (L~_{lc-1}, L~_{lc}) to this
+            // parity's pair: (L~_lc, L~_lc+2) or (L~_lc+1, L~_lc+3)
+            const double2 sv = t.seed_v[(int64_t)m * R + rr];
+            const double t1 = fma(x, sv.y, -(ct_[lc] * sv.x));
+            const double t2 = fma(x, t1, -(ct_[lc + 1] * sv.y));
+            const double t3 = fma(x, t2, -(ct_[lc + 2] * t1));
+            u = pw ? t1 : sv.y;
+            v = pw ? t3 : t2;
           }
 #pragma unroll
-          for (int j = 0; j < kChunk; j += 2) {
-            const double d0 = cur;
-            const double n1 = fma(x, cur, -(ct_[lc + j] * prev));
-            prev = cur;
-            cur = n1;
-            const double d1 = cur;
-            const double n2 = fma(x, cur, -(ct_[lc + j + 1] * prev));
-            prev = cur;
-            cur = n2;
+          for (int i = 0; i < 8; ++i) {
+            const int l = lc + pw + 2 * i;  // wave-uniform
             if constexpr (S == 3)
-              f[j >> 1] = (float)(lam_scale * (pr ? d1 * as_[lc + j + 1] : d0 * as_[lc + j]));
+              f[i] = (float)(lam_scale * (u * as_[l]));
             else
-              f[j >> 1] = (float)(pr ? d1 : d0);
+              f[i] = (float)u;
+            const double w = fma(x2 - dt_[l + 2], v, -(et_[l + 2] * u));
+            u = v;
+            v = w;
           }
           const double sc = cs_[(lc - m) / kChunk];
-          prev *= sc;
-          cur *= sc;
+          u *= sc;
+          v *= sc;
         } else {
 #pragma unroll
           for (int i = 0; i < 8; ++i) f[i] = 0.f;
         }
-        // write the 8 values of this chunk: A[q][pr][ring][8 cc + i]
+        // write the 8 values of this chunk: A[q][ring][8 cc + i]
         if constexpr (S == 0) {
-          float *dst = (float *)abase + (pr * kMixRW + (lane & 31)) * Ly::a_row + 8 * cc;
+          float *dst = (float *)abase + lane * Ly::a_row + 8 * cc;
           *(f32x4 *)dst = (f32x4){f[0], f[1], f[2], f[3]};
           *(f32x4 *)(dst + 4) = (f32x4){f[4], f[5], f[6], f[7]};
         } else {
           // parts packed two 16-bit values per dword: wh (high), wm (middle),
           // wl (low)
           unsigned wh[4], wm[4], wl[4];
-          if constexpr (S == 3) {
 #pragma unroll
-            for (int i = 0; i < 8; i += 2) {
+          for (int i = 0; i < 8; i += 2) {
+            if constexpr (S == 3) {
               const _Float16 h0 = (_Float16)f[i], h1 = (_Float16)f[i + 1];
               const _Float16 m0 = (_Float16)(f[i] - (float)h0);
               const _Float16 m1 = (_Float16)(f[i + 1] - (float)h1);
@@ -398,88 +402,77 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
                            ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
               wm[i >> 1] = __builtin_bit_cast(unsigned short, m0) |
                            ((unsigned)__builtin_bit_cast(unsigned short, m1) << 16);
-            }
-          } else
-#pragma unroll
-          for (int i = 0; i < 8; i += 2) {
-            const __bf16 h0 = (__bf16)f[i], h1 = (__bf16)f[i + 1];
-            const float r0 = f[i] - (float)h0, r1 = f[i + 1] - (float)h1;
-            const __bf16 m0 = (__bf16)r0, m1 = (__bf16)r1;
-            wh[i >> 1] = __builtin_bit_cast(unsigned short, h0) |
-                         ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
-            wm[i >> 1] = __builtin_bit_cast(unsigned short, m0) |
-                         ((unsigned)__builtin_bit_cast(unsigned short, m1) << 16);
-            if constexpr (S == 2) {
-              const __bf16 q0 = (__bf16)(r0 - (float)m0), q1 = (__bf16)(r1 - (float)m1);
-              wl[i >> 1] = __builtin_bit_cast(unsigned short, q0) |
-                           ((unsigned)__builtin_bit_cast(unsigned short, q1) << 16);
+            } else {
+              const __bf16 h0 = (__bf16)f[i], h1 = (__bf16)f[i + 1];
+              const float r0 = f[i] - (float)h0, r1 = f[i + 1] - (float)h1;
+              const __bf16 m0 = (__bf16)r0, m1 = (__bf16)r1;
+              wh[i >> 1] = __builtin_bit_cast(unsigned short, h0) |
+                           ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
+              wm[i >> 1] = __builtin_bit_cast(unsigned short, m0) |
+                           ((unsigned)__builtin_bit_cast(unsigned short, m1) << 16);
+              if constexpr (S == 2) {
+                const __bf16 q0 = (__bf16)(r0 - (float)m0), q1 = (__bf16)(r1 - (float)m1);
+                wl[i >> 1] = __builtin_bit_cast(unsigned short, q0) |
+                             ((unsigned)__builtin_bit_cast(unsigned short, q1) << 16);
+              }
             }
           }
-          unsigned short *dst = (unsigned short *)abase +
-                                (pr * kMixRW + (lane & 31)) * Ly::a_row + 8 * cc;
-          constexpr int qstride = 2 * kMixRW * Ly::a_row;  // elements between parts
+          unsigned short *dst = (unsigned short *)abase + lane * Ly::a_row + 8 * cc;
           *(uint4 *)dst = make_uint4(wh[0], wh[1], wh[2], wh[3]);
-          *(uint4 *)(dst + qstride) = make_uint4(wm[0], wm[1], wm[2], wm[3]);
+          *(uint4 *)(dst + Ly::a_part) = make_uint4(wm[0], wm[1], wm[2], wm[3]);
           if constexpr (S == 2)
-            *(uint4 *)(dst + 2 * qstride) = make_uint4(wl[0], wl[1], wl[2], wl[3]);
+            *(uint4 *)(dst + 2 * Ly::a_part) = make_uint4(wl[0], wl[1], wl[2], wl[3]);
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-      // ---- consumer: MFMA over the super-chunk
+      // ---- consumer: MFMA over the super-chunk (this wave's parity)
       if constexpr (S == 0) {
         const float *af = (const float *)abase;
         const float *bf = (const float *)bbase;
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int s = 0; s < 8; ++s) {
+          float a[4], bv[2];
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            float a[2], bv[2];
+          for (int rt = 0; rt < 4; ++rt) a[rt] = af[(16 * rt + i16) * Ly::a_row + 4 * s + g];
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt) a[rt] = af[(p * kMixRW + 16 * rt + i16) * Ly::a_row + 4 * s + g];
+          for (int ct = 0; ct < 2; ++ct)
+            bv[ct] = bf[((((s >> 1) * 2 + pw) * 8) + 4 * (s & 1) + g) * kMixCols + 16 * ct + i16];
+#pragma unroll
+          for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
             for (int ct = 0; ct < 2; ++ct)
-              bv[ct] = bf[((((s >> 1) * 2 + p) * 8) + 4 * (s & 1) + g) * kMixCols + 16 * ct + i16];
-#pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-              for (int ct = 0; ct < 2; ++ct)
-                acc[p][rt][ct] =
-                    __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[ct], acc[p][rt][ct], 0, 0, 0);
-          }
+              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[ct], acc[rt][ct], 0, 0, 0);
         }
       } else {
+        bf16x8 a[P][4], bv[P][2];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          bf16x8 a[P][2], bv[P][2];
+        for (int q = 0; q < P; ++q) {
 #pragma unroll
-          for (int q = 0; q < P; ++q) {
+          for (int rt = 0; rt < 4; ++rt)
+            a[q][rt] = *(const bf16x8 *)((const unsigned short *)abase + q * Ly::a_part +
+                                         (16 * rt + i16) * Ly::a_row + 8 * g);
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
-              a[q][rt] = *(const bf16x8 *)((const unsigned short *)abase +
-                                           ((q * 2 + p) * kMixRW + 16 * rt + i16) * Ly::a_row + 8 * g);
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-              bv[q][ct] = *(const bf16x8 *)(bbase + q * Ly::b_part +
-                                            (((g * 2 + p) * kMixCols) + 16 * ct + i16) * 16);
-          }
-#pragma unroll
-          for (int pi = 0; pi < Sc::nprod; ++pi)
-#pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-              for (int ct = 0; ct < 2; ++ct) {
-                if constexpr (S == 3)
-                  acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                      __builtin_bit_cast(f16x8, a[Sc::pa(pi)][rt]),
-                      __builtin_bit_cast(f16x8, bv[Sc::pb(pi)][ct]), acc[p][rt][ct], 0, 0, 0);
-                else
-                  acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                      a[Sc::pa(pi)][rt], bv[Sc::pb(pi)][ct], acc[p][rt][ct], 0, 0, 0);
-              }
+          for (int ct = 0; ct < 2; ++ct)
+            bv[q][ct] = *(const bf16x8 *)(bbase + q * Ly::b_part +
+                                          (((g * 2 + pw) * kMixCols) + 16 * ct + i16) * 16);
         }
+#pragma unroll
+        for (int pi = 0; pi < Sc::nprod; ++pi)
+#pragma unroll
+          for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+              if constexpr (S == 3)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                    __builtin_bit_cast(f16x8, a[Sc::pa(pi)][rt]),
+                    __builtin_bit_cast(f16x8, bv[Sc::pb(pi)][ct]), acc[rt][ct], 0, 0, 0);
+              else
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a[Sc::pa(pi)][rt], bv[Sc::pb(pi)][ct], acc[rt][ct], 0, 0, 0);
+            }
       }
       if (++nsc == kFlush) {
         flush();
@@ -491,24 +484,22 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
 #undef WM_STORE_B
   }
 
-  // ---- store: column col = 2 map + (Re, Im) of q(ring, p, map, m)
+  // ---- store: column col = 2 map + (Re, Im) of q(ring, pw, map, m)
   double *qd_out = (double *)t.qbuf;
 #pragma unroll
-  for (int p = 0; p < 2; ++p)
+  for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int r = 0; r < 4; ++r) {
+      const int ring = ring0 + 16 * rt + 4 * g + r;
+      if (ring < R) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ring = ring0 + 16 * rt + 4 * g + r;
-        if (ring < R) {
-#pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
-            const int col = col0 + 16 * ct + i16;
-            if (col < 2 * K)
-              qd_out[2 * q_index(ring, p, col >> 1, m, K, Mp1) + (col & 1)] = qd[p][rt][ct][r];
-          }
+        for (int ct = 0; ct < 2; ++ct) {
+          const int col = col0 + 16 * ct + i16;
+          if (col < 2 * K)
+            qd_out[2 * q_index(ring, pw, col >> 1, m, K, Mp1) + (col & 1)] = qd[rt][ct][r];
         }
       }
+    }
 }
 
 // ------------------------------------------------------------- launchers
@@ -538,13 +529,14 @@ hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, i
 template <int S>
 static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
                                int64_t part_stride, double lam_scale,
-                               const unsigned long long *colmax, hipStream_t s) {
+                               const unsigned long long *colmax, const double *dtil,
+                               const double *etil, hipStream_t s) {
   const int ncb = NCp / kMixCols;
   const int bpm = t.W * ncb;
   const int mgroups = (t.M + 1 + 7) / 8;
   const unsigned grid = (unsigned)(8 * mgroups * bpm);
   hipLaunchKernelGGL(k_mix<S>, dim3(grid), dim3(256), (size_t)MixLds<S>::bytes, s, t, bmat, NCp,
-                     ncb, bpm, part_stride, lam_scale, colmax);
+                     ncb, bpm, part_stride, lam_scale, colmax, dtil, etil);
   return hipGetLastError();
 }
 
@@ -565,12 +557,13 @@ hipError_t mix_set_lds_limits() {
 
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
                       int64_t part_stride, double lam_scale, const unsigned long long *colmax,
-                      hipStream_t s) {
+                      const double *dtil, const double *etil, hipStream_t s) {
   switch (scheme) {
-    case 0: return launch_mix_t<0>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
-    case 1: return launch_mix_t<1>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
-    case 2: return launch_mix_t<2>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
-    default: return launch_mix_t<3>(t, bmat, NCp, part_stride, lam_scale, colmax, s);
+    case 0: return launch_mix_t<0>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 1: return launch_mix_t<1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 2: return launch_mix_t<2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    default:
+      return launch_mix_t<3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
   }
 }
 

@@ -54,6 +54,7 @@ struct _wavemoth_plan {
   int64_t mix_pstride = 0;  // elements per B part
   void *mix_b = nullptr;
   unsigned long long *mix_colmax = nullptr;  // FP16 scheme: per-column max |a|
+  double *mix_dtil = nullptr, *mix_etil = nullptr;  // x^2-form recurrence coefficients
   double mix_lam_scale = 1.0;
   LegCounts cnt;
   int64_t T_exec = 0;
@@ -242,6 +243,25 @@ static bool build_legendre_tables(wavemoth_plan p) {
     WM_TRY(hipStreamSynchronize(p->stream), "mixed B clear");
     p->mix_b = b;
     if (!dalloc(p->arena, &p->mix_colmax, (size_t)p->mix_ncp, "column maxima")) return false;
+    {
+      // d_l = c~_l + c~_{l+1}, e_l = c~_l c~_{l-1}, c~_l = (l^2 - m^2)/(4 l^2 - 1)
+      std::vector<int64_t> offh(p->M + 1);
+      WM_TRY(hipMemcpy(offh.data(), t.off, offh.size() * sizeof(int64_t), hipMemcpyDeviceToHost),
+             "off download");
+      std::vector<double> dt(t.nlpad + 16, 0.0), et(t.nlpad + 16, 0.0);
+      auto ctl = [](double l, double m) { return (l * l - m * m) / (4.0 * l * l - 1.0); };
+      for (int m = 0; m <= p->M; ++m) {
+        const int64_t lp = (p->L - m + kChunk) / kChunk * kChunk;
+        const double dm = (double)m;
+        for (int64_t jj = 0; jj < lp; ++jj) {
+          const double l = dm + (double)jj;
+          dt[offh[m] + jj] = ctl(l, dm) + ctl(l + 1.0, dm);
+          et[offh[m] + jj] = (jj == 0) ? 0.0 : ctl(l, dm) * ctl(l - 1.0, dm);
+        }
+      }
+      if (!upload(p->arena, &p->mix_dtil, dt, "mixed d table")) return false;
+      if (!upload(p->arena, &p->mix_etil, et, "mixed e table")) return false;
+    }
     p->mix_lam_scale = mix_lambda_scale(p->L);
     WM_TRY(mix_set_lds_limits(), "mixed kernel LDS limit");
     if (!dalloc(p->arena, &t.atil, 1, "atil")) return false;
@@ -449,7 +469,7 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
            "mixed prescale kernel");
     if (!mark(p, 1, s)) return false;
     WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, p->mix_lam_scale,
-                      p->mix_colmax, s),
+                      p->mix_colmax, p->mix_dtil, p->mix_etil, s),
            "mixed contraction kernel");
   } else {
     WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");
