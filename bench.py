@@ -2,7 +2,14 @@
 """Throughput of a_lm -> HEALPix synthesis on one MI355X.
 
 Workloads (--workload):
-  fp64   (default) one step = one FP64 synthesis (Legendre sweep + ring
+  mixed  (default) Nside = 2048, lmax = 4096, nmaps = 64: the FP64 path and
+         each mixed-precision contraction scheme (FP32 MFMA, FP16x3, BF16x3,
+         BF16x6) on the same a_lm; one step = one synthesis of all 64 maps.
+         Reports each scheme's maps/s and its rms / max error against the
+         FP64 map for C_l = (l+1)^-2 and for flat C_l = 1, and the FP64
+         path's maps/s at nmaps = 64 and at nmaps = 1.  `value` is the
+         FP32-MFMA scheme's maps/s.
+  fp64   one step = one FP64 synthesis (Legendre sweep + ring
          stage) of an a_lm set already resident in HBM into a map in HBM,
          through the library's device-pointer execute.  Nside = 2048,
          lmax = mmax = 4096, nmaps = 1, seeded power-law a_lm
@@ -10,19 +17,14 @@ Workloads (--workload):
   batch  one step = one execute of a batch of 96 uneven requests (Nside
          uniform over {128 .. 2048}, lmax = mmax = 2 Nside, nmaps over
          {1, 3, 8}, seeded), FP64, replayed from the batch's HIP graph.
-  mixed  Nside = 2048, lmax = 4096, nmaps = 64: the FP64 path and each
-         mixed-precision contraction scheme (FP32 MFMA, BF16x3, BF16x6) on
-         the same a_lm; one step = one synthesis of all 64 maps.  Reports
-         each scheme's maps/s and its rms / max error against the FP64 map
-         for C_l = (l+1)^-2 and for flat C_l = 1; `value` is the FP32-MFMA
-         scheme's maps/s.
 
-Prints one JSON line.  `roofline` prices the Legendre sweep (the dominant
-kernel) against the FP64 peak with the reference's flop convention
-(5 + 4 nmaps flops per evaluated (l, m, ring-pair) triple); its duration is
-measured with HIP events recorded on the execute stream around the kernel.
-`cpu_baseline` (fp64 workload) times the CPU oracle (a C restatement of the
-reference algorithm, OpenMP) on a bounded sample of the same workload.
+Prints one JSON line.  `roofline` prices the dominant kernel: for fp64 and
+batch the Legendre sweep against the FP64 peak with the reference's flop
+convention (5 + 4 nmaps flops per evaluated (l, m, ring-pair) triple); for
+mixed the FP32-MFMA contraction (4 nmaps flops per triple) against the FP32
+matrix peak.  Kernel durations are HIP events recorded on the execute stream
+around the kernel.  `cpu_baseline` times the CPU oracle (a C restatement of
+the reference algorithm, OpenMP) on a bounded sample of the same workload.
 --calibrate-copy MIB runs a device copy of that size first (untimed), as
 the known-size kernel against which HBM byte counters are scaled.
 """
@@ -44,7 +46,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["fp64", "batch", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=["fp64", "batch", "mixed"], default="mixed")
     ap.add_argument("--mixed-nmaps", type=int, default=64)
     ap.add_argument("--schemes", default="f32,f16x3,bf16x3,bf16x6")
     ap.add_argument("--fp64-steps", type=int, default=3,
@@ -395,8 +397,12 @@ def run_mixed(args, dev, res):
         "roofline": {
             "bound": "mfma", "achieved": m["contraction_tflops"],
             "peak": SCHEMES[main_key][2], "unit": "TFLOP/s",
-            "frac": m["mfma_frac"], "traffic": None, "kernel": "k_mix<%d>" %
-            ["f32", "bf16x3", "bf16x6", "f16x3"].index(main_key),
+            "frac": m["mfma_frac"],
+            "traffic": measured_traffic("k_mix<%d> nside=%d lmax=%d nmaps=%d" % (
+                ["f32", "bf16x3", "bf16x6", "f16x3"].index(main_key), N, L, K)),
+            "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                              "x2.0 / x1.0 from a 1 GiB copy)",
+            "kernel": "k_mix<%d>" % ["f32", "bf16x3", "bf16x6", "f16x3"].index(main_key),
             "flops_per_launch": 4.0 * K * T_eps,
             "flop_convention": "4*nmaps per (l, m, ring-pair) triple at or after first l "
                                "(the contraction; the FP64 recurrence is not counted)",
