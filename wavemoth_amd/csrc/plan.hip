@@ -357,10 +357,16 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       return false;
     }
   }
-  // Two launch groups: rings whose LDS need exceeds 80 KiB (one block per
-  // CU) and the rest (two or more per CU); costliest first within each.
+  // Launch groups: rings whose LDS need exceeds 80 KiB (one block per CU),
+  // non-aliasing power-of-two rings (the belt kernel), the rest; costliest
+  // first within each.
   const int big_slots = (80 * 1024) / (int)sizeof(double2);
-  auto cls = [&](int r) { return need[r] > big_slots ? 1 : 0; };
+  std::vector<char> beltr(R, 0);
+  for (int r = 0; r < R; ++r) {
+    const int hh = pairs[r].n / 2;
+    if (pairs[r].P == 0 && M <= hh && hh <= ring_belt_max_h()) beltr[r] = 1;
+  }
+  auto cls = [&](int r) { return beltr[r] ? 1 : (need[r] > big_slots ? 2 : 0); };
   std::vector<int> order(R);
   for (int i = 0; i < R; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -379,6 +385,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     const int g = rt.nlaunch - 1;
     rt.count[g]++;
     rt.lds[g] = std::max(rt.lds[g], need[order[i]]);
+    rt.belt[g] = beltr[order[i]];
     max_need = std::max(max_need, need[order[i]]);
   }
   WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");

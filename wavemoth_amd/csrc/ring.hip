@@ -117,34 +117,32 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         }
         __syncthreads();
       }
-      // 3. phase e^{i pi s b / n}, c2r semantics for DC / Nyquist, into LDS.
-      // The phases of a thread's bins b = tid + NT i follow by angle
-      // addition from two sincospi evaluations (error ~ kBins ulp).
-      {
-        const double2 step = expipi((int64_t)rp.s * NT, n);
-        double2 w = expipi((int64_t)rp.s * tid, n);
+      // 3. the alias sums (unphased) into LDS
 #pragma unroll
-        for (int i = 0; i < kBins; ++i) {
-          const int b = tid + NT * i;
-          if (b < B) {
-            double2 x = cmul(xv[i], w);
-            if (b == 0 || b == h) x.y = 0.0;
-            lds[pidx(xoff + b)] = x;
-          }
-          w = cmul(w, step);
-        }
+      for (int i = 0; i < kBins; ++i) {
+        const int b = tid + NT * i;
+        if (b < B) lds[pidx(xoff + b)] = xv[i];
       }
       __syncthreads();
-      // 4. Z pairs (a, h - a), twiddle e^{2 pi i a / n} by angle addition
+      // 4. per pair (a, h - a): phases e^{i pi s b / n} (angle addition for a,
+      // e^{i pi s / 2} conj(.) for h - a), c2r semantics for DC / Nyquist,
+      // then the Z pair with twiddle e^{2 pi i a / n} (angle addition).  The
+      // belt-ring body computes the same expressions.
       {
         const double2 step = expipi(2 * NT, n);
         double2 om = expipi(2 * tid, n);
+        const double2 pstep = expipi((int64_t)rp.s * NT, n);
+        double2 w = expipi((int64_t)rp.s * tid, n);
+        const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
 #pragma unroll
         for (int i = 0; i < kPairs; ++i) {
           const int a = tid + NT * i;
           if (a <= h / 2) {
             const int c = h - a;
-            const double2 xa = lds[pidx(xoff + a)], xc = lds[pidx(xoff + c)];
+            double2 xa = cmul(lds[pidx(xoff + a)], w);
+            if (a == 0) xa.y = 0.0;
+            double2 xc = cmul(lds[pidx(xoff + c)], c == h ? wh : cmul(wh, cconj(w)));
+            if (c == h) xc.y = 0.0;
             const double2 za =
                 cadd(cadd(xa, cconj(xc)), cmul_i(cmul(om, csub(xa, cconj(xc)))));
             const bool both = (a != 0) && (a != c);
@@ -162,6 +160,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
             }
           }
           om = cmul(om, step);
+          w = cmul(w, pstep);
         }
       }
       __syncthreads();
@@ -201,6 +200,109 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
       __syncthreads();
     }
   }
+}
+
+// Belt rings without aliasing (power-of-two h = n/2 and mmax <= h): bin b
+// takes the single term m = b (and bin h also its conjugate), so each thread
+// reads its two bins a, h - a of E +/- O straight from q, phases them and
+// forms the pre-twiddled Z pair in one step; no staging, no alias
+// accumulators.  Same arithmetic per bin as ring_pair_body (bitwise equal
+// results), with a 4096-point bound on the inlined FFT.
+constexpr int kBeltMaxH = kMaxFFT / 2;
+__device__ __forceinline__ void ring_belt_body(const LegTables &t, const RingTables &rt,
+                                               int pair, int k, double2 *lds, int lds_slots,
+                                               double *__restrict__ outk) {
+  double2 *twl = lds + lds_slots - kTwSlots;
+  const RingPair rp = rt.pairs[pair];
+  const int n = rp.n, h = n / 2, K = t.K, M = t.M;
+  const bool south = rp.pix_s >= 0;
+  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+  constexpr int kP = (kBeltMaxH / 2 + 1 + NT - 1) / NT;  // Z pairs per thread
+  constexpr int kO = (kBeltMaxH + NT - 1) / NT;          // outputs per thread
+  const int nhemi = south ? 2 : 1;
+  for (int hemi = 0; hemi < nhemi; ++hemi) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const double hs = hemi ? -1.0 : 1.0;  // E + O north, E - O south
+    double *out = outk + (hemi ? rp.pix_s : rp.pix_n);
+    // phases e^{i pi s b / n} and twiddles e^{2 pi i a / n} by angle addition
+    const double2 pstep = expipi((int64_t)rp.s * NT, n);
+    double2 w = expipi((int64_t)rp.s * tid, n);
+    const double2 ostep = expipi(2 * NT, n);
+    double2 om = expipi(2 * tid, n);
+    // e^{i pi s h / n} = e^{i pi s / 2}
+    const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < kP; ++i) {
+      const int a = tid + NT * i;
+      if (a <= h / 2) {
+        const int c = h - a;
+        double2 ga = make_double2(0.0, 0.0), gc = make_double2(0.0, 0.0);
+        if (a <= M) {
+          const double2 e = qe[a * qs], o = qo[a * qs];
+          ga = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
+        }
+        if (c <= M) {
+          const double2 e = qe[c * qs], o = qo[c * qs];
+          gc = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
+        }
+        // X_a = w_a g_a (DC: real part only); X_c = w_c g_c with
+        // w_c = e^{i pi s / 2} conj(w_a); bin h also takes its conjugate term
+        double2 xa = cmul(ga, w);
+        if (a == 0) xa.y = 0.0;
+        const double2 wc = cmul(wh, cconj(w));
+        double2 xc;
+        if (c == h) {
+          // X_h = e^{i pi s h/n} [g_h + (-1)^s conj(g_h)], real part only
+          const double sg = rp.s ? -1.0 : 1.0;
+          xc = cmul(make_double2(fma(sg, gc.x, gc.x), fma(-sg, gc.y, gc.y)), wh);
+          xc.y = 0.0;
+        } else {
+          xc = cmul(gc, wc);
+        }
+        const double2 za = cadd(cadd(xa, cconj(xc)), cmul_i(cmul(om, csub(xa, cconj(xc)))));
+        lds[pidx(a)] = za;
+        if (a != 0 && a != c) {
+          const double2 oc = make_double2(-om.x, om.y);  // e^{2 pi i c/n}
+          lds[pidx(c)] = cadd(cadd(xc, cconj(xa)), cmul_i(cmul(oc, csub(xc, cconj(xa)))));
+        }
+      }
+      w = cmul(w, pstep);
+      om = cmul(om, ostep);
+    }
+    __syncthreads();
+    lds_fft_inl<true, kBeltMaxH>(lds, h, twl);
+#pragma unroll
+    for (int i = 0; i < kO; ++i) {
+      const int p = tid + NT * i;
+      if (p < h) *reinterpret_cast<double2 *>(out + 2 * p) = lds[pidx(p)];
+    }
+    __syncthreads();
+  }
+}
+
+// Block -> (ring pair, map), as k_ring_pairs below.
+__device__ __forceinline__ bool ring_block_item(int b, int K, int count, int *i, int *k) {
+  const int G = q_group(K), nG = q_groups(K);
+  const int x = b & 7, s = b >> 3;
+  const int sigma = s / G, j = s - sigma * G;
+  const int gamma = sigma * 8 + x;
+  *i = gamma / nG;
+  *k = (gamma - *i * nG) * G + j;
+  return *i < count && *k < K;
+}
+
+__global__ __launch_bounds__(NT) void k_ring_belt(LegTables t, RingTables rt, int first,
+                                                  int count, int lds_slots,
+                                                  double *__restrict__ map) {
+  extern __shared__ double2 lds[];
+  int i, k;
+  if (!ring_block_item(blockIdx.x, t.K, count, &i, &k)) return;
+  load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
+  ring_belt_body(t, rt, rt.order[first + i], k, lds, lds_slots,
+                 t.K == 1 ? map : t.mtmp + (int64_t)k * t.npix);
 }
 
 // Block -> (ring pair, map).  The G = q_group(K) maps of one q group share
@@ -390,12 +492,18 @@ hipError_t ring_set_lds_limit(int slots) {
   static int cur = 0;
   std::lock_guard<std::mutex> lock(mu);
   if (slots <= cur) return hipSuccess;
-  const hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)((size_t)slots * sizeof(double2)));
+  const int bytes = (int)((size_t)slots * sizeof(double2));
+  hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_ring_belt, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
   if (e == hipSuccess) cur = slots;
   return e;
 }
+
+int ring_belt_lds_slots(int n) { return pidx(n / 2) + 1 + kTwSlots; }
+int ring_belt_max_h() { return kBeltMaxH; }
 
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
                              hipStream_t s) {
@@ -404,8 +512,12 @@ hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *m
     const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
     const int64_t groups = (int64_t)rt.count[i] * q_groups(t.K);
     const int64_t nb = (groups + 7) / 8 * 8 * q_group(t.K);
-    hipLaunchKernelGGL(k_ring_pairs, dim3((unsigned)nb), dim3(NT), lds, s, t, rt, rt.first[i],
-                       rt.count[i], rt.lds[i], map);
+    if (rt.belt[i])
+      hipLaunchKernelGGL(k_ring_belt, dim3((unsigned)nb), dim3(NT), lds, s, t, rt, rt.first[i],
+                         rt.count[i], rt.lds[i], map);
+    else
+      hipLaunchKernelGGL(k_ring_pairs, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
+                         rt.first[i], rt.count[i], rt.lds[i], map);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -85,11 +85,11 @@ static __device__ __forceinline__ double2 twiddle(const double2 *twl, int t) {
   return cmul(twl[t >> 6], twl[kTwC + (t & 63)]);
 }
 
-// One Stockham pass of radix R over P points held in LDS (padded).
-template <int R, bool INV>
+// One Stockham pass of radix R over P <= MAXP points held in LDS (padded).
+template <int R, bool INV, int MAXP = kMaxFFT>
 static __device__ __forceinline__ void stockham_pass(double2 *buf, int P, int Ns,
                                               const double2 *twl) {
-  constexpr int B = (kMaxFFT / R + NT - 1) / NT;
+  constexpr int B = (MAXP / R + NT - 1) / NT;
   const int tid = threadIdx.x;
   const int nb = P / R;
   const int tstep = kMaxFFT / (Ns * R);
@@ -152,6 +152,27 @@ static __device__ void lds_fft(double2 *buf, int P, const double2 *twl) {
       Ns *= 4;
     } else {
       stockham_pass<2, INV>(buf, P, Ns, twl);
+      Ns *= 2;
+    }
+  }
+}
+
+// The same FFT inlined, for P <= MAXP: the register footprint of the
+// passes scales with MAXP, and a call would pin the caller's register
+// budget at the callee's.
+template <bool INV, int MAXP>
+static __device__ __forceinline__ void lds_fft_inl(double2 *buf, int P, const double2 *twl) {
+  int Ns = 1;
+  while (Ns < P) {
+    const int rem = P / Ns;
+    if (rem >= 8) {
+      stockham_pass<8, INV, MAXP>(buf, P, Ns, twl);
+      Ns *= 8;
+    } else if (rem == 4) {
+      stockham_pass<4, INV, MAXP>(buf, P, Ns, twl);
+      Ns *= 4;
+    } else {
+      stockham_pass<2, INV, MAXP>(buf, P, Ns, twl);
       Ns *= 2;
     }
   }

@@ -92,6 +92,7 @@ struct RingTables {
   int first[kMaxLaunches] = {0};
   int count[kMaxLaunches] = {0};
   int lds[kMaxLaunches] = {0};
+  int belt[kMaxLaunches] = {0};  // 1: group of non-aliasing power-of-two rings
 };
 
 // Batched execution of many independent transforms: prescale, Legendre
@@ -174,6 +175,9 @@ hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const i
                                  const int64_t *vhoff, const int *plist, int nh,
                                  hipStream_t s);
 int ring_lds_slots(int n, int P, int M);
+// Belt-ring kernel (power-of-two n/2 <= ring_belt_max_h(), mmax <= n/2).
+int ring_belt_lds_slots(int n);
+int ring_belt_max_h();
 hipError_t ring_set_lds_limit(int slots);
 int adjoint_groups(const LegTables &t);
 hipError_t adjoint_set_lds_limit(int slots);
