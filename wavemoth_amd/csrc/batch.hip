@@ -149,8 +149,6 @@ bool request_tables(DevArena &keep, DevArena &a, Geometry *g, int K, Shape sh, h
   if (!dalloc(a, &t.atil, (size_t)t.nlpad * nkb * sh.KB, "batch atil")) return false;
   if (!dalloc(a, &t.qbuf, (size_t)q_elems(t.R, K, t.M + 1), "batch qbuf")) return false;
   t.npix = g->rhost.npix;
-  t.mtmp = nullptr;
-  if (K > 1 && !dalloc(a, &t.mtmp, (size_t)t.npix * K, "batch map-major output")) return false;
   *out = t;
   return true;
 }
@@ -182,13 +180,6 @@ bool make_dev(DevArena &a, const std::vector<LegTables> &tabs,
   BatchRingItem *dri = nullptr;
   if (!upload(a, &dri, ritems, "ring items")) return false;
   d->ritems = dri;
-  std::vector<int64_t> toff(nreq + 1, 0);
-  for (int r = 0; r < nreq; ++r)
-    toff[r + 1] = toff[r] + (tabs[r].K > 1 ? map_transpose_tiles(tabs[r].npix, tabs[r].K) : 0);
-  int64_t *dto = nullptr;
-  if (!upload(a, &dto, toff, "transpose tile offsets")) return false;
-  d->tile_off = dto;
-  d->ntiles = toff[nreq];
   d->twiddle = twiddle;
   if (!dalloc(a, &d->counters, kBatchCounters, "queue heads")) return false;
   void **p = nullptr;
@@ -331,7 +322,6 @@ bool enqueue(wavemoth_batch b, hipStream_t s) {
     WM_TRY(launch_batch_rings(d, b->ring_first[c], b->ring_count[c], 1 + c, b->ring_slots[c],
                               b->ring_blocks[c], s),
            "batch ring stage");
-  WM_TRY(launch_batch_transpose(d, s), "batch map transpose");
   if (!mark(b, 3, s)) return false;
   return true;
 }

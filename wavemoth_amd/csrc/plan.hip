@@ -270,8 +270,6 @@ static bool build_legendre_tables(wavemoth_plan p) {
   }
   if (!dalloc(p->arena, &t.qbuf, (size_t)q_elems(t.R, p->K, p->M + 1), "qbuf")) return false;
   t.npix = 12 * (int64_t)p->N * p->N;
-  if (p->K > 1 && !dalloc(p->arena, &t.mtmp, (size_t)t.npix * p->K, "map-major ring output"))
-    return false;
   return true;
 }
 

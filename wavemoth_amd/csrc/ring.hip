@@ -31,13 +31,25 @@ namespace wm {
 
 using namespace fft;
 
+// Stores pixels 2p, 2p + 1 of one map: contiguous for a single map, else
+// strided by the number of maps (the (npix, K) layout).
+__device__ __forceinline__ void store_pix2(double *out, int p, double2 z, int ostride) {
+  if (ostride == 1) {
+    *reinterpret_cast<double2 *>(out + 2 * p) = z;
+  } else {
+    out[(int64_t)(2 * p) * ostride] = z.x;
+    out[(int64_t)(2 * p + 1) * ostride] = z.y;
+  }
+}
+
 // Map k of one ring pair (index into rt.pairs) by the whole block; `lds`
 // holds lds_slots complex slots, the last kTwSlots of them the twiddle
-// table (already loaded).  The ring's pixels go to out[pixel] (contiguous:
-// the map itself for one map, else map k's row of the map-major buffer).
+// table (already loaded).  Pixel p of the ring goes to outk[p * ostride]
+// (outk = map + k, ostride = nmaps: the (npix, nmaps) layout).
 // Used by k_ring_pairs and the batched kernel.
 __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair, int k,
-                               double2 *lds, int lds_slots, double *__restrict__ outk) {
+                               double2 *lds, int lds_slots, double *__restrict__ outk,
+                               int ostride) {
   double2 *twl = lds + lds_slots - kTwSlots;
   const RingPair rp = rt.pairs[pair];
   const int n = rp.n, h = n / 2, B = h + 1, K = t.K, M = t.M, P = rp.P;
@@ -63,7 +75,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
       int tid = threadIdx.x;
       asm volatile("" : "+v"(tid));
       const double hs = hemi ? -1.0 : 1.0;  // E + O north, E - O south
-      double *out = outk + (hemi ? rp.pix_s : rp.pix_n);
+      double *out = outk + (hemi ? rp.pix_s : rp.pix_n) * ostride;
       // 1.-2. g_m = E_m +/- O_m staged in LDS, kStage*NT values of m at a
       // time (one round of independent loads), then the signed alias sums
       // per bin: slot (i) -> bin b = slot % B, group g = slot / B; direct
@@ -171,7 +183,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         for (int i = 0; i < kOut; ++i) {
           const int p = tid + NT * i;
           if (p < h) {
-            *reinterpret_cast<double2 *>(out + 2 * p) = lds[pidx(p)];
+            store_pix2(out, p, lds[pidx(p)], ostride);
           }
         }
       } else {
@@ -193,7 +205,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         for (int i = 0; i < kOut; ++i) {
           const int p = tid + NT * i;
           if (p < h) {
-            *reinterpret_cast<double2 *>(out + 2 * p) = cmul(lds[pidx(p)], chirp[p]);
+            store_pix2(out, p, cmul(lds[pidx(p)], chirp[p]), ostride);
           }
         }
       }
@@ -211,7 +223,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
 constexpr int kBeltMaxH = kMaxFFT / 2;
 __device__ __forceinline__ void ring_belt_body(const LegTables &t, const RingTables &rt,
                                                int pair, int k, double2 *lds, int lds_slots,
-                                               double *__restrict__ outk) {
+                                               double *__restrict__ outk, int ostride) {
   double2 *twl = lds + lds_slots - kTwSlots;
   const RingPair rp = rt.pairs[pair];
   const int n = rp.n, h = n / 2, K = t.K, M = t.M;
@@ -226,7 +238,7 @@ __device__ __forceinline__ void ring_belt_body(const LegTables &t, const RingTab
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const double hs = hemi ? -1.0 : 1.0;  // E + O north, E - O south
-    double *out = outk + (hemi ? rp.pix_s : rp.pix_n);
+    double *out = outk + (hemi ? rp.pix_s : rp.pix_n) * ostride;
     // phases e^{i pi s b / n} and twiddles e^{2 pi i a / n} by angle addition
     const double2 pstep = expipi((int64_t)rp.s * NT, n);
     double2 w = expipi((int64_t)rp.s * tid, n);
@@ -277,7 +289,7 @@ __device__ __forceinline__ void ring_belt_body(const LegTables &t, const RingTab
 #pragma unroll
     for (int i = 0; i < kO; ++i) {
       const int p = tid + NT * i;
-      if (p < h) *reinterpret_cast<double2 *>(out + 2 * p) = lds[pidx(p)];
+      if (p < h) store_pix2(out, p, lds[pidx(p)], ostride);
     }
     __syncthreads();
   }
@@ -301,8 +313,7 @@ __global__ __launch_bounds__(NT) void k_ring_belt(LegTables t, RingTables rt, in
   int i, k;
   if (!ring_block_item(blockIdx.x, t.K, count, &i, &k)) return;
   load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
-  ring_belt_body(t, rt, rt.order[first + i], k, lds, lds_slots,
-                 t.K == 1 ? map : t.mtmp + (int64_t)k * t.npix);
+  ring_belt_body(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, t.K);
 }
 
 // Block -> (ring pair, map).  The G = q_group(K) maps of one q group share
@@ -321,71 +332,7 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
   const int i = gamma / nG, k = (gamma - i * nG) * G + j;
   if (i >= count || k >= K) return;
   load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
-  ring_pair_body(t, rt, rt.order[first + i], k, lds, lds_slots,
-                 K == 1 ? map : t.mtmp + (int64_t)k * t.npix);
-}
-
-// (npix, K) map from the map-major [K][npix] ring output: 32 x 32 tiles
-// through LDS, reads coalesced along pixels, writes along maps.
-constexpr int kTT = 32;
-__device__ __forceinline__ void transpose_tile(const double *__restrict__ src,
-                                               double *__restrict__ dst, int64_t npix, int K,
-                                               int64_t tile, double (*sh)[kTT + 1]) {
-  const int64_t tp = (npix + kTT - 1) / kTT;
-  const int64_t p0 = (tile % tp) * kTT;
-  const int k0 = (int)(tile / tp) * kTT;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-#pragma unroll
-  for (int j = 0; j < kTT; j += 8) {
-    const int kk = ty + j;
-    const int64_t p = p0 + tx;
-    if (k0 + kk < K && p < npix) sh[kk][tx] = src[(int64_t)(k0 + kk) * npix + p];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kTT; j += 8) {
-    const int pp = ty + j;
-    if (p0 + pp < npix && k0 + tx < K) dst[(p0 + pp) * K + k0 + tx] = sh[tx][pp];
-  }
-  __syncthreads();
-}
-
-int64_t map_transpose_tiles(int64_t npix, int K) {
-  return ((npix + kTT - 1) / kTT) * ((K + kTT - 1) / kTT);
-}
-
-// Grid-stride over tiles (a grid is limited to 2^32 threads).
-constexpr int kTransposeBlocks = 8192;
-
-__global__ __launch_bounds__(256) void k_map_transpose(const double *__restrict__ src,
-                                                       double *__restrict__ dst, int64_t npix,
-                                                       int K, int64_t ntiles) {
-  __shared__ double sh[kTT][kTT + 1];
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-    transpose_tile(src, dst, npix, K, tile, sh);
-}
-
-__global__ __launch_bounds__(256) void k_batch_transpose(BatchDev b) {
-  __shared__ double sh[kTT][kTT + 1];
-  for (int64_t g = blockIdx.x; g < b.ntiles; g += gridDim.x) {
-    int lo = 0, hi = b.nreq - 1;  // tile_off[lo] <= g < tile_off[lo + 1]
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) / 2;
-      if (b.tile_off[mid] <= g) lo = mid; else hi = mid - 1;
-    }
-    const LegTables &t = b.tabs[lo];
-    transpose_tile(t.mtmp, b.map[lo], t.npix, t.K, g - b.tile_off[lo], sh);
-  }
-}
-
-static unsigned transpose_grid(int64_t ntiles) {
-  return (unsigned)(ntiles < kTransposeBlocks ? ntiles : kTransposeBlocks);
-}
-
-hipError_t launch_batch_transpose(const BatchDev &b, hipStream_t s) {
-  if (b.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_transpose, dim3(transpose_grid(b.ntiles)), dim3(256), 0, s, b);
-  return hipGetLastError();
+  ring_pair_body(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, K);
 }
 
 // Persistent ring stage of a batch: each block takes ring pairs of any
@@ -403,8 +350,8 @@ __global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int c
     if (it >= count) break;
     const BatchRingItem item = b.ritems[first + it];
     const LegTables &t = b.tabs[item.req];
-    ring_pair_body(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
-                   t.K == 1 ? b.map[item.req] : t.mtmp + (int64_t)item.k * t.npix);
+    ring_pair_body(t, b.rtabs[item.req], item.pair, item.k, lds, slots, b.map[item.req] + item.k,
+                   t.K);
   }
 }
 
@@ -520,12 +467,6 @@ hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *m
                          rt.first[i], rt.count[i], rt.lds[i], map);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-  }
-  if (t.K > 1) {
-    const int64_t nt = map_transpose_tiles(t.npix, t.K);
-    hipLaunchKernelGGL(k_map_transpose, dim3(transpose_grid(nt)), dim3(256), 0, s, t.mtmp, map,
-                       t.npix, t.K, nt);
-    return hipGetLastError();
   }
   return hipSuccess;
 }

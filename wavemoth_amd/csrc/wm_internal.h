@@ -43,7 +43,6 @@ struct LegTables {
   int *wseedmax = nullptr;    // [(M+1)*W] max seed_l (<= lmax) over a wave
   double2 *atil = nullptr;    // [ceil(K/KB)][nlpad][KB] prescaled a_lm (per execute)
   double2 *qbuf = nullptr;    // [R][2][K][M+1] even/odd Legendre sums (q_index)
-  double *mtmp = nullptr;     // [K][npix] map-major ring-stage output (K > 1 only)
   int64_t npix = 0;
 };
 
@@ -117,8 +116,6 @@ struct BatchDev {
   int nlitems = 0;
   const BatchRingItem *ritems = nullptr;
   const double2 *twiddle = nullptr;     // [kMaxFFT] shared twiddle table
-  const int64_t *tile_off = nullptr;    // [nreq + 1] prefix of transpose tiles (K > 1)
-  int64_t ntiles = 0;
   int *counters = nullptr;              // [kBatchCounters] work-queue heads
 };
 
@@ -135,9 +132,6 @@ hipError_t launch_batch_legendre(const BatchDev &b, int nblocks, hipStream_t s);
 hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
                               int nblocks, hipStream_t s);
 hipError_t batch_rings_set_lds_limit(int slots);
-// Map-major ring output -> (npix, K) map of every request with K > 1.
-hipError_t launch_batch_transpose(const BatchDev &b, hipStream_t s);
-int64_t map_transpose_tiles(int64_t npix, int K);
 hipError_t batch_legendre_occupancy(int *blocks_per_cu);
 hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu);
 
