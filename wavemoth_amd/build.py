@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBNAME = "libwavemoth_hip.so"
-SOURCES = ["plan.hip", "batch.hip", "legendre.hip", "mixed.hip", "ring.hip", "adjoint.hip"]
+SOURCES = ["plan.hip", "batch.hip", "legendre.hip", "mixed.hip", "ring.hip", "ring16.hip",
+           "adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -101,6 +101,12 @@ struct _wavemoth_batch {
   int nlitems = 0, nritems = 0;
   int ring_first[2] = {0, 0}, ring_count[2] = {0, 0};
   int ring_slots[2] = {0, 0}, ring_blocks[2] = {0, 0};
+  // register-FFT ring classes (kind, FFT size): r16items[first .. first + count)
+  struct R16Class {
+    int kind, P, first, count, blocks;
+  };
+  std::vector<R16Class> r16;
+  int nr16items = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   // device staging of wavemoth_execute_batch, allocated on first use
@@ -147,7 +153,7 @@ bool request_tables(DevArena &keep, DevArena &a, Geometry *g, int K, Shape sh, h
   t.KB = sh.KB;
   const int nkb = (K + sh.KB - 1) / sh.KB;
   if (!dalloc(a, &t.atil, (size_t)t.nlpad * nkb * sh.KB, "batch atil")) return false;
-  if (!dalloc(a, &t.qbuf, (size_t)q_elems(t.R, K, t.M + 1), "batch qbuf")) return false;
+  if (!dalloc(a, &t.qbuf, (size_t)q_alloc_elems(t.R, K, t.M + 1), "batch qbuf")) return false;
   t.npix = g->rhost.npix;
   *out = t;
   return true;
@@ -157,7 +163,8 @@ bool request_tables(DevArena &keep, DevArena &a, Geometry *g, int K, Shape sh, h
 // pointer array (inputs then outputs) is allocated but not filled.
 bool make_dev(DevArena &a, const std::vector<LegTables> &tabs,
               const std::vector<RingTables> &rtabs, const std::vector<BatchLegItem> &litems,
-              const std::vector<BatchRingItem> &ritems, const double2 *twiddle, BatchDev *d,
+              const std::vector<BatchRingItem> &ritems,
+              const std::vector<BatchRing16Item> &r16items, const double2 *twiddle, BatchDev *d,
               void ***d_ptrs) {
   const int nreq = (int)tabs.size();
   d->nreq = nreq;
@@ -180,6 +187,9 @@ bool make_dev(DevArena &a, const std::vector<LegTables> &tabs,
   BatchRingItem *dri = nullptr;
   if (!upload(a, &dri, ritems, "ring items")) return false;
   d->ritems = dri;
+  BatchRing16Item *dr16 = nullptr;
+  if (!upload(a, &dr16, r16items, "register-FFT ring items")) return false;
+  d->r16items = dr16;
   d->twiddle = twiddle;
   if (!dalloc(a, &d->counters, kBatchCounters, "queue heads")) return false;
   void **p = nullptr;
@@ -213,7 +223,7 @@ double time_shape(wavemoth_batch b, Geometry *g, int K, Shape sh) {
     sweep_items(t, 0, items);
     BatchDev d;
     void **dp = nullptr;
-    if (!make_dev(a, {t}, {g->rt}, items, {}, g->rt.twiddle, &d, &dp)) break;
+    if (!make_dev(a, {t}, {g->rt}, items, {}, {}, g->rt.twiddle, &d, &dp)) break;
     std::vector<void *> ptrs = {(void *)alm, nullptr};
     if (hipMemcpy(dp, ptrs.data(), 2 * sizeof(void *), hipMemcpyHostToDevice) != hipSuccess)
       break;
@@ -322,6 +332,11 @@ bool enqueue(wavemoth_batch b, hipStream_t s) {
     WM_TRY(launch_batch_rings(d, b->ring_first[c], b->ring_count[c], 1 + c, b->ring_slots[c],
                               b->ring_blocks[c], s),
            "batch ring stage");
+  for (size_t c = 0; c < b->r16.size(); ++c) {
+    const auto &x = b->r16[c];
+    WM_TRY(launch_batch_ring16(x.kind, x.P, d, x.first, x.count, 3 + (int)c, x.blocks, s),
+           "batch register-FFT ring stage");
+  }
   if (!mark(b, 3, s)) return false;
   return true;
 }
@@ -401,9 +416,49 @@ bool build_batch(wavemoth_batch b) {
   std::vector<RI> cls[2];
   for (int r = 0; r < nreq; ++r) {
     const RingHost &h = b->geoms[b->geom[r]]->rhost;
-    for (int p = 0; p < (int)h.pairs.size(); ++p)
+    for (int p = 0; p < (int)h.pairs.size(); ++p) {
+      if (h.kind[p] != kRingGeneral) continue;
       for (int k = 0; k < b->K[r]; ++k)
         cls[h.need[p] > big_slots ? 0 : 1].push_back({{r, p, k}, h.cost[p], h.need[p]});
+    }
+  }
+  // Register-FFT rings: the blocks of every request's register-FFT launch
+  // groups, one queue per (kind, FFT size), costliest request first.
+  std::vector<BatchRing16Item> r16items;
+  {
+    std::map<std::pair<int, int>, std::vector<BatchRing16Item>> q16;
+    std::vector<int> rord(nreq);
+    std::iota(rord.begin(), rord.end(), 0);
+    std::stable_sort(rord.begin(), rord.end(), [&](int x, int y) {
+      return (int64_t)b->npix[x] * b->K[x] > (int64_t)b->npix[y] * b->K[y];
+    });
+    for (int r : rord) {
+      const RingTables &rt = b->geoms[b->geom[r]]->rt;
+      for (int g = 0; g < rt.nlaunch; ++g) {
+        if (rt.kind[g] == kRingGeneral || rt.count[g] == 0) continue;
+        const int64_t nb = ring16_blocks(rt.kind[g], rt.fftP[g], b->K[r], rt.count[g]);
+        auto &v = q16[{rt.kind[g], rt.fftP[g]}];
+        for (int64_t x = 0; x < nb; ++x) v.push_back({r, g, (int)x});
+      }
+    }
+    if (3 + q16.size() > (size_t)kBatchCounters) {
+      set_err("batch: too many ring classes");
+      return false;
+    }
+    b->r16.clear();
+    for (auto &kv : q16) {
+      _wavemoth_batch::R16Class c;
+      c.kind = kv.first.first;
+      c.P = kv.first.second;
+      c.first = (int)r16items.size();
+      c.count = (int)kv.second.size();
+      int per_cu = 0;
+      WM_TRY(batch_ring16_occupancy(c.kind, c.P, &per_cu), "batch ring occupancy");
+      c.blocks = std::max(1, std::min(std::max(per_cu, 1) * b->cus, c.count));
+      r16items.insert(r16items.end(), kv.second.begin(), kv.second.end());
+      b->r16.push_back(c);
+    }
+    b->nr16items = (int)r16items.size();
   }
   std::vector<BatchRingItem> ritems;
   for (int c = 0; c < 2; ++c) {
@@ -428,8 +483,8 @@ bool build_batch(wavemoth_batch b) {
   b->nritems = (int)ritems.size();
   std::vector<RingTables> rtabs(nreq);
   for (int r = 0; r < nreq; ++r) rtabs[r] = b->geoms[b->geom[r]]->rt;
-  if (!make_dev(b->arena, b->tabs, rtabs, litems, ritems, b->geoms[0]->rt.twiddle, &b->dev,
-                &b->d_ptrs))
+  if (!make_dev(b->arena, b->tabs, rtabs, litems, ritems, r16items, b->geoms[0]->rt.twiddle,
+                &b->dev, &b->d_ptrs))
     return false;
   b->leg_blocks = sweep_blocks(b->cus, b->nlitems);
   WM_TRY(hipHostMalloc((void **)&b->h_ptrs, (size_t)2 * nreq * sizeof(void *),
@@ -655,7 +710,7 @@ int64_t wavemoth_batch_info(wavemoth_batch b, int req, int key) {
       case 4: return b->sum_flops;
       case 5: return (int64_t)b->arena.bytes;
       case 6: return b->nlitems;
-      case 7: return b->nritems;
+      case 7: return (int64_t)b->nritems + b->nr16items;
       case 8: return b->gexec ? 1 : 0;
       case 9: return b->sum_exec;
       case 10: return b->leg_blocks;

@@ -268,7 +268,7 @@ static bool build_legendre_tables(wavemoth_plan p) {
   } else if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil")) {
     return false;
   }
-  if (!dalloc(p->arena, &t.qbuf, (size_t)q_elems(t.R, p->K, p->M + 1), "qbuf")) return false;
+  if (!dalloc(p->arena, &t.qbuf, (size_t)q_alloc_elems(t.R, p->K, p->M + 1), "qbuf")) return false;
   t.npix = 12 * (int64_t)p->N * p->N;
   return true;
 }
@@ -311,10 +311,11 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
   std::vector<RingPair> pairs(R);
   std::vector<double> cost(R);
   std::vector<int> need(R);
-  std::vector<int> hlist, plist;
-  std::vector<int64_t> choff, vhoff;
-  std::vector<int> h_index(2 * N + 1, -1);
-  int64_t chtot = 0, vhtot = 0;
+  std::vector<int> hlist, plist, h2list, p2list;
+  std::vector<int64_t> choff, vhoff, ch2off, vh2off;
+  std::vector<int> h_index(2 * N + 1, -1), i_index(N + 1, -1);
+  std::vector<int> kind(R, kRingGeneral), fftP(R, 0);
+  int64_t chtot = 0, vhtot = 0, ch2tot = 0, vh2tot = 0;
   for (int r = 0; r < R; ++r) {
     // The pair is processed with the southern ring's length and phi0, as
     // src/wavemoth.c:532-546 sets ri->phi0 / ri->length from mid_ring + r;
@@ -327,11 +328,17 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rp.n = n;
     rp.s = phase_shifts ? num : 0;
     rp.r = r;
+    rp.chirp2_off = rp.vhat2_off = -1;
+    rp.P2 = 0;
     const int h = n / 2;
     if (is_pow2(h)) {
       rp.P = 0;
       rp.chirp_off = rp.vhat_off = 0;
       cost[r] = (double)h * std::log2((double)h + 1.0) + (double)(M + 1);
+      if (M <= h && ring16_size_ok(h)) {
+        kind[r] = kRingBelt16;
+        fftP[r] = h;
+      }
     } else {
       const int P = next_pow2(2 * h - 1);
       if (h_index[h] < 0) {
@@ -348,6 +355,24 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       rp.chirp_off = choff[hi];
       rp.vhat_off = vhoff[hi];
       cost[r] = 2.0 * (double)P * std::log2((double)P) + (double)(M + 1);
+      // no wrap (n > mmax): split Bluestein of the even / odd Z entries
+      const int i = h / 2, P2 = next_pow2(2 * i - 1);
+      if (n > M && 2 * i == h && ring16_size_ok(P2)) {
+        if (i_index[i] < 0) {
+          i_index[i] = (int)h2list.size();
+          h2list.push_back(i);
+          p2list.push_back(P2);
+          ch2off.push_back(ch2tot);
+          vh2off.push_back(vh2tot);
+          ch2tot += i;
+          vh2tot += P2;
+        }
+        rp.P2 = P2;
+        rp.chirp2_off = ch2off[i_index[i]];
+        rp.vhat2_off = vh2off[i_index[i]];
+        kind[r] = kRingCap16;
+        fftP[r] = P2;
+      }
     }
     need[r] = ring_lds_slots(n, rp.P, M);
     if (rp.P > kMaxFFT || (rp.P == 0 && h > kMaxFFT)) {
@@ -355,26 +380,30 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       return false;
     }
   }
-  // Launch groups: rings whose LDS need exceeds 80 KiB (one block per CU),
-  // non-aliasing power-of-two rings (the belt kernel), the rest; costliest
-  // first within each.
+  // Launch groups: register-FFT caps by FFT length (largest first), the
+  // belt, then the general kernel's rings whose LDS need exceeds 80 KiB
+  // (one block per CU) and the rest; costliest first within each.
   const int big_slots = (80 * 1024) / (int)sizeof(double2);
-  std::vector<char> beltr(R, 0);
-  for (int r = 0; r < R; ++r) {
-    const int hh = pairs[r].n / 2;
-    if (pairs[r].P == 0 && M <= hh && hh <= ring_belt_max_h()) beltr[r] = 1;
-  }
-  auto cls = [&](int r) { return beltr[r] ? 1 : (need[r] > big_slots ? 2 : 0); };
+  auto key = [&](int r) -> int64_t {
+    if (kind[r] == kRingCap16) return 0 * 100000 + (8192 - fftP[r]);
+    if (kind[r] == kRingBelt16) return 1 * 100000 + (8192 - fftP[r]);
+    return (need[r] > big_slots ? 2 : 3) * 100000;
+  };
   std::vector<int> order(R);
   for (int i = 0; i < R; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    if (cls(a) != cls(b)) return cls(a) > cls(b);
+    if (key(a) != key(b)) return key(a) < key(b);
     return cost[a] > cost[b];
   });
   rt.nlaunch = 0;
   int max_need = 0;
   for (int i = 0; i < R; ++i) {
-    if (rt.nlaunch == 0 || cls(order[i]) != cls(order[rt.first[rt.nlaunch - 1]])) {
+    const int r = order[i];
+    if (rt.nlaunch == 0 || key(r) != key(order[rt.first[rt.nlaunch - 1]])) {
+      if (rt.nlaunch == kMaxLaunches) {
+        set_err("ring stage: too many ring launch groups");
+        return false;
+      }
       rt.first[rt.nlaunch] = i;
       rt.count[rt.nlaunch] = 0;
       rt.lds[rt.nlaunch] = 0;
@@ -382,10 +411,13 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     }
     const int g = rt.nlaunch - 1;
     rt.count[g]++;
-    rt.lds[g] = std::max(rt.lds[g], need[order[i]]);
-    rt.belt[g] = beltr[order[i]];
-    max_need = std::max(max_need, need[order[i]]);
+    rt.kind[g] = kind[r];
+    rt.fftP[g] = fftP[r];
+    rt.lds[g] = kind[r] == kRingGeneral ? std::max(rt.lds[g], need[r])
+                                       : ring16_lds_slots(kind[r], fftP[r]);
+    max_need = std::max(max_need, need[r]);
   }
+  WM_TRY(ring16_set_lds_limits(), "register-FFT ring kernel LDS limit");
   WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");
   WM_TRY(adjoint_set_lds_limit(max_need), "ring adjoint kernel LDS limit");
   std::vector<double2> tw(kMaxFFT);
@@ -398,6 +430,8 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
   if (!upload(a, &rt.twiddle, tw, "twiddle")) return false;
   if (!dalloc(a, &rt.chirp, (size_t)chtot, "chirp")) return false;
   if (!dalloc(a, &rt.vhat, (size_t)vhtot, "vhat")) return false;
+  if (!dalloc(a, &rt.chirp2, (size_t)ch2tot, "split chirp")) return false;
+  if (!dalloc(a, &rt.vhat2, (size_t)vh2tot, "split vhat")) return false;
   if (!hlist.empty()) {
     int *d_h = nullptr, *d_p = nullptr;
     int64_t *d_co = nullptr, *d_vo = nullptr;
@@ -405,13 +439,27 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     if (!upload(a, &d_p, plist, "plist")) return false;
     if (!upload(a, &d_co, choff, "choff")) return false;
     if (!upload(a, &d_vo, vhoff, "vhoff")) return false;
-    WM_TRY(launch_bluestein_prep(rt, d_h, d_co, d_vo, d_p, (int)hlist.size(), stream),
+    WM_TRY(launch_bluestein_prep(rt.twiddle, rt.chirp, rt.vhat, d_h, d_co, d_vo, d_p,
+                                 (int)hlist.size(), stream),
            "bluestein prep");
     WM_TRY(hipStreamSynchronize(stream), "bluestein prep");
+  }
+  if (!h2list.empty()) {
+    int *d_h = nullptr, *d_p = nullptr;
+    int64_t *d_co = nullptr, *d_vo = nullptr;
+    if (!upload(a, &d_h, h2list, "split hlist")) return false;
+    if (!upload(a, &d_p, p2list, "split plist")) return false;
+    if (!upload(a, &d_co, ch2off, "split choff")) return false;
+    if (!upload(a, &d_vo, vh2off, "split vhoff")) return false;
+    WM_TRY(launch_bluestein_prep(rt.twiddle, rt.chirp2, rt.vhat2, d_h, d_co, d_vo, d_p,
+                                 (int)h2list.size(), stream),
+           "split bluestein prep");
+    WM_TRY(hipStreamSynchronize(stream), "split bluestein prep");
   }
   host.pairs = std::move(pairs);
   host.need = std::move(need);
   host.cost = std::move(cost);
+  host.kind = std::move(kind);
   return true;
 }
 

@@ -138,8 +138,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
       __syncthreads();
       // 4. per pair (a, h - a): phases e^{i pi s b / n} (angle addition for a,
       // e^{i pi s / 2} conj(.) for h - a), c2r semantics for DC / Nyquist,
-      // then the Z pair with twiddle e^{2 pi i a / n} (angle addition).  The
-      // belt-ring body computes the same expressions.
+      // then the Z pair with twiddle e^{2 pi i a / n} (angle addition).
       {
         const double2 step = expipi(2 * NT, n);
         double2 om = expipi(2 * tid, n);
@@ -214,108 +213,6 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
   }
 }
 
-// Belt rings without aliasing (power-of-two h = n/2 and mmax <= h): bin b
-// takes the single term m = b (and bin h also its conjugate), so each thread
-// reads its two bins a, h - a of E +/- O straight from q, phases them and
-// forms the pre-twiddled Z pair in one step; no staging, no alias
-// accumulators.  Same arithmetic per bin as ring_pair_body (bitwise equal
-// results), with a 4096-point bound on the inlined FFT.
-constexpr int kBeltMaxH = kMaxFFT / 2;
-__device__ __forceinline__ void ring_belt_body(const LegTables &t, const RingTables &rt,
-                                               int pair, int k, double2 *lds, int lds_slots,
-                                               double *__restrict__ outk, int ostride) {
-  double2 *twl = lds + lds_slots - kTwSlots;
-  const RingPair rp = rt.pairs[pair];
-  const int n = rp.n, h = n / 2, K = t.K, M = t.M;
-  const bool south = rp.pix_s >= 0;
-  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
-  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
-  const int64_t qs = q_group(K);
-  constexpr int kP = (kBeltMaxH / 2 + 1 + NT - 1) / NT;  // Z pairs per thread
-  constexpr int kO = (kBeltMaxH + NT - 1) / NT;          // outputs per thread
-  const int nhemi = south ? 2 : 1;
-  for (int hemi = 0; hemi < nhemi; ++hemi) {
-    int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    const double hs = hemi ? -1.0 : 1.0;  // E + O north, E - O south
-    double *out = outk + (hemi ? rp.pix_s : rp.pix_n) * ostride;
-    // phases e^{i pi s b / n} and twiddles e^{2 pi i a / n} by angle addition
-    const double2 pstep = expipi((int64_t)rp.s * NT, n);
-    double2 w = expipi((int64_t)rp.s * tid, n);
-    const double2 ostep = expipi(2 * NT, n);
-    double2 om = expipi(2 * tid, n);
-    // e^{i pi s h / n} = e^{i pi s / 2}
-    const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
-#pragma unroll
-    for (int i = 0; i < kP; ++i) {
-      const int a = tid + NT * i;
-      if (a <= h / 2) {
-        const int c = h - a;
-        double2 ga = make_double2(0.0, 0.0), gc = make_double2(0.0, 0.0);
-        if (a <= M) {
-          const double2 e = qe[a * qs], o = qo[a * qs];
-          ga = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
-        }
-        if (c <= M) {
-          const double2 e = qe[c * qs], o = qo[c * qs];
-          gc = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
-        }
-        // X_a = w_a g_a (DC: real part only); X_c = w_c g_c with
-        // w_c = e^{i pi s / 2} conj(w_a); bin h also takes its conjugate term
-        double2 xa = cmul(ga, w);
-        if (a == 0) xa.y = 0.0;
-        const double2 wc = cmul(wh, cconj(w));
-        double2 xc;
-        if (c == h) {
-          // X_h = e^{i pi s h/n} [g_h + (-1)^s conj(g_h)], real part only
-          const double sg = rp.s ? -1.0 : 1.0;
-          xc = cmul(make_double2(fma(sg, gc.x, gc.x), fma(-sg, gc.y, gc.y)), wh);
-          xc.y = 0.0;
-        } else {
-          xc = cmul(gc, wc);
-        }
-        const double2 za = cadd(cadd(xa, cconj(xc)), cmul_i(cmul(om, csub(xa, cconj(xc)))));
-        lds[pidx(a)] = za;
-        if (a != 0 && a != c) {
-          const double2 oc = make_double2(-om.x, om.y);  // e^{2 pi i c/n}
-          lds[pidx(c)] = cadd(cadd(xc, cconj(xa)), cmul_i(cmul(oc, csub(xc, cconj(xa)))));
-        }
-      }
-      w = cmul(w, pstep);
-      om = cmul(om, ostep);
-    }
-    __syncthreads();
-    lds_fft_inl<true, kBeltMaxH>(lds, h, twl);
-#pragma unroll
-    for (int i = 0; i < kO; ++i) {
-      const int p = tid + NT * i;
-      if (p < h) store_pix2(out, p, lds[pidx(p)], ostride);
-    }
-    __syncthreads();
-  }
-}
-
-// Block -> (ring pair, map), as k_ring_pairs below.
-__device__ __forceinline__ bool ring_block_item(int b, int K, int count, int *i, int *k) {
-  const int G = q_group(K), nG = q_groups(K);
-  const int x = b & 7, s = b >> 3;
-  const int sigma = s / G, j = s - sigma * G;
-  const int gamma = sigma * 8 + x;
-  *i = gamma / nG;
-  *k = (gamma - *i * nG) * G + j;
-  return *i < count && *k < K;
-}
-
-__global__ __launch_bounds__(NT) void k_ring_belt(LegTables t, RingTables rt, int first,
-                                                  int count, int lds_slots,
-                                                  double *__restrict__ map) {
-  extern __shared__ double2 lds[];
-  int i, k;
-  if (!ring_block_item(blockIdx.x, t.K, count, &i, &k)) return;
-  load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
-  ring_belt_body(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, t.K);
-}
-
 // Block -> (ring pair, map).  The G = q_group(K) maps of one q group share
 // its 128-byte lines, so their G workgroups are put on one XCD (hardware
 // dispatches block b to XCD b % 8) at consecutive dispatch slots: block
@@ -384,7 +281,9 @@ hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, 
 // Plan-time Bluestein tables, one block per distinct h:
 // chirp[a] = e^{i pi a^2 / h} (a < h), and vhat = FFT_P(v) / P with
 // v[d] = conj(chirp[|d|]) placed cyclically (d in (-h, h)).
-__global__ __launch_bounds__(NT) void k_bluestein_prep(RingTables rt,
+__global__ __launch_bounds__(NT) void k_bluestein_prep(const double2 *__restrict__ twiddle,
+                                                       double2 *__restrict__ chirp_all,
+                                                       double2 *__restrict__ vhat_all,
                                                        const int *__restrict__ hlist,
                                                        const int64_t *__restrict__ choff,
                                                        const int64_t *__restrict__ vhoff,
@@ -393,9 +292,9 @@ __global__ __launch_bounds__(NT) void k_bluestein_prep(RingTables rt,
   const int tid = threadIdx.x;
   const int h = hlist[blockIdx.x], P = plist[blockIdx.x];
   double2 *twl = lds + pidx(kMaxFFT - 1) + 1;
-  load_twiddles(twl, rt.twiddle);
-  double2 *chirp = rt.chirp + choff[blockIdx.x];
-  double2 *vhat = rt.vhat + vhoff[blockIdx.x];
+  load_twiddles(twl, twiddle);
+  double2 *chirp = chirp_all + choff[blockIdx.x];
+  double2 *vhat = vhat_all + vhoff[blockIdx.x];
   for (int a = tid; a < P; a += NT) lds[pidx(a)] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int a = tid; a < h; a += NT) {
@@ -421,15 +320,16 @@ int ring_lds_slots(int n, int P, int M) {
   return need + kTwSlots;
 }
 
-hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const int64_t *choff,
-                                 const int64_t *vhoff, const int *plist, int nh, hipStream_t s) {
+hipError_t launch_bluestein_prep(const double2 *twiddle, double2 *chirp, double2 *vhat,
+                                 const int *hlist, const int64_t *choff, const int64_t *vhoff,
+                                 const int *plist, int nh, hipStream_t s) {
   if (nh <= 0) return hipSuccess;
   const size_t lds = (size_t)(pidx(kMaxFFT - 1) + 1 + kTwSlots) * sizeof(double2);
   hipError_t e = hipFuncSetAttribute((const void *)k_bluestein_prep,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bluestein_prep, dim3(nh), dim3(NT), lds, s, rt, hlist, choff, vhoff,
-                     plist);
+  hipLaunchKernelGGL(k_bluestein_prep, dim3(nh), dim3(NT), lds, s, twiddle, chirp, vhat, hlist,
+                     choff, vhoff, plist);
   return hipGetLastError();
 }
 
@@ -440,32 +340,27 @@ hipError_t ring_set_lds_limit(int slots) {
   std::lock_guard<std::mutex> lock(mu);
   if (slots <= cur) return hipSuccess;
   const int bytes = (int)((size_t)slots * sizeof(double2));
-  hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)k_ring_belt, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes);
+  const hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess) cur = slots;
   return e;
 }
-
-int ring_belt_lds_slots(int n) { return pidx(n / 2) + 1 + kTwSlots; }
-int ring_belt_max_h() { return kBeltMaxH; }
 
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
                              hipStream_t s) {
   for (int i = 0; i < rt.nlaunch; ++i) {
     if (rt.count[i] == 0) continue;
-    const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
-    const int64_t groups = (int64_t)rt.count[i] * q_groups(t.K);
-    const int64_t nb = (groups + 7) / 8 * 8 * q_group(t.K);
-    if (rt.belt[i])
-      hipLaunchKernelGGL(k_ring_belt, dim3((unsigned)nb), dim3(NT), lds, s, t, rt, rt.first[i],
-                         rt.count[i], rt.lds[i], map);
-    else
+    hipError_t e;
+    if (rt.kind[i] != kRingGeneral) {
+      e = launch_ring16(rt.kind[i], rt.fftP[i], t, rt, rt.first[i], rt.count[i], map, s);
+    } else {
+      const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
+      const int64_t groups = (int64_t)rt.count[i] * q_groups(t.K);
+      const int64_t nb = (groups + 7) / 8 * 8 * q_group(t.K);
       hipLaunchKernelGGL(k_ring_pairs, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
                          rt.first[i], rt.count[i], rt.lds[i], map);
-    hipError_t e = hipGetLastError();
+      e = hipGetLastError();
+    }
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -1,0 +1,547 @@
+// Ring stage, register-FFT kernels for the rings whose m-range does not
+// wrap around the ring (n > mmax).  Same semantics as ring.hip
+// (src/wavemoth.c:1083-1191): G_m = E_m +/- O_m, phase e^{i m phi0},
+// bins (m mod n) and conjugates ((n - m) mod n) up to n/2, FFTW-style c2r of
+// length n (imaginary parts of DC and Nyquist ignored, unnormalised), with
+// the same Z-pair reduction of the c2r to a complex DFT of h = n/2.
+//
+// No wrap means bin b receives at most g_b (b <= mmax) and the conjugate
+// term (-1)^s conj g_{n-b} (n - b <= mmax), so the fold is two passes over
+// LDS with one writer per bin and no reduction.
+//
+// k_ring16_belt: h a power of two and mmax <= h (the equatorial belt): one
+//   job per (ring pair, map), both rings from one read of E and O, one
+//   P = h point inverse FFT per ring.
+// k_ring16_cap: h = 2i not a power of two (polar caps with 4i > mmax):
+//   the length-h inverse DFT splits into the DFTs of the even and odd Z
+//   entries, z_p = A_p + e^{2 pi i p/h} B_p, z_{p+i} = A_p - e^{2 pi i p/h} B_p,
+//   each a length-i Bluestein convolution of P = next_pow2(2i - 1) <= 4096
+//   points (chirp c_k = e^{i pi k^2/i}, kernel spectrum vhat tabulated per
+//   i at plan time).  Splitting keeps every transform within 4096 points.
+//
+// One job = T = P/16 threads holding 16 points each (wm_fft16.h); a block of
+// 256 threads runs 256/T jobs.  Jobs of the G maps of one q group (they
+// share 128-byte lines of q) and the blocks holding them are placed on one
+// XCD at consecutive dispatch slots.
+#include <mutex>
+
+#include "wm_fft16.h"
+
+namespace wm {
+
+using namespace fft;
+using fft16::pad;
+using fft16::job_slots;
+constexpr int RB = 2 * fft16::kThreads;  // threads per block of both kernels
+
+__host__ __device__ inline int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Job (pair index, map) of job slot jb of block b.  U = pow2_ceil(q_group(K))
+// jobs per unit (pair, q group), J jobs per block.  Returns false for a
+// padding job.
+__device__ __forceinline__ bool ring16_job(int b, int J, int jb, int K, int count, int *pi,
+                                           int *k) {
+  const int G = q_group(K), nG = q_groups(K), U = pow2_ceil(G);
+  int gamma, u;
+  if (J <= U) {
+    const int BU = U / J;  // blocks per unit, same XCD
+    const int x = b & 7, s = b >> 3;
+    const int sigma = s / BU, r = s - sigma * BU;
+    gamma = sigma * 8 + x;
+    u = r * J + jb;
+  } else {
+    const int UB = J / U;  // units per block
+    gamma = b * UB + jb / U;
+    u = jb & (U - 1);
+  }
+  *pi = gamma / nG;
+  *k = (gamma - *pi * nG) * G + u;
+  return *pi < count && u < G && *k < K;
+}
+
+// Blocks of a launch: belt jobs are T = P/16 threads, cap jobs 2T threads,
+// both in 512-thread blocks.
+__host__ __device__ constexpr int ring16_jobs(int kind, int P) {
+  return kind == kRingCap16 ? RB / (P / 8) : RB / (P / 16);
+}
+
+int64_t ring16_blocks(int kind, int P, int K, int count) {
+  const int J = ring16_jobs(kind, P);
+  const int G = q_group(K), U = pow2_ceil(G);
+  const int64_t units = (int64_t)count * q_groups(K);
+  if (J <= U) return (units + 7) / 8 * 8 * (U / J);
+  const int UB = J / U;
+  return (units + UB - 1) / UB;
+}
+
+__host__ __device__ constexpr int ring16_slots(int kind, int P) {
+  return (kind == kRingCap16 ? 2 : 1) * ring16_jobs(kind, P) * job_slots(P) + kTwSlots;
+}
+int ring16_lds_slots(int kind, int P) { return ring16_slots(kind, P); }
+
+// Z_a = (X_a + conj X_c) + i e^{2 pi i a/n} (X_a - conj X_c), c = h - a
+__device__ __forceinline__ double2 zpair(double2 xa, double2 xc, double2 om) {
+  const double2 cc = cconj(xc);
+  return cadd(cadd(xa, cc), cmul_i(cmul(om, csub(xa, cc))));
+}
+
+// pixels 2p, 2p + 1 of one map of the (npix, K) map
+__device__ __forceinline__ void put2(double *out, int p, double2 z, int K) {
+  if (K == 1) {
+    *reinterpret_cast<double2 *>(out + 2 * p) = z;
+  } else {
+    out[(int64_t)(2 * p) * K] = z.x;
+    out[(int64_t)(2 * p + 1) * K] = z.y;
+  }
+}
+
+// pixels 2p, 2p + 1 (p = j + T rho) of the 16 values of one thread
+template <int T>
+__device__ __forceinline__ void put16(double *out, int j, const double2 v[16], int K) {
+  if (K == 1) {
+    double2 *o = reinterpret_cast<double2 *>(out) + j;
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) o[T * rho] = v[rho];
+  } else {
+    double *o = out + (int64_t)(2 * j) * K;
+    const int64_t st = (int64_t)(2 * T) * K;
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      o[0] = v[rho].x;
+      o[K] = v[rho].y;
+      o += st;
+    }
+  }
+}
+
+// Z pairs of the X values in buf (bins 0 .. h), the inverse FFT, the pixels.
+template <int P>
+__device__ __forceinline__ void belt_fft_out(double2 *buf, const double2 *twl, int j, int n,
+                                             bool on, int K, double *__restrict__ out) {
+  constexpr int T = P / 16, h = P;
+  double2 v[16];
+  {
+    const double2 ostep = expipi(2 * T, n);
+    double2 om = expipi(2 * j, n);
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int a = j + T * rho;
+      v[rho] = zpair(buf[pad(a)], buf[pad(h - a)], om);
+      om = cmul(om, ostep);
+    }
+  }
+  fft16::fft<P, true>(v, buf, twl, j);
+  if (on) put16<T>(out, j, v, K);
+}
+
+// Both rings of a belt pair, one map, by the T threads of a job: E and O
+// are read once; X^N = w (E + O) goes through LDS while X^S = w (E - O)
+// waits in registers.  The q rows past mmax are read (the q buffer has a
+// tail of kQTail rows) and discarded, so that all loads are unconditional
+// and in flight together.
+template <int P>
+__device__ __forceinline__ void belt_pair(const RingPair &rp, const double2 *__restrict__ qe,
+                                          const double2 *__restrict__ qo, int64_t qs, int M,
+                                          int K, double2 *buf, const double2 *twl, int j,
+                                          bool act, bool south, double *__restrict__ outn,
+                                          double *__restrict__ outs) {
+  constexpr int T = P / 16, h = P;
+  const int n = rp.n;
+  // 1. X_a = w_a (E_a +/- O_a), w_a = e^{i pi s a/n}, a = j + T rho; DC
+  //    keeps its real part
+  double2 xs[16];
+  {
+    const double2 wstep = expipi((int64_t)rp.s * T, n);
+    double2 w = expipi((int64_t)rp.s * j, n);
+    const double2 *pe = qe + j * qs, *po = qo + j * qs;
+    const int64_t st = T * qs;
+    double2 e[16], o[16];
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      e[rho] = pe[rho * st];
+      o[rho] = po[rho * st];
+    }
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int a = j + T * rho;
+      const bool in = act && a <= M;
+      double2 xn = in ? cmul(cadd(e[rho], o[rho]), w) : make_double2(0.0, 0.0);
+      xs[rho] = in ? cmul(csub(e[rho], o[rho]), w) : make_double2(0.0, 0.0);
+      if (a == 0) xn.y = xs[rho].y = 0.0;
+      buf[pad(a)] = xn;
+      w = cmul(w, wstep);
+    }
+  }
+  // Nyquist bin h: e^{i pi s/2} (g_h + (-1)^s conj g_h), real part
+  double2 xsh = make_double2(0.0, 0.0);
+  if (j == 0) {
+    double2 gn = make_double2(0.0, 0.0), gs = gn;
+    if (act && h <= M) {
+      const double2 e = qe[h * qs], o = qo[h * qs];
+      gn = cadd(e, o);
+      gs = csub(e, o);
+    }
+    const double sg = rp.s ? -1.0 : 1.0;
+    const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
+    double2 xn = cmul(make_double2(fma(sg, gn.x, gn.x), fma(-sg, gn.y, gn.y)), wh);
+    xsh = cmul(make_double2(fma(sg, gs.x, gs.x), fma(-sg, gs.y, gs.y)), wh);
+    xn.y = xsh.y = 0.0;
+    buf[pad(h)] = xn;
+  }
+  __syncthreads();
+  // 2. northern ring
+  belt_fft_out<P>(buf, twl, j, n, act, K, outn);
+  // 3. southern ring: X^S through the same buffer
+  __syncthreads();  // last FFT pass reads done
+#pragma unroll
+  for (int rho = 0; rho < 16; ++rho) buf[pad(j + T * rho)] = xs[rho];
+  if (j == 0) buf[pad(h)] = xsh;
+  __syncthreads();
+  asm volatile("" : "+v"(j));  // no address of the northern pass kept live
+  belt_fft_out<P>(buf, twl, j, n, act && south, K, outs);
+}
+
+// Job slot and thread-in-job of thread tid for T-thread jobs.  Jobs of
+// T >= 64 threads go in pairs whose waves hold 32 lanes of each: lane l
+// and lane l + 32 then handle the same positions of two adjacent maps,
+// whose q rows share a line and whose pixels share a map row, so each
+// load and store request serves both.  Smaller jobs sit side by side in a
+// wave already.
+template <int T>
+__device__ __forceinline__ void job_lane(int tid, int *jb, int *j) {
+  if constexpr (T >= 64) {
+    constexpr int WPP = T / 32;  // waves per job pair
+    const int l = tid & 63, w = tid >> 6;
+    const int pr = w / WPP, wi = w - pr * WPP;
+    *jb = 2 * pr + (l >> 5);
+    *j = wi * 32 + (l & 31);
+  } else {
+    *jb = tid / T;
+    *j = tid % T;
+  }
+}
+
+// Block b of a belt launch over pairs order[first .. first + count).
+template <int P>
+__device__ __forceinline__ void belt_block(const LegTables &t, const RingTables &rt, int first,
+                                           int count, double *__restrict__ map, int b,
+                                           double2 *lds, const double2 *twl) {
+  constexpr int T = P / 16, J = RB / T, S = job_slots(P);
+  int jb, j;
+  job_lane<T>(threadIdx.x, &jb, &j);
+  double2 *buf = lds + jb * S;
+  int pi, k;
+  const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
+  if (!act) {
+    pi = 0;
+    k = 0;
+  }
+  const RingPair rp = rt.pairs[rt.order[first + pi]];
+  const int K = t.K, M = t.M;
+  const bool south = rp.pix_s >= 0;
+  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+  belt_pair<P>(rp, qe, qo, qs, M, K, buf, twl, j, act, south, map + k + rp.pix_n * K,
+               map + k + (south ? rp.pix_s : 0) * K);
+}
+
+// Cap jobs are 2T threads per (ring pair, map), the first T threads
+// transforming the even Z entries (A), the other T the odd ones (B), each
+// in its own LDS buffer, so that no thread holds one transform's result
+// through the other's.
+
+// One cap ring (hemisphere sign hs) of one map: fold, even / odd Z entries,
+// the two Bluestein convolutions, the pixels.  tt = thread in the job
+// (0 .. 2T), bufA / bufB the job's two buffers.
+template <int P>
+__device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__restrict__ qe,
+                                         const double2 *__restrict__ qo, int64_t qs, int M,
+                                         int K, const double2 *__restrict__ chirp,
+                                         const double2 *__restrict__ vhat, double2 *bufA,
+                                         double2 *bufB, const double2 *twl, int tt, bool on,
+                                         double hs, double *__restrict__ out) {
+  constexpr int T = P / 16, T2 = 2 * T;
+  asm volatile("" : "+v"(tt));  // see belt_ring
+  const int n = rp.n, h = n / 2, i = h / 2;
+  const int half = tt / T, j = tt - half * T;
+  const double sg = rp.s ? -1.0 : 1.0;
+  // 1. fold into bufA: F_b = g_b (b <= min(h, mmax)), then F_{n-m} +=
+  //    (-1)^s conj g_m for h <= m <= mmax (one writer per bin in each
+  //    pass); h < P and n > mmax gives mmax - h < h, so 8 rows per thread
+  //    cover each pass
+  {
+    double2 e[8], o[8];
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int b = tt + T2 * rho;
+      const int64_t bc = b <= M ? b : M;
+      e[rho] = qe[bc * qs];
+      o[rho] = qo[bc * qs];
+    }
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int b = tt + T2 * rho;
+      double2 g = make_double2(fma(hs, o[rho].x, e[rho].x), fma(hs, o[rho].y, e[rho].y));
+      if (!(on && b <= M)) g = make_double2(0.0, 0.0);
+      if (b <= h) bufA[pad(b)] = g;
+    }
+  }
+  __syncthreads();
+  {
+    double2 e[8], o[8];
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int m = h + tt + T2 * rho;
+      const int64_t mc = m <= M ? m : M;
+      e[rho] = qe[mc * qs];
+      o[rho] = qo[mc * qs];
+    }
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int m = h + tt + T2 * rho;
+      if (on && m <= M) {
+        const double2 g = make_double2(fma(hs, o[rho].x, e[rho].x), fma(hs, o[rho].y, e[rho].y));
+        double2 &f = bufA[pad(n - m)];
+        f.x = fma(sg, g.x, f.x);
+        f.y = fma(-sg, g.y, f.y);
+      }
+    }
+  }
+  __syncthreads();
+  asm volatile("" : "+v"(tt));  // no table load hoisted above the barriers
+  // 2. Z entry b = 2 kappa + half (kappa = j + T rho < i < P/2, so rho < 8)
+  //    times the chirp; phase w_b = e^{i pi s b/n}, partner phase
+  //    e^{i pi s/2} conj(w_b), twiddle e^{2 pi i b/n}; DC / Nyquist real
+  const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
+  double2 v[16];
+  {
+    const double2 wstep = expipi((int64_t)rp.s * 2 * T, n), ostep = expipi(4 * T, n);
+    double2 w = expipi((int64_t)rp.s * (2 * j + half), n), om = expipi(2 * (2 * j + half), n);
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) v[rho] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int kap = j + T * rho;
+      const bool in = kap < i;
+      const double2 c = chirp[in ? kap : 0];
+      const int b = in ? 2 * kap + half : 0;
+      double2 xa = cmul(bufA[pad(b)], w);
+      double2 xc = cmul(bufA[pad(h - b)], b == 0 ? wh : cmul(wh, cconj(w)));
+      if (b == 0) xa.y = xc.y = 0.0;
+      if (in) v[rho] = cmul(zpair(xa, xc, om), c);
+      w = cmul(w, wstep);
+      om = cmul(om, ostep);
+    }
+  }
+  // 3. Bluestein: y = IFFT(vhat . FFT(c . Z)) in the half's own buffer
+  double2 *buf = half ? bufB : bufA;
+  fft16::fft<P, false>(v, buf, twl, j);
+  asm volatile("" : "+v"(tt));
+#pragma unroll
+  for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], vhat[j + T * rho]);
+  fft16::fft<P, true>(v, buf, twl, j);
+  asm volatile("" : "+v"(tt));
+  // A_p = c_p y_p into bufA, e^{2 pi i p/h} c_p y_p (B) into bufB, p < i
+  __syncthreads();  // last-pass reads of buf done
+  {
+    const double2 ostep = expipi(4 * T, n);
+    double2 om = half ? expipi(4 * j, n) : make_double2(1.0, 0.0);
+    const double2 st = half ? ostep : make_double2(1.0, 0.0);
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int p = j + T * rho;
+      if (p < i) buf[pad(p)] = cmul(cmul(v[rho], chirp[p]), om);
+      om = cmul(om, st);
+    }
+  }
+  __syncthreads();
+  // 4. z_p = A_p + B_p, z_{p+i} = A_p - B_p
+  if (on) {
+#pragma unroll
+    for (int rho = 0; rho < 4; ++rho) {
+      const int p = tt + T2 * rho;
+      if (p < i) {
+        const double2 a = bufA[pad(p)], b = bufB[pad(p)];
+        put2(out, p, cadd(a, b), K);
+        put2(out, p + i, csub(a, b), K);
+      }
+    }
+  }
+}
+
+// Block b of a cap launch over pairs order[first .. first + count).
+template <int P>
+__device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &rt, int first,
+                                          int count, double *__restrict__ map, int b,
+                                          double2 *lds, const double2 *twl) {
+  constexpr int T = P / 16, J = RB / (2 * T), S = job_slots(P);
+  const int jb = threadIdx.x / (2 * T), tt = threadIdx.x % (2 * T);
+  double2 *bufA = lds + 2 * jb * S, *bufB = bufA + S;
+  int pi, k;
+  const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
+  if (!act) {
+    pi = 0;
+    k = 0;
+  }
+  const RingPair rp = rt.pairs[rt.order[first + pi]];
+  const int K = t.K, M = t.M;
+  const bool south = rp.pix_s >= 0;
+  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+  // every pair of a cap launch (also the padding jobs' stand-in) has tables
+  const double2 *chirp = rt.chirp2 + rp.chirp2_off;  // [i]
+  const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
+#pragma unroll 1
+  for (int hemi = 0; hemi < 2; ++hemi) {
+    cap_ring<P>(rp, qe, qo, qs, M, K, chirp, vhat, bufA, bufB, twl, tt,
+                act && (hemi == 0 || south), hemi ? -1.0 : 1.0,
+                map + k + (hemi ? (south ? rp.pix_s : 0) : rp.pix_n) * K);
+    __syncthreads();  // buffers reused by the southern ring
+  }
+}
+
+template <int KIND, int P>
+__device__ __forceinline__ void ring16_block(const LegTables &t, const RingTables &rt, int first,
+                                             int count, double *__restrict__ map, int b,
+                                             double2 *lds, const double2 *twl) {
+  if constexpr (KIND == kRingBelt16)
+    belt_block<P>(t, rt, first, count, map, b, lds, twl);
+  else
+    cap_block<P>(t, rt, first, count, map, b, lds, twl);
+}
+
+__device__ __forceinline__ void load_twiddles16(double2 *twl, const double2 *__restrict__ tw) {
+  for (int u = threadIdx.x; u < kTwSlots; u += RB)
+    twl[u] = (u < kTwC) ? tw[u * 64] : tw[u - kTwC];
+}
+
+template <int KIND, int P>
+__global__ __launch_bounds__(RB, 1) void k_ring16(LegTables t, RingTables rt, int first,
+                                                  int count, double *__restrict__ map) {
+  extern __shared__ double2 lds[];
+  double2 *twl = lds + ring16_slots(KIND, P) - kTwSlots;
+  load_twiddles16(twl, rt.twiddle);  // first read after a barrier
+  ring16_block<KIND, P>(t, rt, first, count, map, blockIdx.x, lds, twl);
+}
+
+// Persistent ring kernel of a batch: blocks pull (request, launch group,
+// block) items r16items[first .. first + count) through queue head ctr and
+// run them as the single-plan kernel runs block b (bitwise the same maps).
+template <int KIND, int P>
+__global__ __launch_bounds__(RB, 1) void k_batch_ring16(BatchDev d, int first, int count,
+                                                        int ctr) {
+  extern __shared__ double2 lds[];
+  __shared__ int s_item;
+  double2 *twl = lds + ring16_slots(KIND, P) - kTwSlots;
+  load_twiddles16(twl, d.twiddle);
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&d.counters[ctr], 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= count) break;
+    const BatchRing16Item x = d.r16items[first + it];
+    const LegTables &t = d.tabs[x.req];
+    const RingTables &rt = d.rtabs[x.req];
+    ring16_block<KIND, P>(t, rt, rt.first[x.g], rt.count[x.g], d.map[x.req], x.b, lds, twl);
+    __syncthreads();  // LDS reused by the next item
+  }
+}
+
+#define WM_RING16_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+bool ring16_size_ok(int P) { return P >= 16 && P <= 4096 && (P & (P - 1)) == 0; }
+
+hipError_t ring16_set_lds_limits() {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+#define WM_SET(P)                                                                          \
+  for (const void *f : {(const void *)k_ring16<kRingBelt16, P>,                            \
+                        (const void *)k_batch_ring16<kRingBelt16, P>})                     \
+    if (err == hipSuccess)                                                                 \
+      err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,             \
+                                (int)(ring16_slots(kRingBelt16, P) * sizeof(double2)));    \
+  for (const void *f : {(const void *)k_ring16<kRingCap16, P>,                             \
+                        (const void *)k_batch_ring16<kRingCap16, P>})                      \
+    if (err == hipSuccess)                                                                 \
+      err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,             \
+                                (int)(ring16_slots(kRingCap16, P) * sizeof(double2)));
+    WM_RING16_SIZES(WM_SET)
+#undef WM_SET
+  });
+  return err;
+}
+
+// pairs order[first .. first + count) of one launch group (kind, FFT size P)
+hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &rt, int first,
+                         int count, double *map, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int64_t nb = ring16_blocks(kind, P, t.K, count);
+  const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
+  switch (P) {
+#define WM_CASE(PP)                                                                        \
+  case PP:                                                                                 \
+    if (kind == kRingBelt16)                                                               \
+      hipLaunchKernelGGL((k_ring16<kRingBelt16, PP>), dim3((unsigned)nb), dim3(RB), lds, s, \
+                         t, rt, first, count, map);                                        \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_ring16<kRingCap16, PP>), dim3((unsigned)nb), dim3(RB), lds, s, \
+                         t, rt, first, count, map);                                        \
+    break;
+    WM_RING16_SIZES(WM_CASE)
+#undef WM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Persistent batch launch of class (kind, P): items r16items[first ..
+// first + count) through queue head ctr, nblocks workgroups.
+hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count, int ctr,
+                               int nblocks, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
+  switch (P) {
+#define WM_CASE(PP)                                                                        \
+  case PP:                                                                                 \
+    if (kind == kRingBelt16)                                                               \
+      hipLaunchKernelGGL((k_batch_ring16<kRingBelt16, PP>), dim3(nblocks), dim3(RB), lds,  \
+                         s, d, first, count, ctr);                                         \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_batch_ring16<kRingCap16, PP>), dim3(nblocks), dim3(RB), lds,   \
+                         s, d, first, count, ctr);                                         \
+    break;
+    WM_RING16_SIZES(WM_CASE)
+#undef WM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu) {
+  const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
+  switch (P) {
+#define WM_CASE(PP)                                                                        \
+  case PP:                                                                                 \
+    return kind == kRingBelt16                                                             \
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
+                     blocks_per_cu, k_batch_ring16<kRingBelt16, PP>, RB, lds)              \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
+                     blocks_per_cu, k_batch_ring16<kRingCap16, PP>, RB, lds);
+    WM_RING16_SIZES(WM_CASE)
+#undef WM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace wm

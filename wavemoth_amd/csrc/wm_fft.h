@@ -157,27 +157,6 @@ static __device__ void lds_fft(double2 *buf, int P, const double2 *twl) {
   }
 }
 
-// The same FFT inlined, for P <= MAXP: the register footprint of the
-// passes scales with MAXP, and a call would pin the caller's register
-// budget at the callee's.
-template <bool INV, int MAXP>
-static __device__ __forceinline__ void lds_fft_inl(double2 *buf, int P, const double2 *twl) {
-  int Ns = 1;
-  while (Ns < P) {
-    const int rem = P / Ns;
-    if (rem >= 8) {
-      stockham_pass<8, INV, MAXP>(buf, P, Ns, twl);
-      Ns *= 8;
-    } else if (rem == 4) {
-      stockham_pass<4, INV, MAXP>(buf, P, Ns, twl);
-      Ns *= 4;
-    } else {
-      stockham_pass<2, INV, MAXP>(buf, P, Ns, twl);
-      Ns *= 2;
-    }
-  }
-}
-
 // Fills the LDS twiddle table from the global one (block-wide; syncs).
 static __device__ __forceinline__ void load_twiddles(double2 *twl, const double2 *__restrict__ tw) {
   for (int u = threadIdx.x; u < kTwSlots; u += NT)

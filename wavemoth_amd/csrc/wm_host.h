@@ -85,6 +85,7 @@ struct RingHost {
   std::vector<RingPair> pairs;
   std::vector<int> need;     // complex LDS slots of the ring kernel
   std::vector<double> cost;  // relative cost of the pair
+  std::vector<int> kind;     // RingKind of the pair
   int64_t npix = 0;
 };
 

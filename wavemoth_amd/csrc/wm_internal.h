@@ -62,6 +62,13 @@ __host__ __device__ inline int64_t q_index(int r, int p, int k, int m, int K, in
 __host__ __device__ inline int64_t q_elems(int R, int K, int Mp1) {
   return (int64_t)R * 2 * q_groups(K) * q_group(K) * Mp1;
 }
+// Rows of q past the last (ring pair, parity, group) that the ring kernels
+// may read (and discard): a row index up to kMaxFFT/2 past mmax stays
+// inside the allocation.
+constexpr int64_t kQTail = (int64_t)(kMaxFFT / 2) * 8;
+__host__ __device__ inline int64_t q_alloc_elems(int R, int K, int Mp1) {
+  return q_elems(R, K, Mp1) + kQTail;
+}
 
 // One ring pair of the output map (northern ring + its southern mirror),
 // for the ring (phase + alias + c2r) stage.
@@ -70,13 +77,20 @@ struct RingPair {
   int64_t pix_s;     // first pixel of the southern ring, -1 for the equator
   int64_t chirp_off; // Bluestein chirp table offset (h entries)
   int64_t vhat_off;  // Bluestein kernel spectrum offset (P entries)
+  int64_t chirp2_off; // split Bluestein (length h/2) chirp offset, -1 if none
+  int64_t vhat2_off;  // its kernel spectrum offset (P2 entries)
+  int P2;            // split Bluestein FFT length next_pow2(h - 1), 0 if none
   int n;             // pixels per ring
   int s;             // phi0 = pi * s / n, s in {0, 1}
   int P;             // Bluestein FFT length, 0 when n/2 is a power of two
   int r;             // ring pair index, 0 = equator
 };
 
-constexpr int kMaxLaunches = 8;
+constexpr int kMaxLaunches = 16;
+
+// Ring kernel of a launch group: the general LDS kernel (k_ring_pairs) or a
+// register-FFT kernel of ring16.hip.
+enum RingKind { kRingGeneral = 0, kRingBelt16 = 1, kRingCap16 = 2 };
 
 struct RingTables {
   int npairs = 0;
@@ -85,13 +99,16 @@ struct RingTables {
   double2 *twiddle = nullptr; // [kMaxFFT] e^{-2 pi i t / kMaxFFT}
   double2 *chirp = nullptr;
   double2 *vhat = nullptr;
+  double2 *chirp2 = nullptr;  // split Bluestein tables (k_ring16_cap)
+  double2 *vhat2 = nullptr;
   // launch groups (host side): order[first[i] .. first[i]+count[i]) with
   // lds[i] complex slots of dynamic LDS
   int nlaunch = 0;
   int first[kMaxLaunches] = {0};
   int count[kMaxLaunches] = {0};
   int lds[kMaxLaunches] = {0};
-  int belt[kMaxLaunches] = {0};  // 1: group of non-aliasing power-of-two rings
+  int kind[kMaxLaunches] = {0};  // RingKind
+  int fftP[kMaxLaunches] = {0};  // FFT length of a register-FFT group
 };
 
 // Batched execution of many independent transforms: prescale, Legendre
@@ -103,7 +120,13 @@ struct BatchLegItem {
 struct BatchRingItem {
   int req, pair, k;  // request, ring pair, map
 };
-constexpr int kBatchCounters = 4;  // queue heads: sweep, ring class 0, ring class 1
+// (request, launch group of its ring tables, block of that launch) of the
+// batched register-FFT ring kernels
+struct BatchRing16Item {
+  int req, g, b;
+};
+// queue heads: sweep, two general ring classes, register-FFT ring classes
+constexpr int kBatchCounters = 32;
 struct BatchDev {
   int nreq = 0;
   const LegTables *tabs = nullptr;      // [nreq] per-request tables (K, KB, RL, buffers)
@@ -115,6 +138,7 @@ struct BatchDev {
   const BatchLegItem *litems = nullptr;
   int nlitems = 0;
   const BatchRingItem *ritems = nullptr;
+  const BatchRing16Item *r16items = nullptr;
   const double2 *twiddle = nullptr;     // [kMaxFFT] shared twiddle table
   int *counters = nullptr;              // [kBatchCounters] work-queue heads
 };
@@ -165,14 +189,23 @@ hipError_t launch_seeds(const LegTables &t, hipStream_t s);
 hipError_t launch_wave_bounds(const LegTables &t, hipStream_t s);
 hipError_t launch_prep_alm(const LegTables &t, const double2 *alm, hipStream_t s);
 hipError_t launch_legendre(const LegTables &t, hipStream_t s);
-hipError_t launch_bluestein_prep(const RingTables &rt, const int *hlist, const int64_t *choff,
-                                 const int64_t *vhoff, const int *plist, int nh,
-                                 hipStream_t s);
+// Bluestein tables of the lengths hlist[0..nh): chirp[choff + a] =
+// e^{i pi a^2/h} (a < h), vhat[vhoff + .] = FFT_P(conj chirp, cyclic) / P.
+hipError_t launch_bluestein_prep(const double2 *twiddle, double2 *chirp, double2 *vhat,
+                                 const int *hlist, const int64_t *choff, const int64_t *vhoff,
+                                 const int *plist, int nh, hipStream_t s);
 int ring_lds_slots(int n, int P, int M);
-// Belt-ring kernel (power-of-two n/2 <= ring_belt_max_h(), mmax <= n/2).
-int ring_belt_lds_slots(int n);
-int ring_belt_max_h();
 hipError_t ring_set_lds_limit(int slots);
+// Register-FFT ring kernels (ring16.hip) for FFT lengths P = 16 .. 4096.
+bool ring16_size_ok(int P);
+int ring16_lds_slots(int kind, int P);
+int64_t ring16_blocks(int kind, int P, int K, int count);
+hipError_t ring16_set_lds_limits();
+hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &rt, int first,
+                         int count, double *map, hipStream_t s);
+hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count, int ctr,
+                               int nblocks, hipStream_t s);
+hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu);
 int adjoint_groups(const LegTables &t);
 hipError_t adjoint_set_lds_limit(int slots);
 hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double *map,
