@@ -37,8 +37,10 @@ struct Geometry {
   int N = 0, L = 0, M = 0;
   LegTables base;  // tables of (N, L, M); no wave bounds, no per-map buffers
   LegCounts cnt;
-  RingTables rt;
-  RingHost rhost;
+  // ring tables for few maps ([0]) and for kRingWideMaps or more ([1])
+  std::array<RingTables, 2> rt;
+  std::array<RingHost, 2> rhost;
+  std::array<bool, 2> have_rt{};
   std::array<LegTables, 5> waves;  // base + wave bounds, indexed by RL
   std::array<int64_t, 5> T_exec{};
   std::array<bool, 5> have{};
@@ -129,9 +131,23 @@ Geometry *get_geometry(wavemoth_batch b, int N, int L, int M) {
   g->L = L;
   g->M = M;
   if (!build_leg_geometry(b->arena, g->base, N, L, M, b->stream, &g->cnt)) return nullptr;
-  if (!build_ring_geometry(b->arena, g->rt, g->rhost, N, M, true, b->stream)) return nullptr;
   b->geoms.push_back(std::move(g));
   return b->geoms.back().get();
+}
+
+// Ring tables of a geometry for requests of K maps (built on first use).
+bool ensure_rings(wavemoth_batch b, Geometry *g, int K) {
+  const int w = K >= kRingWideMaps ? 1 : 0;
+  if (g->have_rt[w]) return true;
+  if (!build_ring_geometry(b->arena, g->rt[w], g->rhost[w], g->N, g->M, true, w == 1,
+                           b->stream))
+    return false;
+  g->have_rt[w] = true;
+  return true;
+}
+const RingTables &rings_of(const Geometry *g, int K) { return g->rt[K >= kRingWideMaps ? 1 : 0]; }
+const RingHost &rhost_of(const Geometry *g, int K) {
+  return g->rhost[K >= kRingWideMaps ? 1 : 0];
 }
 
 bool ensure_waves(DevArena &a, Geometry *g, int RL, hipStream_t s) {
@@ -154,7 +170,7 @@ bool request_tables(DevArena &keep, DevArena &a, Geometry *g, int K, Shape sh, h
   const int nkb = (K + sh.KB - 1) / sh.KB;
   if (!dalloc(a, &t.atil, (size_t)t.nlpad * nkb * sh.KB, "batch atil")) return false;
   if (!dalloc(a, &t.qbuf, (size_t)q_alloc_elems(t.R, K, t.M + 1), "batch qbuf")) return false;
-  t.npix = g->rhost.npix;
+  t.npix = 12 * (int64_t)g->N * g->N;
   *out = t;
   return true;
 }
@@ -223,7 +239,9 @@ double time_shape(wavemoth_batch b, Geometry *g, int K, Shape sh) {
     sweep_items(t, 0, items);
     BatchDev d;
     void **dp = nullptr;
-    if (!make_dev(a, {t}, {g->rt}, items, {}, {}, g->rt.twiddle, &d, &dp)) break;
+    if (!ensure_rings(b, g, K)) break;
+    if (!make_dev(a, {t}, {rings_of(g, K)}, items, {}, {}, rings_of(g, K).twiddle, &d, &dp))
+      break;
     std::vector<void *> ptrs = {(void *)alm, nullptr};
     if (hipMemcpy(dp, ptrs.data(), 2 * sizeof(void *), hipMemcpyHostToDevice) != hipSuccess)
       break;
@@ -370,12 +388,12 @@ bool build_batch(wavemoth_batch b) {
   b->exec_slots.resize(nreq);
   for (int r = 0; r < nreq; ++r) {
     Geometry *g = get_geometry(b, b->N[r], b->L[r], b->M[r]);
-    if (!g) return false;
+    if (!g || !ensure_rings(b, g, b->K[r])) return false;
     b->geom[r] = -1;
     for (size_t i = 0; i < b->geoms.size(); ++i)
       if (b->geoms[i].get() == g) b->geom[r] = (int)i;
     b->nalm[r] = nalm_of(b->L[r], b->M[r]);
-    b->npix[r] = g->rhost.npix;
+    b->npix[r] = rhost_of(g, b->K[r]).npix;
   }
   for (int r = 0; r < nreq; ++r) {
     Geometry *g = b->geoms[b->geom[r]].get();
@@ -415,7 +433,7 @@ bool build_batch(wavemoth_batch b) {
   };
   std::vector<RI> cls[2];
   for (int r = 0; r < nreq; ++r) {
-    const RingHost &h = b->geoms[b->geom[r]]->rhost;
+    const RingHost &h = rhost_of(b->geoms[b->geom[r]].get(), b->K[r]);
     for (int p = 0; p < (int)h.pairs.size(); ++p) {
       if (h.kind[p] != kRingGeneral) continue;
       for (int k = 0; k < b->K[r]; ++k)
@@ -433,7 +451,7 @@ bool build_batch(wavemoth_batch b) {
       return (int64_t)b->npix[x] * b->K[x] > (int64_t)b->npix[y] * b->K[y];
     });
     for (int r : rord) {
-      const RingTables &rt = b->geoms[b->geom[r]]->rt;
+      const RingTables &rt = rings_of(b->geoms[b->geom[r]].get(), b->K[r]);
       for (int g = 0; g < rt.nlaunch; ++g) {
         if (rt.kind[g] == kRingGeneral || rt.count[g] == 0) continue;
         const int64_t nb = ring16_blocks(rt.kind[g], rt.fftP[g], b->K[r], rt.count[g]);
@@ -482,8 +500,8 @@ bool build_batch(wavemoth_batch b) {
   b->nlitems = (int)litems.size();
   b->nritems = (int)ritems.size();
   std::vector<RingTables> rtabs(nreq);
-  for (int r = 0; r < nreq; ++r) rtabs[r] = b->geoms[b->geom[r]]->rt;
-  if (!make_dev(b->arena, b->tabs, rtabs, litems, ritems, r16items, b->geoms[0]->rt.twiddle,
+  for (int r = 0; r < nreq; ++r) rtabs[r] = rings_of(b->geoms[b->geom[r]].get(), b->K[r]);
+  if (!make_dev(b->arena, b->tabs, rtabs, litems, ritems, r16items, rtabs[0].twiddle,
                 &b->dev, &b->d_ptrs))
     return false;
   b->leg_blocks = sweep_blocks(b->cus, b->nlitems);

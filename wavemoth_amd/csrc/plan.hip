@@ -284,7 +284,7 @@ static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 namespace wm {
 
 bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int M,
-                         bool phase_shifts, hipStream_t stream) {
+                         bool phase_shifts, bool wide, hipStream_t stream) {
   const int nrings = 4 * N - 1, mid = 2 * N - 1, R = 2 * N;
   // Ring length of iring and phi0 = pi * num / den of grid entry iring
   // (src/wavemoth.c:1199-1232); den always equals the ring length.
@@ -335,10 +335,6 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       rp.P = 0;
       rp.chirp_off = rp.vhat_off = 0;
       cost[r] = (double)h * std::log2((double)h + 1.0) + (double)(M + 1);
-      if (M <= h && ring16_size_ok(h)) {
-        kind[r] = kRingBelt16;
-        fftP[r] = h;
-      }
     } else {
       const int P = next_pow2(2 * h - 1);
       if (h_index[h] < 0) {
@@ -355,9 +351,16 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       rp.chirp_off = choff[hi];
       rp.vhat_off = vhoff[hi];
       cost[r] = 2.0 * (double)P * std::log2((double)P) + (double)(M + 1);
-      // no wrap (n > mmax): split Bluestein of the even / odd Z entries
+    }
+    if (is_pow2(h) && M <= h && ring16_size_ok(h)) {
+      // non-aliasing power-of-two ring: one FFT of h points
+      kind[r] = kRingBelt16;
+      fftP[r] = h;
+    } else if (n % 4 == 0) {
+      // split Bluestein of the even / odd Z entries (length i = h/2), with
+      // the two-pass fold (n > mmax) or the wrapping fold
       const int i = h / 2, P2 = next_pow2(2 * i - 1);
-      if (n > M && 2 * i == h && ring16_size_ok(P2)) {
+      if (ring16_size_ok(P2)) {
         if (i_index[i] < 0) {
           i_index[i] = (int)h2list.size();
           h2list.push_back(i);
@@ -370,8 +373,16 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
         rp.P2 = P2;
         rp.chirp2_off = ch2off[i_index[i]];
         rp.vhat2_off = vh2off[i_index[i]];
-        kind[r] = kRingCap16;
         fftP[r] = P2;
+        // the two-pass fold covers bins up to P2 - 1: h < P2 unless h is a
+        // power of two (P2 = h), which takes the wrapping fold; that one
+        // walks (mmax + 1) / n row blocks one after the other, so short
+        // rings (P2 < 256) stay with the general kernel
+        if (n > M && h < P2) {
+          kind[r] = kRingCap16;
+        } else if (P2 >= 256 && wide) {
+          kind[r] = kRingWrap16;
+        }
       }
     }
     need[r] = ring_lds_slots(n, rp.P, M);
@@ -380,14 +391,16 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       return false;
     }
   }
-  // Launch groups: register-FFT caps by FFT length (largest first), the
-  // belt, then the general kernel's rings whose LDS need exceeds 80 KiB
-  // (one block per CU) and the rest; costliest first within each.
+  // Launch groups: register-FFT caps and wrapping rings by FFT length
+  // (largest first), the belt, then the general kernel's rings whose LDS
+  // need exceeds 80 KiB (one block per CU) and the rest; costliest first
+  // within each.
   const int big_slots = (80 * 1024) / (int)sizeof(double2);
   auto key = [&](int r) -> int64_t {
     if (kind[r] == kRingCap16) return 0 * 100000 + (8192 - fftP[r]);
-    if (kind[r] == kRingBelt16) return 1 * 100000 + (8192 - fftP[r]);
-    return (need[r] > big_slots ? 2 : 3) * 100000;
+    if (kind[r] == kRingWrap16) return 1 * 100000 + (8192 - fftP[r]);
+    if (kind[r] == kRingBelt16) return 2 * 100000 + (8192 - fftP[r]);
+    return (need[r] > big_slots ? 3 : 4) * 100000;
   };
   std::vector<int> order(R);
   for (int i = 0; i < R; ++i) order[i] = i;
@@ -466,7 +479,8 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
 }  // namespace wm
 
 static bool build_ring_tables(wavemoth_plan p) {
-  if (!build_ring_geometry(p->arena, p->rt, p->rhost, p->N, p->M, p->phase_shifts, p->stream))
+  if (!build_ring_geometry(p->arena, p->rt, p->rhost, p->N, p->M, p->phase_shifts,
+                           p->K >= kRingWideMaps, p->stream))
     return false;
   p->npix = p->rhost.npix;
   return true;

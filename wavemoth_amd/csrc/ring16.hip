@@ -66,7 +66,7 @@ __device__ __forceinline__ bool ring16_job(int b, int J, int jb, int K, int coun
 // Blocks of a launch: belt jobs are T = P/16 threads, cap jobs 2T threads,
 // both in 512-thread blocks.
 __host__ __device__ constexpr int ring16_jobs(int kind, int P) {
-  return kind == kRingCap16 ? RB / (P / 8) : RB / (P / 16);
+  return kind == kRingBelt16 ? RB / (P / 16) : RB / (P / 8);
 }
 
 int64_t ring16_blocks(int kind, int P, int K, int count) {
@@ -79,7 +79,7 @@ int64_t ring16_blocks(int kind, int P, int K, int count) {
 }
 
 __host__ __device__ constexpr int ring16_slots(int kind, int P) {
-  return (kind == kRingCap16 ? 2 : 1) * ring16_jobs(kind, P) * job_slots(P) + kTwSlots;
+  return (kind == kRingBelt16 ? 1 : 2) * ring16_jobs(kind, P) * job_slots(P) + kTwSlots;
 }
 int ring16_lds_slots(int kind, int P) { return ring16_slots(kind, P); }
 
@@ -255,10 +255,65 @@ __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables 
 // in its own LDS buffer, so that no thread holds one transform's result
 // through the other's.
 
+// Fold of a ring whose m-range wraps (n <= mmax), into F[0 .. h] at
+// buf[pad(.)] (buf spans the job's two buffers, >= n padded slots):
+//   P_rho = sum_w (-1)^{s w} g_{w n + rho}  (rho < n, blocks of n rows),
+//   F_b = P_b + (-1)^s conj P_{n-b}  (0 < b <= h),
+//   F_0 = P_0 + conj(P_0 - g_0),
+// the direct terms m = b + w n and conjugate terms m = w n - b of every bin
+// regrouped by row block.  Thread tt owns bins rho = tt + 2T q (q < 16;
+// n < 2P = 16 (2T)) and sums their row blocks in order in registers.
+template <int P>
+__device__ __forceinline__ void wrap_fold(const double2 *__restrict__ qe,
+                                          const double2 *__restrict__ qo, int64_t qs, int M,
+                                          int n, int s, double2 *buf, int tt, bool on, double hs) {
+  constexpr int T2 = P / 8;
+  const int h = n / 2;
+  double2 acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = make_double2(0.0, 0.0);
+  double2 g0 = make_double2(0.0, 0.0);
+  for (int w0 = 0; w0 <= M; w0 += n) {
+    const double se = (s && ((w0 / n) & 1)) ? -1.0 : 1.0;
+    const double so = se * hs;
+    double2 e[16], o[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int m = w0 + tt + T2 * q;
+      const int64_t mc = m <= M ? m : M;
+      e[q] = qe[mc * qs];
+      o[q] = qo[mc * qs];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int rho = tt + T2 * q, m = w0 + rho;
+      if (on && rho < n && m <= M) {
+        acc[q].x += fma(so, o[q].x, se * e[q].x);
+        acc[q].y += fma(so, o[q].y, se * e[q].y);
+      }
+    }
+    if (w0 == 0 && tt == 0) g0 = make_double2(fma(hs, o[0].x, e[0].x), fma(hs, o[0].y, e[0].y));
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int rho = tt + T2 * q;
+    if (rho < n) buf[pad(rho)] = acc[q];
+  }
+  __syncthreads();
+  const double sg = s ? -1.0 : 1.0;
+  for (int b = tt; b <= h; b += T2) {
+    const double2 pb = buf[pad(b)];
+    const double2 pc = b == 0 ? csub(pb, g0) : buf[pad(n - b)];
+    const double c = b == 0 ? 1.0 : sg;
+    buf[pad(b)] = make_double2(fma(c, pc.x, pb.x), fma(-c, pc.y, pb.y));
+  }
+  __syncthreads();
+}
+
 // One cap ring (hemisphere sign hs) of one map: fold, even / odd Z entries,
 // the two Bluestein convolutions, the pixels.  tt = thread in the job
 // (0 .. 2T), bufA / bufB the job's two buffers.
-template <int P>
+template <int P, bool WRAP>
 __device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__restrict__ qe,
                                          const double2 *__restrict__ qo, int64_t qs, int M,
                                          int K, const double2 *__restrict__ chirp,
@@ -270,6 +325,9 @@ __device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__re
   const int n = rp.n, h = n / 2, i = h / 2;
   const int half = tt / T, j = tt - half * T;
   const double sg = rp.s ? -1.0 : 1.0;
+  if constexpr (WRAP) {
+    wrap_fold<P>(qe, qo, qs, M, n, rp.s, bufA, tt, on, hs);
+  } else {
   // 1. fold into bufA: F_b = g_b (b <= min(h, mmax)), then F_{n-m} +=
   //    (-1)^s conj g_m for h <= m <= mmax (one writer per bin in each
   //    pass); h < P and n > mmax gives mmax - h < h, so 8 rows per thread
@@ -313,6 +371,7 @@ __device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__re
     }
   }
   __syncthreads();
+  }
   asm volatile("" : "+v"(tt));  // no table load hoisted above the barriers
   // 2. Z entry b = 2 kappa + half (kappa = j + T rho < i < P/2, so rho < 8)
   //    times the chirp; phase w_b = e^{i pi s b/n}, partner phase
@@ -374,8 +433,9 @@ __device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__re
   }
 }
 
-// Block b of a cap launch over pairs order[first .. first + count).
-template <int P>
+// Block b of a cap (WRAP: wrapping-ring) launch over pairs
+// order[first .. first + count).
+template <int P, bool WRAP>
 __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &rt, int first,
                                           int count, double *__restrict__ map, int b,
                                           double2 *lds, const double2 *twl) {
@@ -399,7 +459,7 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
   const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
 #pragma unroll 1
   for (int hemi = 0; hemi < 2; ++hemi) {
-    cap_ring<P>(rp, qe, qo, qs, M, K, chirp, vhat, bufA, bufB, twl, tt,
+    cap_ring<P, WRAP>(rp, qe, qo, qs, M, K, chirp, vhat, bufA, bufB, twl, tt,
                 act && (hemi == 0 || south), hemi ? -1.0 : 1.0,
                 map + k + (hemi ? (south ? rp.pix_s : 0) : rp.pix_n) * K);
     __syncthreads();  // buffers reused by the southern ring
@@ -413,7 +473,7 @@ __device__ __forceinline__ void ring16_block(const LegTables &t, const RingTable
   if constexpr (KIND == kRingBelt16)
     belt_block<P>(t, rt, first, count, map, b, lds, twl);
   else
-    cap_block<P>(t, rt, first, count, map, b, lds, twl);
+    cap_block<P, KIND == kRingWrap16>(t, rt, first, count, map, b, lds, twl);
 }
 
 __device__ __forceinline__ void load_twiddles16(double2 *twl, const double2 *__restrict__ tw) {
@@ -462,19 +522,15 @@ hipError_t ring16_set_lds_limits() {
   static std::once_flag once;
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
-#define WM_SET(P)                                                                          \
-  for (const void *f : {(const void *)k_ring16<kRingBelt16, P>,                            \
-                        (const void *)k_batch_ring16<kRingBelt16, P>})                     \
+#define WM_SET_K(KIND, P)                                                                  \
+  for (const void *f : {(const void *)k_ring16<KIND, P>, (const void *)k_batch_ring16<KIND, P>}) \
     if (err == hipSuccess)                                                                 \
       err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,             \
-                                (int)(ring16_slots(kRingBelt16, P) * sizeof(double2)));    \
-  for (const void *f : {(const void *)k_ring16<kRingCap16, P>,                             \
-                        (const void *)k_batch_ring16<kRingCap16, P>})                      \
-    if (err == hipSuccess)                                                                 \
-      err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,             \
-                                (int)(ring16_slots(kRingCap16, P) * sizeof(double2)));
+                                (int)(ring16_slots(KIND, P) * sizeof(double2)));
+#define WM_SET(P) WM_SET_K(kRingBelt16, P) WM_SET_K(kRingCap16, P) WM_SET_K(kRingWrap16, P)
     WM_RING16_SIZES(WM_SET)
 #undef WM_SET
+#undef WM_SET_K
   });
   return err;
 }
@@ -485,22 +541,17 @@ hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &
   if (count <= 0) return hipSuccess;
   const int64_t nb = ring16_blocks(kind, P, t.K, count);
   const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
-  switch (P) {
-#define WM_CASE(PP)                                                                        \
-  case PP:                                                                                 \
-    if (kind == kRingBelt16)                                                               \
-      hipLaunchKernelGGL((k_ring16<kRingBelt16, PP>), dim3((unsigned)nb), dim3(RB), lds, s, \
-                         t, rt, first, count, map);                                        \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_ring16<kRingCap16, PP>), dim3((unsigned)nb), dim3(RB), lds, s, \
-                         t, rt, first, count, map);                                        \
-    break;
-    WM_RING16_SIZES(WM_CASE)
-#undef WM_CASE
-    default:
-      return hipErrorInvalidValue;
+#define WM_CASE_K(KIND, PP)                                                                \
+  if (kind == KIND && P == PP) {                                                           \
+    hipLaunchKernelGGL((k_ring16<KIND, PP>), dim3((unsigned)nb), dim3(RB), lds, s, t, rt,  \
+                       first, count, map);                                                 \
+    return hipGetLastError();                                                              \
   }
-  return hipGetLastError();
+#define WM_CASE(PP) WM_CASE_K(kRingBelt16, PP) WM_CASE_K(kRingCap16, PP) WM_CASE_K(kRingWrap16, PP)
+  WM_RING16_SIZES(WM_CASE)
+#undef WM_CASE
+#undef WM_CASE_K
+  return hipErrorInvalidValue;
 }
 
 // Persistent batch launch of class (kind, P): items r16items[first ..
@@ -509,39 +560,30 @@ hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, in
                                int nblocks, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
-  switch (P) {
-#define WM_CASE(PP)                                                                        \
-  case PP:                                                                                 \
-    if (kind == kRingBelt16)                                                               \
-      hipLaunchKernelGGL((k_batch_ring16<kRingBelt16, PP>), dim3(nblocks), dim3(RB), lds,  \
-                         s, d, first, count, ctr);                                         \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_batch_ring16<kRingCap16, PP>), dim3(nblocks), dim3(RB), lds,   \
-                         s, d, first, count, ctr);                                         \
-    break;
-    WM_RING16_SIZES(WM_CASE)
-#undef WM_CASE
-    default:
-      return hipErrorInvalidValue;
+#define WM_CASE_K(KIND, PP)                                                                \
+  if (kind == KIND && P == PP) {                                                           \
+    hipLaunchKernelGGL((k_batch_ring16<KIND, PP>), dim3(nblocks), dim3(RB), lds, s, d,     \
+                       first, count, ctr);                                                 \
+    return hipGetLastError();                                                              \
   }
-  return hipGetLastError();
+#define WM_CASE(PP) WM_CASE_K(kRingBelt16, PP) WM_CASE_K(kRingCap16, PP) WM_CASE_K(kRingWrap16, PP)
+  WM_RING16_SIZES(WM_CASE)
+#undef WM_CASE
+#undef WM_CASE_K
+  return hipErrorInvalidValue;
 }
 
 hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu) {
   const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
-  switch (P) {
-#define WM_CASE(PP)                                                                        \
-  case PP:                                                                                 \
-    return kind == kRingBelt16                                                             \
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
-                     blocks_per_cu, k_batch_ring16<kRingBelt16, PP>, RB, lds)              \
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
-                     blocks_per_cu, k_batch_ring16<kRingCap16, PP>, RB, lds);
-    WM_RING16_SIZES(WM_CASE)
+#define WM_CASE_K(KIND, PP)                                                                \
+  if (kind == KIND && P == PP)                                                             \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                     \
+                                                        k_batch_ring16<KIND, PP>, RB, lds);
+#define WM_CASE(PP) WM_CASE_K(kRingBelt16, PP) WM_CASE_K(kRingCap16, PP) WM_CASE_K(kRingWrap16, PP)
+  WM_RING16_SIZES(WM_CASE)
 #undef WM_CASE
-    default:
-      return hipErrorInvalidValue;
-  }
+#undef WM_CASE_K
+  return hipErrorInvalidValue;
 }
 
 }  // namespace wm

@@ -90,7 +90,9 @@ struct RingHost {
 };
 
 // Ring geometry, Bluestein tables and the single-transform launch groups.
+// wide: plan for kRingWideMaps or more maps (wrapping rings then take the
+// register-FFT kernel, which needs many maps to fill the device).
 bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int M,
-                         bool phase_shifts, hipStream_t s);
+                         bool phase_shifts, bool wide, hipStream_t s);
 
 }  // namespace wm

@@ -90,7 +90,11 @@ constexpr int kMaxLaunches = 16;
 
 // Ring kernel of a launch group: the general LDS kernel (k_ring_pairs) or a
 // register-FFT kernel of ring16.hip.
-enum RingKind { kRingGeneral = 0, kRingBelt16 = 1, kRingCap16 = 2 };
+enum RingKind { kRingGeneral = 0, kRingBelt16 = 1, kRingCap16 = 2, kRingWrap16 = 3 };
+// From this many maps the wrapping rings (n <= mmax) take the register-FFT
+// kernel; with fewer, its few long-latency jobs leave the device idle and
+// the general kernel is faster.
+constexpr int kRingWideMaps = 8;
 
 struct RingTables {
   int npairs = 0;
