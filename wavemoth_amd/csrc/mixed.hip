@@ -12,9 +12,9 @@
 // monic scaling, seeds and chunk rescale as the FP64 kernel).  Every 64 l
 // (a super-chunk) each lane writes its 32 values, converted to the scheme's
 // input format, into a wave-private LDS tile; the wave then reads MFMA
-// A-fragments from it.  The
-// B operand (prescaled a_lm, pre-split at prescale time) is staged in LDS
-// per workgroup, prefetched one super-chunk ahead.  MFMA accumulates in
+// A-fragments from it.  The B operand (prescaled a_lm, pre-split at
+// prescale time) is read by each wave straight into MFMA B-fragments, so
+// the waves of a workgroup never wait for each other.  MFMA accumulates in
 // FP32; every kFlush super-chunks the FP32 sums are added into FP64
 // accumulators (the FP64 reduction across l-chunks), which are written to
 // the same q buffer the FP64 path fills.
@@ -31,6 +31,7 @@
 //            when the FP32 sums are added into FP64.
 // BF16 schemes use v_mfma_f32_16x16x32_bf16, FP16x3 v_mfma_f32_16x16x32_f16.
 #include <cmath>
+#include <type_traits>
 
 #include "wm_internal.h"
 
@@ -39,6 +40,9 @@ namespace wm {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+// native 16-byte vector: arrays of it stay in registers (HIP's uint4 struct
+// in an array held across a loop is kept in scratch memory)
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 #define WM_CONST __attribute__((address_space(4)))
 template <typename T>
@@ -46,10 +50,8 @@ __device__ __forceinline__ const WM_CONST T *cview(const T *p) {
   return (const WM_CONST T *)p;
 }
 
-constexpr int kMixWaves = 4;
-constexpr int kMixRW = 32;                     // rings per wave
-constexpr int kMixRB = kMixWaves * kMixRW;     // rings per workgroup (= 64 * RL with RL 2)
-constexpr int kMixCols = 32;                   // real columns per workgroup
+constexpr int kMixCols = 32;                   // real columns per column block
+constexpr int kMixCB = 4;                      // column blocks per workgroup (at most)
 constexpr int kSC = 64;                        // l per super-chunk
 constexpr int kKK = kSC / 2;                   // same-parity l per super-chunk
 constexpr int kFlush = 2;                      // super-chunks per FP32 accumulation
@@ -206,32 +208,46 @@ __global__ __launch_bounds__(256) void k_mix_prep(LegTables t, const double2 *__
 
 // ------------------------------------------------------------- contraction
 
-template <int S>
+// Workgroup over 64 ring pairs of one m and CB column blocks of 32 columns:
+// waves 0 and 1 are producers (parity 0 / 1: one ring per lane, the FP64
+// recurrence, conversion to the scheme's parts, the A tile into LDS, plus
+// staging of the B rows), waves 2 + 2 c + p consumers (column block c,
+// parity p: MFMA over the tiles, FP32 sums, FP64 reduction).  The Lambda
+// tile is produced once and read by every column block; tiles and B rows
+// are double-buffered, one workgroup barrier per super-chunk.
+template <int S, int CB>
 struct MixLds {
-  // A tile per wave and part: 64 rings x (32 values of the wave's parity + pad)
+  static constexpr int P = MixScheme<S>::parts;
+  // A tile per buffer, parity and part: 64 rings x (32 values + pad)
   static constexpr int a_row = (S == 0) ? (kKK + 4) : (kKK + 8);  // elements
   static constexpr int a_elem = (S == 0) ? 4 : 2;                   // bytes
-  static constexpr int a_part = 64 * a_row;                         // elements
-  static constexpr int a_wave = MixScheme<S>::parts * a_part * a_elem;
-  // B tile: per part, 4 chunks x 2 parities x 8 l x 32 cols
-  static constexpr int b_part = 4 * 2 * 8 * kMixCols * ((S == 0) ? 4 : 2);
-  static constexpr int bytes = kMixWaves * a_wave + MixScheme<S>::parts * b_part;
+  static constexpr int a_part = 64 * a_row * a_elem;                // bytes
+  static constexpr int a_par = P * a_part;
+  static constexpr int a_buf = 2 * a_par;
+  // B rows per buffer: FP32 [4 chunks][2 parities][8 l][32 CB cols + 16]
+  // floats (the pad puts the 4 l of one MFMA step on distinct banks);
+  // 16-bit per part [4 chunks][2 parities][32 CB cols][8 l]
+  static constexpr int b_row = kMixCols * CB + 16;  // FP32 row stride (floats)
+  static constexpr int b_part = (S == 0) ? 64 * b_row * 4 : 4 * 2 * 8 * kMixCols * CB * 2;
+  static constexpr int b_buf = P * b_part;
+  static constexpr int b_u128 = P * 4 * 2 * 8 * kMixCols * CB * ((S == 0) ? 4 : 2) / 16;
+  // recurrence coefficients per buffer and parity: d_{l+2}, e_{l+2} and
+  // (FP16) A_l of the 32 l of the parity, as doubles
+  static constexpr int c_par = 100 * 8;  // + the 4 chunk rescale factors
+  static constexpr int a_off = 0;
+  static constexpr int b_off = 2 * a_buf;
+  static constexpr int c_off = b_off + 2 * b_buf;
+  static constexpr int bytes = c_off + 2 * 2 * c_par;
 };
 
-// Workgroup = 4 waves over 128 ring pairs x 32 columns of one m: wave w
-// takes rings 64 (w >> 1) .. +63 (one per lane) and the l of parity w & 1,
-// advanced two at a time by L~_{l+2} = (x^2 - d_l) L~_l - e_l L~_{l-2}
-// (d_l = c~_l + c~_{l+1}, e_l = c~_l c~_{l-1}; the x^2 form of
-// src/legendre_transform.c.in:680-716), so every coefficient is
-// wave-uniform and no lane repeats another's work.
-template <int S>
-__global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, int NCp, int ncb,
-                                                int bpm, int64_t part_stride, double lam_scale,
-                                                const unsigned long long *colmax,
-                                                const double *dtil, const double *etil) {
+template <int S, int CB>
+__global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
+    LegTables t, const void *bmat, int NCp, int ncb, int bpm, int64_t part_stride,
+    double lam_scale, const unsigned long long *colmax, const double *dtil, const double *etil) {
   using Sc = MixScheme<S>;
-  using Ly = MixLds<S>;
+  using Ly = MixLds<S, CB>;
   constexpr int P = Sc::parts;
+  constexpr int NT = 64 * (2 + 2 * CB);
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
 
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -241,19 +257,22 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
   const int m = (b & 7) + 8 * (s8 / bpm);
   if (m > t.M) return;
   const int item = s8 % bpm;
-  const int cb = item / t.W, wblk = item - cb * t.W;
+  const int cg = item / t.W, wblk = item - cg * t.W;  // column group, ring block
   const int L = t.L, R = t.R, Mp1 = t.M + 1, K = t.K;
-  const int pw = wv & 1;                            // parity of this wave
-  const int ring0 = wblk * kMixRB + (wv >> 1) * 64;  // first ring of this wave
-
-  // output mapping of the 16x16 MFMA tiles: row 4 g + r, column i16
+  const int ring0 = wblk * 64;
+  const bool producer = wv < 2;
+  const int cw = wv - 2;                 // consumer index
+  const int pw = producer ? wv : (cw & 1);  // parity of this wave
+  const int cb = cg * CB + (cw >> 1);       // column block of a consumer
+  const bool cons_on = !producer && cb < ncb;
   const int i16 = lane & 15, g = lane >> 4;
-  const int col0 = cb * kMixCols;
+  const int colg0 = cg * CB * kMixCols;     // first column of the group
 
   int l0 = __builtin_amdgcn_readfirstlane(t.wstart[(int64_t)m * t.W + wblk]);
   const int lsmax = __builtin_amdgcn_readfirstlane(t.wseedmax[(int64_t)m * t.W + wblk]);
+  const int64_t offm = t.off[m];
 
-  double qd[4][2][4];  // [row tile][col tile][r]
+  double qd[4][2][4];  // consumers: [row tile][col tile][r]
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -262,128 +281,143 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
       for (int r = 0; r < 4; ++r) qd[rt][ct][r] = 0.0;
 
   if (l0 <= L) {
-    const int64_t offm = t.off[m];
+    const int nsc = (L - l0) / kSC + 1;  // super-chunks, block-uniform
+    // ---------------------------------------------------------- producer
     const WM_CONST double *ct_ = cview(t.ctil + (offm - m));
-    const WM_CONST double *dt_ = cview(dtil + (offm - m));
-    const WM_CONST double *et_ = cview(etil + (offm - m));
-    const WM_CONST double *cs_ = cview(t.cscale + offm / kChunk);
-    const WM_CONST double *as_ = cview(t.ascale + (offm - m));  // A_l, 0 past lmax
     const int nch = (L - m + kChunk) / kChunk;  // chunks of m's padded l-range
-
     const int rr = ring0 + lane;
-    const bool rok = rr < R;
+    const bool rok = producer && rr < R;
     const double x = rok ? t.ring_x[rr] : 0.0;
     const double x2 = x * x;
     const int sl = rok ? t.seed_l[(int64_t)m * R + rr] : kSeedNone;
     double u = 0.0, v = 0.0;  // L~ at the next two l of parity pw
-
-    unsigned char *abase = lds + wv * Ly::a_wave;
-    unsigned char *bbase = lds + kMixWaves * Ly::a_wave;
-
-    // B staging: thread -> 16 (BF16/FP16, per part) or 32 (FP32) bytes per
-    // super-chunk, prefetched into b0..b2 one super-chunk ahead
-    const int tid = threadIdx.x;
-    const int brow = (S == 0) ? (tid >> 2) : (tid >> 5);  // FP32: (cc, p, i); 16-bit: (cc, p)
-    const int bcol = (S == 0) ? 8 * (tid & 3) : (tid & 31);
-    const char *bsrc0;
-    if constexpr (S == 0)
-      bsrc0 = (const char *)((const float *)bmat +
-                             ((int64_t)(brow >> 4) * 16 + (brow & 15)) * NCp + col0 + bcol);
-    else
-      bsrc0 = (const char *)((const unsigned short *)bmat +
-                             ((int64_t)(brow >> 1) * 2 * NCp + (brow & 1) * NCp + col0 + bcol) * 8);
-    // byte stride between consecutive chunks of the B layout
-    const int64_t bchunk = (S == 0) ? (int64_t)16 * NCp * 4 : (int64_t)2 * NCp * 8 * 2;
-    const int64_t bpart = part_stride * 2;
-    unsigned char *bdst = (S == 0) ? bbase + (brow * kMixCols + bcol) * 4
-                                   : bbase + (brow * kMixCols + bcol) * 16;
-    uint4 b0, b1, b2;
-#define WM_LOAD_B(lsc)                                                              \
-  do {                                                                              \
-    const char *src_ = bsrc0 + (offm / kChunk + ((lsc) - m) / kChunk) * bchunk;     \
-    b0 = *(const uint4 *)src_;                                                      \
-    if constexpr (S == 0) b1 = *(const uint4 *)(src_ + 16);                         \
-    if constexpr (S >= 1) b1 = *(const uint4 *)(src_ + bpart);                      \
-    if constexpr (S == 2) b2 = *(const uint4 *)(src_ + 2 * bpart);                  \
-  } while (0)
-#define WM_STORE_B()                                                                \
-  do {                                                                              \
-    *(uint4 *)bdst = b0;                                                            \
-    if constexpr (S == 0) *(uint4 *)(bdst + 16) = b1;                               \
-    if constexpr (S >= 1) *(uint4 *)(bdst + Ly::b_part) = b1;                       \
-    if constexpr (S == 2) *(uint4 *)(bdst + 2 * Ly::b_part) = b2;                   \
-  } while (0)
-
-    // undoes the FP16 scheme's scales of this lane's two columns (exact)
-    double unscale[2] = {1.0, 1.0};
-    if constexpr (S == 3) {
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        const int col = col0 + 16 * ct + i16;
-        unscale[ct] = 1.0 / (lam_scale * col_scale(colmax, col < NCp ? col : 0));
-      }
+    // this lane's start: three-term steps from its seed (L~_{sl-1}, L~_{sl})
+    // to the parity's pair (L~_sl, L~_sl+2) or (L~_sl+1, L~_sl+3), computed
+    // up front so that the step loop never waits on memory
+    double us = 0.0, vs = 0.0;
+    if (sl <= L) {
+      const double2 sv = t.seed_v[(int64_t)m * R + rr];
+      const double t1 = fma(x, sv.y, -(ct_[sl] * sv.x));
+      const double t2 = fma(x, t1, -(ct_[sl + 1] * sv.y));
+      const double t3 = fma(x, t2, -(ct_[sl + 2] * t1));
+      us = pw ? t1 : sv.y;
+      vs = pw ? t3 : t2;
     }
-    f32x4 acc[4][2];
-    auto zero_acc = [&]() {
-#pragma unroll
-      for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    };
-    auto flush = [&]() {
-#pragma unroll
-      for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) qd[rt][ct][r] += unscale[ct] * (double)acc[rt][ct][r];
-      zero_acc();
-    };
-    zero_acc();
 
-    WM_LOAD_B(l0);
-    int nsc = 0;
-    for (; l0 <= L; l0 += kSC) {
-      __syncthreads();  // previous super-chunk's B reads are done
-      WM_STORE_B();
-      __syncthreads();
-      if (l0 + kSC <= L) WM_LOAD_B(l0 + kSC);
+    // B rows of a super-chunk, moved by the 128 CB consumer lanes: loaded
+    // into registers one super-chunk ahead of their store into an LDS
+    // buffer (macros, so that the register array is never addressed
+    // indirectly)
+    constexpr int NC = 128 * CB;          // consumer lanes
+    constexpr int BU = Ly::b_u128 / NC;   // 16-byte vectors per consumer lane
+    // per-lane byte offsets of this lane's B vectors: source relative to the
+    // super-chunk's first row, destination within an LDS buffer (columns
+    // past the last block read column 0)
+    auto b_offsets = [&](int x, int64_t *src, int *dst) {
+      const int e = threadIdx.x - 128 + NC * x;
+      if constexpr (S == 0) {
+        constexpr int U = 8 * CB;  // rows (chunk, parity, i) of 32 CB floats
+        const int row = e / U, cu = e - row * U;
+        const int col = colg0 + 4 * cu < NCp ? colg0 + 4 * cu : 0;
+        *src = ((int64_t)row * NCp + col) * 4;
+        *dst = (row * Ly::b_row + 4 * cu) * 4;
+      } else {
+        constexpr int U = kMixCols * CB;  // per part, rows (chunk, parity)
+        const int q = e / (8 * U), rem = e - q * (8 * U);
+        const int row = rem / U, cu = rem - row * U;
+        const int col = colg0 + cu < NCp ? colg0 + cu : 0;
+        *src = ((int64_t)q * part_stride + ((int64_t)row * NCp + col) * 8) * 2;
+        *dst = q * Ly::b_part + rem * 16;
+      }
+    };
+    // bytes between the B rows of consecutive chunks
+    const int64_t bchunk = (S == 0) ? (int64_t)16 * NCp * 4 : (int64_t)2 * NCp * 8 * 2;
+    const char *bsc0 = (const char *)bmat + (offm / kChunk + (l0 - m) / kChunk) * bchunk;
+#define WM_LOAD_B(breg, sc)                                                                 \
+  do {                                                                                      \
+    const char *p_ = bsc0 + (int64_t)(sc) * 4 * bchunk;                                     \
+    _Pragma("unroll") for (int x = 0; x < BU; ++x) breg[x] = *(const u32x4 *)(p_ + bso[x]); \
+  } while (0)
+#define WM_STORE_B(breg, buf)                                                               \
+  do {                                                                                      \
+    unsigned char *d_ = lds + Ly::b_off + (buf) * Ly::b_buf;                                \
+    _Pragma("unroll") for (int x = 0; x < BU; ++x) *(u32x4 *)(d_ + bdo[x]) = breg[x];       \
+  } while (0)
 
-      // ---- producer: 4 chunks of 16 l; this lane's ring, the wave's parity
+    // Coefficients of the 32 l of this wave's parity in a super-chunk: lane
+    // k < 32 holds d_{l+2}, lane 32 + k e_{l+2} (l = lsc + pw + 2k), and for
+    // FP16 lane k also A_l; loaded one super-chunk ahead, then put in the
+    // wave's LDS slot so that the step loop reads them without waiting on
+    // memory.  Indices past the tables are clamped (those chunks are unused).
+    double creg0 = 0.0, creg1 = 0.0, creg2 = 0.0;
+    auto load_coef = [&](int lsc) {
+      const int k = lane & 31;
+      const int64_t j = offm - m + lsc + pw + 2 * k;  // table index of l
+      const int64_t jd = j + 2 < t.nlpad + 15 ? j + 2 : t.nlpad + 15;
+      creg0 = lane < 32 ? dtil[jd] : etil[jd];
+      if constexpr (S == 3) creg1 = t.ascale[j < t.nlpad ? j : t.nlpad - 1];
+      // chunk rescale factors of the 4 chunks (lanes 0 .. 3)
+      const int64_t jc = (offm - m + lsc) / kChunk + (lane & 3);
+      creg2 = t.cscale[jc < t.nlpad / kChunk ? jc : t.nlpad / kChunk - 1];
+    };
+    auto store_coef = [&](int buf) {
+      double *cb_ = (double *)(lds + Ly::c_off + (buf * 2 + pw) * Ly::c_par);
+      cb_[lane] = creg0;
+      if constexpr (S == 3)
+        if (lane < 32) cb_[64 + lane] = creg1;
+      if (lane < 4) cb_[96 + lane] = creg2;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    // Lambda of super-chunk at lsc (this lane's ring, the wave's parity)
+    // into A buffer buf
+    auto produce = [&](int lsc, int buf) {
+      unsigned char *abase = lds + Ly::a_off + buf * Ly::a_buf + pw * Ly::a_par;
+      const double *cf = (const double *)(lds + Ly::c_off + (buf * 2 + pw) * Ly::c_par);
+      // coefficient sets of two chunks in registers: the next chunk's are
+      // read from LDS while this chunk's steps run
+      constexpr int NA = (S == 3) ? 8 : 1;
+      double cd[2][8], ce[2][8], ca[2][NA], cs[2];
+      auto read_set = [&](int cc, int x) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          cd[x][i] = cf[8 * cc + i];
+          ce[x][i] = cf[32 + 8 * cc + i];
+          if constexpr (S == 3) ca[x][i] = cf[64 + 8 * cc + i];
+        }
+        cs[x] = cf[96 + cc];
+      };
+      read_set(0, 0);
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
-        const int lc = l0 + kChunk * cc;
+        const int x = cc & 1;
+        if (cc + 1 < 4) read_set(cc + 1, x ^ 1);
+        const int lc = lsc + kChunk * cc;
         float f[8];
         if ((lc - m) / kChunk < nch) {
           if (lc <= lsmax && sl == lc) {
-            // three-term steps from the seed (L~_{lc-1}, L~_{lc}) to this
-            // parity's pair: (L~_lc, L~_lc+2) or (L~_lc+1, L~_lc+3)
-            const double2 sv = t.seed_v[(int64_t)m * R + rr];
-            const double t1 = fma(x, sv.y, -(ct_[lc] * sv.x));
-            const double t2 = fma(x, t1, -(ct_[lc + 1] * sv.y));
-            const double t3 = fma(x, t2, -(ct_[lc + 2] * t1));
-            u = pw ? t1 : sv.y;
-            v = pw ? t3 : t2;
+            u = us;
+            v = vs;
           }
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            const int l = lc + pw + 2 * i;  // wave-uniform
+          for (int i = 0; i < 8; ++i) {  // l = lc + pw + 2 i
             if constexpr (S == 3)
-              f[i] = (float)(lam_scale * (u * as_[l]));
+              f[i] = (float)(lam_scale * (u * ca[x][i]));
             else
               f[i] = (float)u;
-            const double w = fma(x2 - dt_[l + 2], v, -(et_[l + 2] * u));
+            const double w = fma(x2 - cd[x][i], v, -(ce[x][i] * u));
             u = v;
             v = w;
           }
-          const double sc = cs_[(lc - m) / kChunk];
-          u *= sc;
-          v *= sc;
+          u *= cs[x];
+          v *= cs[x];
         } else {
 #pragma unroll
           for (int i = 0; i < 8; ++i) f[i] = 0.f;
         }
-        // write the 8 values of this chunk: A[q][ring][8 cc + i]
+        // the 8 values of this chunk: A[q][ring][8 cc + i]
         if constexpr (S == 0) {
           float *dst = (float *)abase + lane * Ly::a_row + 8 * cc;
           *(f32x4 *)dst = (f32x4){f[0], f[1], f[2], f[3]};
@@ -417,18 +451,47 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
               }
             }
           }
-          unsigned short *dst = (unsigned short *)abase + lane * Ly::a_row + 8 * cc;
-          *(uint4 *)dst = make_uint4(wh[0], wh[1], wh[2], wh[3]);
-          *(uint4 *)(dst + Ly::a_part) = make_uint4(wm[0], wm[1], wm[2], wm[3]);
+          unsigned char *dst = abase + (lane * Ly::a_row + 8 * cc) * 2;
+          *(u32x4 *)dst = (u32x4){wh[0], wh[1], wh[2], wh[3]};
+          *(u32x4 *)(dst + Ly::a_part) = (u32x4){wm[0], wm[1], wm[2], wm[3]};
           if constexpr (S == 2)
-            *(uint4 *)(dst + 2 * Ly::a_part) = make_uint4(wl[0], wl[1], wl[2], wl[3]);
+            *(u32x4 *)(dst + 2 * Ly::a_part) = (u32x4){wl[0], wl[1], wl[2], wl[3]};
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
 
-      // ---- consumer: MFMA over the super-chunk (this wave's parity)
+    // ---------------------------------------------------------- consumer
+    // undoes the FP16 scheme's scales of this lane's two columns (exact)
+    double unscale[2] = {1.0, 1.0};
+    if constexpr (S == 3) {
+      if (cons_on) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const int col = cb * kMixCols + 16 * ct + i16;
+          unscale[ct] = 1.0 / (lam_scale * col_scale(colmax, col < NCp ? col : 0));
+        }
+      }
+    }
+    f32x4 acc[4][2];
+    auto zero_acc = [&]() {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    };
+    auto flush = [&]() {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) qd[rt][ct][r] += unscale[ct] * (double)acc[rt][ct][r];
+      zero_acc();
+    };
+    const int cl = (cw >> 1) * kMixCols;  // first column of this consumer in the group
+    auto consume = [&](int buf) {
+      const unsigned char *abase = lds + Ly::a_off + buf * Ly::a_buf + pw * Ly::a_par;
+      const unsigned char *bbase = lds + Ly::b_off + buf * Ly::b_buf;
       if constexpr (S == 0) {
         const float *af = (const float *)abase;
         const float *bf = (const float *)bbase;
@@ -439,67 +502,116 @@ __global__ __launch_bounds__(256, 2) void k_mix(LegTables t, const void *bmat, i
           for (int rt = 0; rt < 4; ++rt) a[rt] = af[(16 * rt + i16) * Ly::a_row + 4 * s + g];
 #pragma unroll
           for (int ct = 0; ct < 2; ++ct)
-            bv[ct] = bf[((((s >> 1) * 2 + pw) * 8) + 4 * (s & 1) + g) * kMixCols + 16 * ct + i16];
+            bv[ct] = bf[((((s >> 1) * 2 + pw) * 8) + 4 * (s & 1) + g) * Ly::b_row + cl +
+                        16 * ct + i16];
 #pragma unroll
           for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
             for (int ct = 0; ct < 2; ++ct)
-              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[ct], acc[rt][ct], 0, 0, 0);
+              acc[rt][ct] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[ct], acc[rt][ct], 0, 0, 0);
         }
       } else {
-        bf16x8 a[P][4], bv[P][2];
+        bf16x8 bv[P][2];
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-#pragma unroll
-          for (int rt = 0; rt < 4; ++rt)
-            a[q][rt] = *(const bf16x8 *)((const unsigned short *)abase + q * Ly::a_part +
-                                         (16 * rt + i16) * Ly::a_row + 8 * g);
+        for (int q = 0; q < P; ++q)
 #pragma unroll
           for (int ct = 0; ct < 2; ++ct)
             bv[q][ct] = *(const bf16x8 *)(bbase + q * Ly::b_part +
-                                          (((g * 2 + pw) * kMixCols) + 16 * ct + i16) * 16);
-        }
+                                          ((g * 2 + pw) * kMixCols * CB + cl + 16 * ct + i16) *
+                                              16);
 #pragma unroll
-        for (int pi = 0; pi < Sc::nprod; ++pi)
+        for (int rt = 0; rt < 4; ++rt) {
+          bf16x8 a[P];
 #pragma unroll
-          for (int rt = 0; rt < 4; ++rt)
+          for (int q = 0; q < P; ++q)
+            a[q] = *(const bf16x8 *)(abase + q * Ly::a_part +
+                                     ((16 * rt + i16) * Ly::a_row + 8 * g) * 2);
+#pragma unroll
+          for (int pi = 0; pi < Sc::nprod; ++pi)
 #pragma unroll
             for (int ct = 0; ct < 2; ++ct) {
               if constexpr (S == 3)
                 acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    __builtin_bit_cast(f16x8, a[Sc::pa(pi)][rt]),
+                    __builtin_bit_cast(f16x8, a[Sc::pa(pi)]),
                     __builtin_bit_cast(f16x8, bv[Sc::pb(pi)][ct]), acc[rt][ct], 0, 0, 0);
               else
                 acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    a[Sc::pa(pi)][rt], bv[Sc::pb(pi)][ct], acc[rt][ct], 0, 0, 0);
+                    a[Sc::pa(pi)], bv[Sc::pb(pi)][ct], acc[rt][ct], 0, 0, 0);
             }
+        }
       }
-      if (++nsc == kFlush) {
-        flush();
-        nsc = 0;
+    };
+
+    // ---------------------------------------------------------- pipeline
+    // two loops with the same barrier sequence (one before the first
+    // super-chunk, one after each), so that each role keeps only its own
+    // registers live
+    if (producer) {
+      load_coef(l0);
+      store_coef(0);
+      if (nsc > 1) load_coef(l0 + kSC);
+      produce(l0, 0);
+      __syncthreads();
+      for (int sc = 0; sc < nsc; ++sc) {
+        if (sc + 1 < nsc) {
+          store_coef((sc + 1) & 1);
+          if (sc + 2 < nsc) load_coef(l0 + kSC * (sc + 2));
+          produce(l0 + kSC * (sc + 1), (sc + 1) & 1);
+        }
+        __syncthreads();
       }
+    } else {
+      int64_t bso[BU];
+      int bdo[BU];
+#pragma unroll
+      for (int x = 0; x < BU; ++x) b_offsets(x, &bso[x], &bdo[x]);
+      u32x4 breg[BU];
+      WM_LOAD_B(breg, 0);
+      WM_STORE_B(breg, 0);
+      if (nsc > 1) WM_LOAD_B(breg, 1);
+      zero_acc();
+      __syncthreads();
+      int nfl = 0;
+      for (int sc = 0; sc < nsc; ++sc) {
+        // B of super-chunk sc + 1 into the buffer read last in sc - 1
+        if (sc + 1 < nsc) {
+          WM_STORE_B(breg, (sc + 1) & 1);
+          if (sc + 2 < nsc) WM_LOAD_B(breg, sc + 2);
+        }
+        if (cons_on) {
+          consume(sc & 1);
+          if (++nfl == kFlush) {
+            flush();
+            nfl = 0;
+          }
+        }
+        __syncthreads();
+      }
+      if (cons_on) flush();
     }
-    flush();
 #undef WM_LOAD_B
 #undef WM_STORE_B
   }
 
-  // ---- store: column col = 2 map + (Re, Im) of q(ring, pw, map, m)
-  double *qd_out = (double *)t.qbuf;
+  // ---- store (consumers): column col = 2 map + (Re, Im) of q(ring, pw, map, m)
+  if (cons_on) {
+    double *qd_out = (double *)t.qbuf;
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int ring = ring0 + 16 * rt + 4 * g + r;
-      if (ring < R) {
+      for (int r = 0; r < 4; ++r) {
+        const int ring = ring0 + 16 * rt + 4 * g + r;
+        if (ring < R) {
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-          const int col = col0 + 16 * ct + i16;
-          if (col < 2 * K)
-            qd_out[2 * q_index(ring, pw, col >> 1, m, K, Mp1) + (col & 1)] = qd[rt][ct][r];
+          for (int ct = 0; ct < 2; ++ct) {
+            const int col = cb * kMixCols + 16 * ct + i16;
+            if (col < 2 * K)
+              qd_out[2 * q_index(ring, pw, col >> 1, m, K, Mp1) + (col & 1)] = qd[rt][ct][r];
+          }
         }
       }
-    }
+  }
 }
 
 // ------------------------------------------------------------- launchers
@@ -526,32 +638,47 @@ hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, i
   return hipGetLastError();
 }
 
-template <int S>
+template <int S, int CB>
 static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
                                int64_t part_stride, double lam_scale,
                                const unsigned long long *colmax, const double *dtil,
                                const double *etil, hipStream_t s) {
   const int ncb = NCp / kMixCols;
-  const int bpm = t.W * ncb;
+  const int ncg = (ncb + CB - 1) / CB;
+  const int bpm = t.W * ncg;
   const int mgroups = (t.M + 1 + 7) / 8;
   const unsigned grid = (unsigned)(8 * mgroups * bpm);
-  hipLaunchKernelGGL(k_mix<S>, dim3(grid), dim3(256), (size_t)MixLds<S>::bytes, s, t, bmat, NCp,
+  constexpr size_t lds = MixLds<S, CB>::bytes;
+  hipLaunchKernelGGL((k_mix<S, CB>), dim3(grid), dim3(64 * (2 + 2 * CB)), lds, s, t, bmat, NCp,
                      ncb, bpm, part_stride, lam_scale, colmax, dtil, etil);
   return hipGetLastError();
 }
 
+template <int S>
+static hipError_t launch_mix_s(const LegTables &t, const void *bmat, int NCp,
+                               int64_t part_stride, double lam_scale,
+                               const unsigned long long *colmax, const double *dtil,
+                               const double *etil, hipStream_t s) {
+  switch (NCp / kMixCols) {
+    case 1: return launch_mix_t<S, 1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 2: return launch_mix_t<S, 2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 3: return launch_mix_t<S, 3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    default:
+      return launch_mix_t<S, kMixCB>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+  }
+}
+
 hipError_t mix_set_lds_limits() {
-  hipError_t e = hipFuncSetAttribute((const void *)k_mix<0>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, MixLds<0>::bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)k_mix<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            MixLds<1>::bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)k_mix<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            MixLds<2>::bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)k_mix<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            MixLds<3>::bytes);
+  hipError_t e = hipSuccess;
+#define WM_SET(S, CB)                                                                      \
+  if (e == hipSuccess)                                                                     \
+    e = hipFuncSetAttribute((const void *)k_mix<S, CB>,                                    \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                    \
+                            (int)(MixLds<S, CB>::bytes));
+#define WM_SET_S(S) WM_SET(S, 1) WM_SET(S, 2) WM_SET(S, 3) WM_SET(S, 4)
+  WM_SET_S(0) WM_SET_S(1) WM_SET_S(2) WM_SET_S(3)
+#undef WM_SET_S
+#undef WM_SET
   return e;
 }
 
@@ -559,11 +686,11 @@ hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
                       int64_t part_stride, double lam_scale, const unsigned long long *colmax,
                       const double *dtil, const double *etil, hipStream_t s) {
   switch (scheme) {
-    case 0: return launch_mix_t<0>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
-    case 1: return launch_mix_t<1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
-    case 2: return launch_mix_t<2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 0: return launch_mix_s<0>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 1: return launch_mix_s<1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 2: return launch_mix_s<2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
     default:
-      return launch_mix_t<3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+      return launch_mix_s<3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
   }
 }
 
@@ -575,6 +702,6 @@ double mix_lambda_scale(int lmax) {
 }
 
 int mix_cols_per_block() { return kMixCols; }
-int mix_rings_per_lane_table() { return kMixRB / 64; }
+int mix_rings_per_lane_table() { return 1; }  // 64 rings per workgroup
 
 }  // namespace wm
