@@ -219,7 +219,9 @@ template <int S, int CB>
 struct MixLds {
   static constexpr int P = MixScheme<S>::parts;
   // A tile per buffer, parity and part: 64 rings x (32 values + pad)
-  static constexpr int a_row = (S == 0) ? (kKK + 4) : (kKK + 8);  // elements
+  // FP32 rows of 34 floats: the MFMA A reads (row 16 rt + i16, column
+  // 4 s + g) then fall on distinct banks for 32 and for 64 banks
+  static constexpr int a_row = (S == 0) ? (kKK + 2) : (kKK + 8);  // elements
   static constexpr int a_elem = (S == 0) ? 4 : 2;                   // bytes
   static constexpr int a_part = 64 * a_row * a_elem;                // bytes
   static constexpr int a_par = P * a_part;
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 
     // Coefficients of the 32 l of this wave's parity in a super-chunk: lane
     // k < 32 holds d_{l+2}, lane 32 + k e_{l+2} (l = lsc + pw + 2k), and for
-    // FP16 lane k also A_l; loaded one super-chunk ahead, then put in the
+    // FP16 lane k also lam_scale A_l; loaded one super-chunk ahead, then put in the
     // wave's LDS slot so that the step loop reads them without waiting on
     // memory.  Indices past the tables are clamped (those chunks are unused).
     double creg0 = 0.0, creg1 = 0.0, creg2 = 0.0;
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
       const int64_t j = offm - m + lsc + pw + 2 * k;  // table index of l
       const int64_t jd = j + 2 < t.nlpad + 15 ? j + 2 : t.nlpad + 15;
       creg0 = lane < 32 ? dtil[jd] : etil[jd];
-      if constexpr (S == 3) creg1 = t.ascale[j < t.nlpad ? j : t.nlpad - 1];
+      if constexpr (S == 3) creg1 = lam_scale * t.ascale[j < t.nlpad ? j : t.nlpad - 1];
       // chunk rescale factors of the 4 chunks (lanes 0 .. 3)
       const int64_t jc = (offm - m + lsc) / kChunk + (lane & 3);
       creg2 = t.cscale[jc < t.nlpad / kChunk ? jc : t.nlpad / kChunk - 1];
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 #pragma unroll
           for (int i = 0; i < 8; ++i) {  // l = lc + pw + 2 i
             if constexpr (S == 3)
-              f[i] = (float)(lam_scale * (u * ca[x][i]));
+              f[i] = (float)(u * ca[x][i]);  // lam_scale A_l (exact power-of-two scale)
             else
               f[i] = (float)u;
             const double w = fma(x2 - cd[x][i], v, -(ce[x][i] * u));
@@ -419,9 +421,10 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
         }
         // the 8 values of this chunk: A[q][ring][8 cc + i]
         if constexpr (S == 0) {
-          float *dst = (float *)abase + lane * Ly::a_row + 8 * cc;
-          *(f32x4 *)dst = (f32x4){f[0], f[1], f[2], f[3]};
-          *(f32x4 *)(dst + 4) = (f32x4){f[4], f[5], f[6], f[7]};
+          typedef float f32x2 __attribute__((ext_vector_type(2)));
+          float *dst = (float *)abase + lane * Ly::a_row + 8 * cc;  // 8-byte aligned
+#pragma unroll
+          for (int i = 0; i < 8; i += 2) *(f32x2 *)(dst + i) = (f32x2){f[i], f[i + 1]};
         } else {
           // parts packed two 16-bit values per dword: wh (high), wm (middle),
           // wl (low)
