@@ -432,13 +432,14 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 #pragma unroll
           for (int i = 0; i < 8; i += 2) {
             if constexpr (S == 3) {
-              const _Float16 h0 = (_Float16)f[i], h1 = (_Float16)f[i + 1];
-              const _Float16 m0 = (_Float16)(f[i] - (float)h0);
-              const _Float16 m1 = (_Float16)(f[i + 1] - (float)h1);
-              wh[i >> 1] = __builtin_bit_cast(unsigned short, h0) |
-                           ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
-              wm[i >> 1] = __builtin_bit_cast(unsigned short, m0) |
-                           ((unsigned)__builtin_bit_cast(unsigned short, m1) << 16);
+              // packed conversions (round toward zero for the high part; the
+              // low part takes the exact residual, < 1 ulp of the high part)
+              typedef __fp16 h16x2 __attribute__((ext_vector_type(2)));
+              const h16x2 h = __builtin_amdgcn_cvt_pkrtz(f[i], f[i + 1]);
+              const h16x2 r =
+                  __builtin_amdgcn_cvt_pkrtz(f[i] - (float)h[0], f[i + 1] - (float)h[1]);
+              wh[i >> 1] = __builtin_bit_cast(unsigned, h);
+              wm[i >> 1] = __builtin_bit_cast(unsigned, r);
             } else {
               const __bf16 h0 = (__bf16)f[i], h1 = (__bf16)f[i + 1];
               const float r0 = f[i] - (float)h0, r1 = f[i + 1] - (float)h1;
