@@ -130,6 +130,20 @@ def test_repeat_executes_bitwise_identical():
         assert hashlib.md5(plan.execute().tobytes()).hexdigest() == h0
 
 
+@pytest.mark.parametrize("N,L,K", [(128, 512, 8), (96, 384, 8)])
+def test_wrapping_rings_many_maps(oracle, N, L, K):
+    """Rings with n <= mmax on an FFT length of 256 or more, at 8 maps or
+    more: the register kernel that folds the wrapped coefficients (belt
+    rings of Nside 128 with mmax 512; Bluestein belts of Nside 96)."""
+    print("case N=%d L=%d K=%d" % (N, L, K), flush=True)
+    plan, alm = _plan(N, L, L, K, seed=77 + N)
+    gpu = plan.execute().copy()
+    ref = oracle.OraclePlan(N, L, L, K).synthesize(alm)
+    err = rel_rms(gpu, ref)
+    print("rel rms", err)
+    assert err <= 1e-12
+
+
 @pytest.mark.parametrize("N,L,M,K", [(16, 32, 32, 10), (8, 24, 12, 5), (32, 60, 60, 8)])
 def test_many_maps_and_mmax(oracle, N, L, M, K):
     print("case N=%d L=%d M=%d K=%d" % (N, L, M, K), flush=True)
