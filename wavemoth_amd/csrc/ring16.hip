@@ -253,53 +253,16 @@ __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables 
 // Cap jobs are 2T threads per (ring pair, map), the first T threads
 // transforming the even Z entries (A), the other T the odd ones (B), each
 // in its own LDS buffer, so that no thread holds one transform's result
-// through the other's.
+// through the other's.  Both rings of the pair come from one read of E and
+// O; the Z entries of the southern ring wait in registers (8 per thread)
+// while the northern ring is transformed.
 
-// Fold of a ring whose m-range wraps (n <= mmax), into F[0 .. h] at
-// buf[pad(.)] (buf spans the job's two buffers, >= n padded slots):
-//   P_rho = sum_w (-1)^{s w} g_{w n + rho}  (rho < n, blocks of n rows),
-//   F_b = P_b + (-1)^s conj P_{n-b}  (0 < b <= h),
-//   F_0 = P_0 + conj(P_0 - g_0),
-// the direct terms m = b + w n and conjugate terms m = w n - b of every bin
-// regrouped by row block.  Thread tt owns bins rho = tt + 2T q (q < 16;
-// n < 2P = 16 (2T)) and sums their row blocks in order in registers.
+// F_b = P_b + (-1)^s conj P_{n-b} (0 < b <= h), F_0 = P_0 + conj(P_0 - g0),
+// in place over the P_rho (rho < n) in buf
 template <int P>
-__device__ __forceinline__ void wrap_fold(const double2 *__restrict__ qe,
-                                          const double2 *__restrict__ qo, int64_t qs, int M,
-                                          int n, int s, double2 *buf, int tt, bool on, double hs) {
+__device__ __forceinline__ void wrap_fstep(double2 *buf, int n, int s, int tt, double2 g0) {
   constexpr int T2 = P / 8;
   const int h = n / 2;
-  double2 acc[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) acc[q] = make_double2(0.0, 0.0);
-  double2 g0 = make_double2(0.0, 0.0);
-  for (int w0 = 0; w0 <= M; w0 += n) {
-    const double se = (s && ((w0 / n) & 1)) ? -1.0 : 1.0;
-    const double so = se * hs;
-    double2 e[16], o[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int m = w0 + tt + T2 * q;
-      const int64_t mc = m <= M ? m : M;
-      e[q] = qe[mc * qs];
-      o[q] = qo[mc * qs];
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int rho = tt + T2 * q, m = w0 + rho;
-      if (on && rho < n && m <= M) {
-        acc[q].x += fma(so, o[q].x, se * e[q].x);
-        acc[q].y += fma(so, o[q].y, se * e[q].y);
-      }
-    }
-    if (w0 == 0 && tt == 0) g0 = make_double2(fma(hs, o[0].x, e[0].x), fma(hs, o[0].y, e[0].y));
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int rho = tt + T2 * q;
-    if (rho < n) buf[pad(rho)] = acc[q];
-  }
-  __syncthreads();
   const double sg = s ? -1.0 : 1.0;
   for (int b = tt; b <= h; b += T2) {
     const double2 pb = buf[pad(b)];
@@ -307,97 +270,185 @@ __device__ __forceinline__ void wrap_fold(const double2 *__restrict__ qe,
     const double c = b == 0 ? 1.0 : sg;
     buf[pad(b)] = make_double2(fma(c, pc.x, pb.x), fma(-c, pc.y, pb.y));
   }
-  __syncthreads();
 }
 
-// One cap ring (hemisphere sign hs) of one map: fold, even / odd Z entries,
-// the two Bluestein convolutions, the pixels.  tt = thread in the job
-// (0 .. 2T), bufA / bufB the job's two buffers.
-template <int P, bool WRAP>
-__device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__restrict__ qe,
-                                         const double2 *__restrict__ qo, int64_t qs, int M,
-                                         int K, const double2 *__restrict__ chirp,
-                                         const double2 *__restrict__ vhat, double2 *bufA,
-                                         double2 *bufB, const double2 *twl, int tt, bool on,
-                                         double hs, double *__restrict__ out) {
-  constexpr int T = P / 16, T2 = 2 * T;
-  asm volatile("" : "+v"(tt));  // see belt_ring
-  const int n = rp.n, h = n / 2, i = h / 2;
-  const int half = tt / T, j = tt - half * T;
-  const double sg = rp.s ? -1.0 : 1.0;
-  if constexpr (WRAP) {
-    wrap_fold<P>(qe, qo, qs, M, n, rp.s, bufA, tt, on, hs);
-  } else {
-  // 1. fold into bufA: F_b = g_b (b <= min(h, mmax)), then F_{n-m} +=
-  //    (-1)^s conj g_m for h <= m <= mmax (one writer per bin in each
-  //    pass); h < P and n > mmax gives mmax - h < h, so 8 rows per thread
-  //    cover each pass
-  {
-    double2 e[8], o[8];
+// Fold of a ring pair whose m-range wraps (n <= mmax), into F[0 .. h] at
+// buf[pad(.)] (buf spans the job's two buffers, >= n padded slots):
+//   P_rho = sum_w (-1)^{s w} g_{w n + rho}  (rho < n, blocks of n rows),
+//   F_b = P_b + (-1)^s conj P_{n-b}  (0 < b <= h),
+//   F_0 = P_0 + conj(P_0 - g_0),
+// the direct terms m = b + w n and conjugate terms m = w n - b of every bin
+// regrouped by row block.  Thread tt owns bins rho = tt + 2T q (q < 16;
+// n < 2P = 16 (2T)) and sums their row blocks in order in registers, for
+// g = E + O (north, folded into buf here) and g = E - O (south, returned
+// in ps[] and *g0s for wrap_fold_south).  Rows of bins rho >= n are not
+// read: those lanes load the row of bin tt again.
+template <int P>
+__device__ __forceinline__ void wrap_fold2(const double2 *__restrict__ qe,
+                                           const double2 *__restrict__ qo, int64_t qs, int M,
+                                           int n, int s, double2 *buf, int tt, bool on_n,
+                                           bool on_s, double2 ps[16], double2 *g0s) {
+  constexpr int T2 = P / 8;
+  double2 an[16];
 #pragma unroll
-    for (int rho = 0; rho < 8; ++rho) {
-      const int b = tt + T2 * rho;
-      const int64_t bc = b <= M ? b : M;
-      e[rho] = qe[bc * qs];
-      o[rho] = qo[bc * qs];
-    }
+  for (int q = 0; q < 16; ++q) an[q] = ps[q] = make_double2(0.0, 0.0);
+  double2 g0n = make_double2(0.0, 0.0);
+  *g0s = g0n;
+  for (int w0 = 0; w0 <= M; w0 += n) {
+    const double se = (s && ((w0 / n) & 1)) ? -1.0 : 1.0;
+    const int64_t m0 = w0 + tt <= M ? w0 + tt : M;
 #pragma unroll
-    for (int rho = 0; rho < 8; ++rho) {
-      const int b = tt + T2 * rho;
-      double2 g = make_double2(fma(hs, o[rho].x, e[rho].x), fma(hs, o[rho].y, e[rho].y));
-      if (!(on && b <= M)) g = make_double2(0.0, 0.0);
-      if (b <= h) bufA[pad(b)] = g;
-    }
-  }
-  __syncthreads();
-  {
-    double2 e[8], o[8];
+    for (int h8 = 0; h8 < 16; h8 += 8) {
+      double2 e[8], o[8];
 #pragma unroll
-    for (int rho = 0; rho < 8; ++rho) {
-      const int m = h + tt + T2 * rho;
-      const int64_t mc = m <= M ? m : M;
-      e[rho] = qe[mc * qs];
-      o[rho] = qo[mc * qs];
-    }
+      for (int q = 0; q < 8; ++q) {
+        const int rho = tt + T2 * (h8 + q), m = w0 + rho;
+        const int64_t mc = (rho < n && m <= M) ? m : m0;
+        e[q] = qe[mc * qs];
+        o[q] = qo[mc * qs];
+      }
 #pragma unroll
-    for (int rho = 0; rho < 8; ++rho) {
-      const int m = h + tt + T2 * rho;
-      if (on && m <= M) {
-        const double2 g = make_double2(fma(hs, o[rho].x, e[rho].x), fma(hs, o[rho].y, e[rho].y));
-        double2 &f = bufA[pad(n - m)];
-        f.x = fma(sg, g.x, f.x);
-        f.y = fma(-sg, g.y, f.y);
+      for (int q = 0; q < 8; ++q) {
+        const int rho = tt + T2 * (h8 + q), m = w0 + rho;
+        if (rho < n && m <= M) {
+          const double ex = se * e[q].x, ey = se * e[q].y;
+          an[h8 + q].x += fma(se, o[q].x, ex);
+          an[h8 + q].y += fma(se, o[q].y, ey);
+          ps[h8 + q].x += fma(-se, o[q].x, ex);
+          ps[h8 + q].y += fma(-se, o[q].y, ey);
+        }
+      }
+      if (h8 == 0 && w0 == 0 && tt == 0) {
+        g0n = cadd(e[0], o[0]);
+        *g0s = csub(e[0], o[0]);
       }
     }
   }
-  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int rho = tt + T2 * q;
+    if (rho < n) buf[pad(rho)] = on_n ? an[q] : make_double2(0.0, 0.0);
+    if (!on_s) ps[q] = make_double2(0.0, 0.0);
   }
-  asm volatile("" : "+v"(tt));  // no table load hoisted above the barriers
-  // 2. Z entry b = 2 kappa + half (kappa = j + T rho < i < P/2, so rho < 8)
-  //    times the chirp; phase w_b = e^{i pi s b/n}, partner phase
-  //    e^{i pi s/2} conj(w_b), twiddle e^{2 pi i b/n}; DC / Nyquist real
-  const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
-  double2 v[16];
-  {
-    const double2 wstep = expipi((int64_t)rp.s * 2 * T, n), ostep = expipi(4 * T, n);
-    double2 w = expipi((int64_t)rp.s * (2 * j + half), n), om = expipi(2 * (2 * j + half), n);
+  __syncthreads();
+  wrap_fstep<P>(buf, n, s, tt, on_n ? g0n : make_double2(0.0, 0.0));
+  __syncthreads();
+}
+
+// The southern fold of wrap_fold2 into buf.  The caller has passed a
+// barrier after its last read of buf.
+template <int P>
+__device__ __forceinline__ void wrap_fold_south(double2 *buf, int n, int s, int tt,
+                                                const double2 ps[16], double2 g0s) {
+  constexpr int T2 = P / 8;
 #pragma unroll
-    for (int rho = 0; rho < 16; ++rho) v[rho] = make_double2(0.0, 0.0);
+  for (int q = 0; q < 16; ++q) {
+    const int rho = tt + T2 * q;
+    if (rho < n) buf[pad(rho)] = ps[q];
+  }
+  __syncthreads();
+  wrap_fstep<P>(buf, n, s, tt, g0s);
+  __syncthreads();
+}
+
+// Fold of a cap ring pair without wrap (n > mmax): F_b = g_b
+// (b <= min(h, mmax)), then F_{n-m} += (-1)^s conj g_m for h <= m <= mmax
+// (one writer per bin in each pass); h < P and n > mmax give mmax - h < h,
+// so 8 rows per thread cover each pass.  g = E + O (north) into bufN,
+// g = E - O (south) into bufS.
+template <int P>
+__device__ __forceinline__ void cap_fold2(const double2 *__restrict__ qe,
+                                          const double2 *__restrict__ qo, int64_t qs, int M,
+                                          int n, int s, double2 *bufN, double2 *bufS, int tt,
+                                          bool on_n, bool on_s) {
+  constexpr int T2 = P / 8;
+  const int h = n / 2;
+  const double sg = s ? -1.0 : 1.0;
+  double2 e[16], o[16];
 #pragma unroll
-    for (int rho = 0; rho < 8; ++rho) {
-      const int kap = j + T * rho;
-      const bool in = kap < i;
-      const double2 c = chirp[in ? kap : 0];
-      const int b = in ? 2 * kap + half : 0;
-      double2 xa = cmul(bufA[pad(b)], w);
-      double2 xc = cmul(bufA[pad(h - b)], b == 0 ? wh : cmul(wh, cconj(w)));
-      if (b == 0) xa.y = xc.y = 0.0;
-      if (in) v[rho] = cmul(zpair(xa, xc, om), c);
-      w = cmul(w, wstep);
-      om = cmul(om, ostep);
+  for (int rho = 0; rho < 8; ++rho) {
+    const int b = tt + T2 * rho, m = h + b;
+    const int64_t bc = b <= M ? b : M, mc = m <= M ? m : M;
+    e[rho] = qe[bc * qs];
+    o[rho] = qo[bc * qs];
+    e[8 + rho] = qe[mc * qs];
+    o[8 + rho] = qo[mc * qs];
+  }
+  double2 cn[8], cs[8];
+#pragma unroll
+  for (int rho = 0; rho < 8; ++rho) {
+    const int b = tt + T2 * rho, m = h + b;
+    const bool db = b <= M, dm = m <= M;
+    const double2 zero = make_double2(0.0, 0.0);
+    if (b <= h) {
+      bufN[pad(b)] = (on_n && db) ? cadd(e[rho], o[rho]) : zero;
+      bufS[pad(b)] = (on_s && db) ? csub(e[rho], o[rho]) : zero;
+    }
+    const double2 c = cadd(e[8 + rho], o[8 + rho]), d = csub(e[8 + rho], o[8 + rho]);
+    cn[rho] = (on_n && dm) ? make_double2(sg * c.x, -sg * c.y) : zero;
+    cs[rho] = (on_s && dm) ? make_double2(sg * d.x, -sg * d.y) : zero;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rho = 0; rho < 8; ++rho) {
+    const int m = h + tt + T2 * rho;
+    if (m <= M) {
+      double2 &fn = bufN[pad(n - m)];
+      fn = cadd(fn, cn[rho]);
+      double2 &fs = bufS[pad(n - m)];
+      fs = cadd(fs, cs[rho]);
     }
   }
-  // 3. Bluestein: y = IFFT(vhat . FFT(c . Z)) in the half's own buffer
+  __syncthreads();
+}
+
+// Z entries of one cap ring from its fold F[0 .. h] in bufF: entry
+// b = 2 kappa + half (kappa = j + T rho < i < P/2, so rho < 8) times the
+// chirp c_kappa; phase w_b = e^{i pi s b/n}, partner phase
+// e^{i pi s/2} conj(w_b), twiddle e^{2 pi i b/n}; DC / Nyquist real.
+template <int P>
+__device__ __forceinline__ void cap_z(const RingPair &rp, const double2 *__restrict__ chirp,
+                                      const double2 *bufF, int tt, double2 z[8]) {
+  constexpr int T = P / 16;
+  const int n = rp.n, h = n / 2, i = h / 2;
+  const int half = tt / T, j = tt - half * T;
+  const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
+  const double2 wstep = expipi((int64_t)rp.s * 2 * T, n), ostep = expipi(4 * T, n);
+  double2 w = expipi((int64_t)rp.s * (2 * j + half), n), om = expipi(2 * (2 * j + half), n);
+#pragma unroll
+  for (int rho = 0; rho < 8; ++rho) {
+    const int kap = j + T * rho;
+    const bool in = kap < i;
+    const double2 c = chirp[in ? kap : 0];
+    const int b = in ? 2 * kap + half : 0;
+    double2 xa = cmul(bufF[pad(b)], w);
+    double2 xc = cmul(bufF[pad(h - b)], b == 0 ? wh : cmul(wh, cconj(w)));
+    if (b == 0) xa.y = xc.y = 0.0;
+    z[rho] = in ? cmul(zpair(xa, xc, om), c) : make_double2(0.0, 0.0);
+    w = cmul(w, wstep);
+    om = cmul(om, ostep);
+  }
+}
+
+// One cap ring of one map from its Z entries: the two Bluestein
+// convolutions, the pixels.  tt = thread in the job (0 .. 2T), bufA / bufB
+// the job's two buffers (free: the transform starts with a barrier).
+template <int P>
+__device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
+                                          const double2 *__restrict__ chirp,
+                                          const double2 *__restrict__ vhat, const double2 z[8],
+                                          double2 *bufA, double2 *bufB, const double2 *twl,
+                                          int tt, bool on, double *__restrict__ out) {
+  constexpr int T = P / 16, T2 = 2 * T;
+  const int n = rp.n, h = n / 2, i = h / 2;
+  const int half = tt / T, j = tt - half * T;
+  double2 v[16];
+#pragma unroll
+  for (int rho = 0; rho < 8; ++rho) {
+    v[rho] = z[rho];
+    v[8 + rho] = make_double2(0.0, 0.0);
+  }
+  // Bluestein: y = IFFT(vhat . FFT(c . Z)) in the half's own buffer
   double2 *buf = half ? bufB : bufA;
   fft16::fft<P, false>(v, buf, twl, j);
   asm volatile("" : "+v"(tt));
@@ -419,7 +470,7 @@ __device__ __forceinline__ void cap_ring(const RingPair &rp, const double2 *__re
     }
   }
   __syncthreads();
-  // 4. z_p = A_p + B_p, z_{p+i} = A_p - B_p
+  // z_p = A_p + B_p, z_{p+i} = A_p - B_p
   if (on) {
 #pragma unroll
     for (int rho = 0; rho < 4; ++rho) {
@@ -440,7 +491,8 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
                                           int count, double *__restrict__ map, int b,
                                           double2 *lds, const double2 *twl) {
   constexpr int T = P / 16, J = RB / (2 * T), S = job_slots(P);
-  const int jb = threadIdx.x / (2 * T), tt = threadIdx.x % (2 * T);
+  const int jb = threadIdx.x / (2 * T);
+  int tt = threadIdx.x % (2 * T);
   double2 *bufA = lds + 2 * jb * S, *bufB = bufA + S;
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
@@ -457,13 +509,25 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
   // every pair of a cap launch (also the padding jobs' stand-in) has tables
   const double2 *chirp = rt.chirp2 + rp.chirp2_off;  // [i]
   const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
-#pragma unroll 1
-  for (int hemi = 0; hemi < 2; ++hemi) {
-    cap_ring<P, WRAP>(rp, qe, qo, qs, M, K, chirp, vhat, bufA, bufB, twl, tt,
-                act && (hemi == 0 || south), hemi ? -1.0 : 1.0,
-                map + k + (hemi ? (south ? rp.pix_s : 0) : rp.pix_n) * K);
-    __syncthreads();  // buffers reused by the southern ring
+  double2 zn[8], zs[8];
+  if constexpr (WRAP) {
+    double2 ps[16], g0s;
+    wrap_fold2<P>(qe, qo, qs, M, rp.n, rp.s, bufA, tt, act, act && south, ps, &g0s);
+    cap_z<P>(rp, chirp, bufA, tt, zn);
+    __syncthreads();  // reads of the northern fold done
+    wrap_fold_south<P>(bufA, rp.n, rp.s, tt, ps, g0s);
+    cap_z<P>(rp, chirp, bufA, tt, zs);
+  } else {
+    cap_fold2<P>(qe, qo, qs, M, rp.n, rp.s, bufA, bufB, tt, act, act && south);
+    cap_z<P>(rp, chirp, bufA, tt, zn);
+    cap_z<P>(rp, chirp, bufB, tt, zs);
   }
+  asm volatile("" : "+v"(tt));  // no table load hoisted above the barriers
+  cap_xform<P>(rp, K, chirp, vhat, zn, bufA, bufB, twl, tt, act, map + k + rp.pix_n * K);
+  __syncthreads();  // the northern ring's reads of bufA / bufB done
+  asm volatile("" : "+v"(tt));
+  cap_xform<P>(rp, K, chirp, vhat, zs, bufA, bufB, twl, tt, act && south,
+               map + k + (south ? rp.pix_s : 0) * K);
 }
 
 template <int KIND, int P>
