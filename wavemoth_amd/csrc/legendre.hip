@@ -142,11 +142,29 @@ __device__ __forceinline__ void prep_item(const LegTables &t, const double2 *__r
   }
 }
 
+// The single-plan prescale: one thread per (padded l-entry j, map slot kk),
+// maps fastest, so that a wave reads the consecutive a_lm of one l (and of
+// JB = 256 / Kp consecutive l) and writes KB-map runs of atil.
 __global__ __launch_bounds__(256) void k_prep_alm(LegTables t, const double2 *__restrict__ alm,
                                                   int KB, int nkb) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= t.nlpad) return;
-  prep_item(t, alm, j, KB, nkb);
+  const int Kp = KB * nkb, K = t.K, L = t.L;
+  const int JB = Kp >= 256 ? 1 : 256 / Kp;
+  const int64_t j0 = (int64_t)blockIdx.x * JB;
+  for (int u = threadIdx.x; u < JB * Kp; u += blockDim.x) {
+    const int jl = u / Kp, kk = u - jl * Kp;
+    const int64_t j = j0 + jl;
+    if (j >= t.nlpad) break;  // jl only grows with u
+    const int m = t.chunk_m[j / kChunk];
+    const int64_t jj = j - t.off[m];
+    const int l = m + (int)jj;
+    const bool live = l <= L && kk < K;
+    const double sc = t.ascale[j];
+    const int64_t idx = (int64_t)m * (2 * L - m + 3) / 2 + (l <= L ? jj : 0);
+    const double2 a = alm[idx * K + (kk < K ? kk : K - 1)];
+    const int kb = kk / KB;
+    t.atil[((int64_t)kb * t.nlpad + j) * KB + (kk - kb * KB)] =
+        live ? make_double2(a.x * sc, a.y * sc) : make_double2(0.0, 0.0);
+  }
 }
 
 // One wave's share of the sweep: ring chunk w (rings w*64*RL + lane + 64*i,
@@ -312,8 +330,10 @@ int legendre_padded_maps(int K) {
 
 hipError_t launch_prep_alm(const LegTables &t, const double2 *alm, hipStream_t s) {
   const int KB = t.KB;
-  hipLaunchKernelGGL(k_prep_alm, dim3(nblocks(t.nlpad, 256)), dim3(256), 0, s, t, alm, KB,
-                     (t.K + KB - 1) / KB);
+  const int nkb = (t.K + KB - 1) / KB, Kp = KB * nkb;
+  const int JB = Kp >= 256 ? 1 : 256 / Kp;
+  hipLaunchKernelGGL(k_prep_alm, dim3((unsigned)((t.nlpad + JB - 1) / JB)), dim3(256), 0, s, t,
+                     alm, KB, nkb);
   return hipGetLastError();
 }
 

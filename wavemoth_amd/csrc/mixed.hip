@@ -105,6 +105,8 @@ __global__ __launch_bounds__(256) void k_mix_colmax_clear(unsigned long long *co
   if (c < NCp) colmax[c] = 0ull;
 }
 
+// Block b covers rows [r0, r1) of the (nrows, C2) array; thread tid owns
+// column cb + tid % cols of every RP-th row and keeps its max in a register.
 __global__ __launch_bounds__(256) void k_mix_colmax(const double *__restrict__ alm,
                                                     int64_t nrows, int C2,
                                                     unsigned long long *colmax) {
@@ -114,20 +116,22 @@ __global__ __launch_bounds__(256) void k_mix_colmax(const double *__restrict__ a
   const int64_t per = (nrows + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = blockIdx.x * per;
   const int64_t r1 = r0 + per < nrows ? r0 + per : nrows;
-  for (int64_t e = r0 * C2 + threadIdx.x; e < r1 * C2; e += blockDim.x) {
-    const int c = (int)(e % C2);
-    atomicMax(&smax[c], __builtin_bit_cast(unsigned long long, fabs(alm[e])));
+  const int cols = C2 < 256 ? C2 : 256, RP = 256 / cols;
+  const int rr = threadIdx.x / cols, cc = threadIdx.x - rr * cols;
+  for (int cb = 0; cb < C2; cb += cols) {
+    const int c = cb + cc;
+    if (rr < RP && c < C2) {
+      unsigned long long mx = 0ull;
+      for (int64_t r = r0 + rr; r < r1; r += RP) {
+        const unsigned long long v = __builtin_bit_cast(unsigned long long, fabs(alm[r * C2 + c]));
+        mx = v > mx ? v : mx;
+      }
+      if (mx) atomicMax(&smax[c], mx);
+    }
   }
   __syncthreads();
   for (int c = threadIdx.x; c < C2; c += blockDim.x)
     if (smax[c]) atomicMax(&colmax[c], smax[c]);
-}
-
-__device__ __forceinline__ unsigned short f16_bits(double v) {
-  return __builtin_bit_cast(unsigned short, (_Float16)(float)v);
-}
-__device__ __forceinline__ double f16_value(unsigned short b) {
-  return (double)(float)__builtin_bit_cast(_Float16, b);
 }
 
 // ------------------------------------------------------------- prescale
@@ -139,69 +143,112 @@ __device__ __forceinline__ double f16_value(unsigned short b) {
 // Values are A_l a_lm (the FP64 path's prescale), zero past lmax and for
 // padded columns.  Three trailing zero chunks let a super-chunk of the last
 // m read past its end.
-__device__ __forceinline__ unsigned short bf16_bits(double v) {
-  const __bf16 h = (__bf16)(float)v;
-  return __builtin_bit_cast(unsigned short, h);
-}
-__device__ __forceinline__ double bf16_value(unsigned short b) {
-  return (double)__builtin_bit_cast(float, (unsigned int)b << 16);
+
+// Value of B at (l = t-index j of chunk c, map k, Re/Im): A_l a_lm (FP16:
+// the column-scaled a_lm, Lambda carries A_l), zero past lmax and for
+// padded maps.
+// csr / csi: the FP16 column scales of (k, Re) and (k, Im), used when
+// scheme == 3.
+__device__ __forceinline__ void mix_prep_pair(const LegTables &t, const double2 *__restrict__ alm,
+                                              int64_t j, int m, int k, int scheme, double csr,
+                                              double csi, double *vr, double *vi) {
+  // unconditional loads (clamped l and k) so that the loads of all calls
+  // of a thread are in flight together
+  const double A = t.ascale[j];
+  const int l = m + (int)(j - t.off[m]);
+  const int lc = l <= t.L ? l : t.L, kc = k < t.K ? k : t.K - 1;
+  const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + (lc - m);
+  const double2 a = alm[idx * t.K + kc];
+  const bool on = k < t.K && A != 0.0;
+  *vr = on ? (scheme == 3 ? csr : A) * a.x : 0.0;
+  *vi = on ? (scheme == 3 ? csi : A) * a.y : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_mix_prep(LegTables t, const double2 *__restrict__ alm,
-                                                  void *bmat, int NCp, int scheme,
-                                                  int64_t part_stride,
-                                                  const unsigned long long *colmax) {
-  const int64_t total = t.nlpad * NCp;
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    // thread order follows the output layout (innermost index fastest)
-    int64_t c;
-    int p, i, col;
-    if (scheme == 0) {
-      col = (int)(g % NCp);
-      const int64_t r = g / NCp;  // (c * 2 + p) * 8 + i
-      i = (int)(r & 7);
-      p = (int)((r >> 3) & 1);
-      c = r >> 4;
-    } else {
-      i = (int)(g & 7);
-      const int64_t r = g >> 3;  // (c * 2 + p) * NCp + col
-      col = (int)(r % NCp);
-      p = (int)((r / NCp) & 1);
-      c = r / NCp >> 1;
-    }
-    const int64_t j = c * kChunk + 2 * i + p;
+// 16-bit parts of v: FP16 hi + lo (scheme 3), else BF16 hi, mid, lo.
+// Formed from f = (float)v: the parts carry 22 (FP16) or 24 (BF16) bits,
+// within f's 24, and each residual is exact in FP32, so the conversions
+// stay single hardware instructions.
+__device__ __forceinline__ void mix_split16(double v, int scheme, unsigned short *h0,
+                                            unsigned short *h1, unsigned short *h2) {
+  const float f = (float)v;
+  if (scheme == 3) {
+    const _Float16 a = (_Float16)f;
+    const float r = f - (float)a;
+    *h0 = __builtin_bit_cast(unsigned short, a);
+    *h1 = __builtin_bit_cast(unsigned short, (_Float16)r);
+    *h2 = 0;
+  } else {
+    const __bf16 a = (__bf16)f;
+    const float r1 = f - (float)a;
+    const __bf16 b = (__bf16)r1;
+    const float r2 = r1 - (float)b;
+    *h0 = __builtin_bit_cast(unsigned short, a);
+    *h1 = __builtin_bit_cast(unsigned short, b);
+    *h2 = __builtin_bit_cast(unsigned short, (__bf16)r2);
+  }
+}
+
+// FP32: one chunk per block; thread u -> (l slot li = u / NK, map k = u % NK)
+// reads one a_lm (lanes over maps: coalesced) and writes its two columns.
+__global__ __launch_bounds__(256) void k_mix_prep_f32(LegTables t, const double2 *__restrict__ alm,
+                                                      float *__restrict__ bf, int NCp) {
+  const int64_t c = blockIdx.x;
+  const int m = t.chunk_m[c];
+  const int NK = NCp / 2;
+  for (int u = threadIdx.x; u < kChunk * NK; u += blockDim.x) {
+    const int li = u / NK, k = u - li * NK;  // li = 2 i + p
+    const int i = li >> 1, p = li & 1;
+    double vr, vi;
+    mix_prep_pair(t, alm, c * kChunk + li, m, k, 0, 1.0, 1.0, &vr, &vi);
+    *reinterpret_cast<float2 *>(bf + ((c * 2 + p) * 8 + i) * NCp + 2 * k) =
+        make_float2((float)vr, (float)vi);
+  }
+}
+
+// 16-bit parts: thread u -> (chunk c = blockIdx.x * CPB + u / (2 NK), parity
+// p, map k) reads the 8 a_lm of (c, p, k) and writes 16 bytes per part and
+// column.
+template <int S>
+__global__ __launch_bounds__(256) void k_mix_prep_16(LegTables t, const double2 *__restrict__ alm,
+                                                     unsigned short *__restrict__ bq, int NCp,
+                                                     int64_t part_stride,
+                                                     const unsigned long long *colmax,
+                                                     int64_t nchunks) {
+  const int NK = NCp / 2, per = 2 * NK;
+  const int CPB = blockDim.x / per > 0 ? blockDim.x / per : 1;
+  for (int u = threadIdx.x; u < CPB * per; u += blockDim.x) {
+    const int cl = u / per, r = u - cl * per;
+    const int p = r / NK, k = r - p * NK;
+    const int64_t c = (int64_t)blockIdx.x * CPB + cl;
+    if (c >= nchunks) continue;
     const int m = t.chunk_m[c];
-    const int l = m + (int)(j - t.off[m]);
-    const double A = t.ascale[j];
-    const int k = col >> 1;
-    double v = 0.0;
-    if (k < t.K && A != 0.0) {
-      const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + (l - m);
-      const double2 a = alm[idx * t.K + k];
-      // FP16: the true a, column-scaled (Lambda carries A_l); else A_l a
-      v = (scheme == 3 ? col_scale(colmax, col) : A) * ((col & 1) ? a.y : a.x);
+    const int kc = k < t.K ? k : 0;
+    const double csr = S == 3 ? col_scale(colmax, 2 * kc) : 1.0;
+    const double csi = S == 3 ? col_scale(colmax, 2 * kc + 1) : 1.0;
+    unsigned short hr[3][8], hi[3][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      double vr, vi;
+      mix_prep_pair(t, alm, c * kChunk + 2 * i + p, m, k, S, csr, csi, &vr, &vi);
+      mix_split16(vr, S, &hr[0][i], &hr[1][i], &hr[2][i]);
+      mix_split16(vi, S, &hi[0][i], &hi[1][i], &hi[2][i]);
     }
-    if (scheme == 3) {
-      unsigned short *bq = (unsigned short *)bmat;
-      const unsigned short h = f16_bits(v);
-      bq[g] = h;
-      bq[part_stride + g] = f16_bits(v - f16_value(h));
-      continue;
-    }
-    if (scheme == 0) {
-      ((float *)bmat)[g] = (float)v;
-      continue;
-    }
-    unsigned short *bq = (unsigned short *)bmat;
-    const unsigned short h = bf16_bits(v);
-    const double r1 = v - bf16_value(h);
-    const unsigned short mid = bf16_bits(r1);
-    bq[g] = h;
-    bq[part_stride + g] = mid;
-    if (scheme == 2) {
-      const double r2 = r1 - bf16_value(mid);
-      bq[2 * part_stride + g] = bf16_bits(r2);
+    constexpr int nparts = MixScheme<S>::parts;
+    const int64_t at = ((c * 2 + p) * NCp + 2 * k) * 8;
+#pragma unroll
+    for (int q = 0; q < nparts; ++q) {
+      uint4 wr, wi;
+      wr.x = hr[q][0] | (unsigned)hr[q][1] << 16;
+      wr.y = hr[q][2] | (unsigned)hr[q][3] << 16;
+      wr.z = hr[q][4] | (unsigned)hr[q][5] << 16;
+      wr.w = hr[q][6] | (unsigned)hr[q][7] << 16;
+      wi.x = hi[q][0] | (unsigned)hi[q][1] << 16;
+      wi.y = hi[q][2] | (unsigned)hi[q][3] << 16;
+      wi.z = hi[q][4] | (unsigned)hi[q][5] << 16;
+      wi.w = hi[q][6] | (unsigned)hi[q][7] << 16;
+      uint4 *o = reinterpret_cast<uint4 *>(bq + q * part_stride + at);
+      o[0] = wr;
+      o[1] = wi;
     }
   }
 }
@@ -226,16 +273,15 @@ struct MixLds {
   static constexpr int a_part = 64 * a_row * a_elem;                // bytes
   static constexpr int a_par = P * a_part;
   static constexpr int a_buf = 2 * a_par;
-  // B rows per buffer: FP32 [4 chunks][2 parities][8 l][32 CB cols + 16]
-  // floats (the pad puts the 4 l of one MFMA step on distinct banks);
-  // 16-bit per part [4 chunks][2 parities][32 CB cols][8 l]
-  static constexpr int b_row = kMixCols * CB + 16;  // FP32 row stride (floats)
-  static constexpr int b_part = (S == 0) ? 64 * b_row * 4 : 4 * 2 * 8 * kMixCols * CB * 2;
-  static constexpr int b_buf = P * b_part;
-  static constexpr int b_u128 = P * 4 * 2 * 8 * kMixCols * CB * ((S == 0) ? 4 : 2) / 16;
   // recurrence coefficients per buffer and parity: d_{l+2}, e_{l+2} and
   // (FP16) A_l of the 32 l of the parity, as doubles
   static constexpr int c_par = 100 * 8;  // + the 4 chunk rescale factors
+  // FP32 scheme: B rows staged per buffer, [4 chunks][2 parities][8 l]
+  // [32 CB cols + 16] floats (the pad puts the 4 l of one MFMA step on
+  // distinct banks); the 16-bit schemes read B straight into registers
+  static constexpr int b_row = kMixCols * CB + 16;  // row stride (floats)
+  static constexpr int b_buf = (S == 0) ? 64 * b_row * 4 : 0;
+  static constexpr int b_u128 = 4 * 2 * 8 * kMixCols * CB * 4 / 16;
   static constexpr int a_off = 0;
   static constexpr int b_off = 2 * a_buf;
   static constexpr int c_off = b_off + 2 * b_buf;
@@ -306,35 +352,45 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
       vs = pw ? t3 : t2;
     }
 
-    // B rows of a super-chunk, moved by the 128 CB consumer lanes: loaded
-    // into registers one super-chunk ahead of their store into an LDS
-    // buffer (macros, so that the register array is never addressed
-    // indirectly)
+    // 16-bit schemes: B fragments of a consumer (its 32 columns, its
+    // parity) straight from global memory into registers, one super-chunk
+    // ahead of their use (each B element is used by one wave of the
+    // workgroup, so it gains nothing from LDS): lane (i16, g) holds the 8 l
+    // of chunk g for column 16 ct + i16, 16 bytes per part and ct.  Columns
+    // past the last block read column 0.
+    const int64_t chunk0 = (offm + l0 - m) / kChunk;  // first chunk of super-chunk 0
+    int colc[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int col = cb * kMixCols + 16 * ct + i16;
+      colc[ct] = col < NCp ? col : 0;
+    }
+    typedef u32x4 BFrag[P][2];
+    auto load_b = [&](BFrag &f, int sc) {
+      const unsigned short *b0 =
+          (const unsigned short *)bmat + ((chunk0 + 4 * sc + g) * 2 + pw) * (int64_t)NCp * 8;
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+          f[q][ct] = *(const u32x4 *)(b0 + q * part_stride + colc[ct] * 8);
+    };
+    // FP32 scheme: B rows of a super-chunk moved into LDS by the 128 CB
+    // consumer lanes, loaded into registers one super-chunk ahead of their
+    // store into an LDS buffer (macros, so that the register array is never
+    // addressed indirectly)
     constexpr int NC = 128 * CB;          // consumer lanes
     constexpr int BU = Ly::b_u128 / NC;   // 16-byte vectors per consumer lane
-    // per-lane byte offsets of this lane's B vectors: source relative to the
-    // super-chunk's first row, destination within an LDS buffer (columns
-    // past the last block read column 0)
     auto b_offsets = [&](int x, int64_t *src, int *dst) {
       const int e = threadIdx.x - 128 + NC * x;
-      if constexpr (S == 0) {
-        constexpr int U = 8 * CB;  // rows (chunk, parity, i) of 32 CB floats
-        const int row = e / U, cu = e - row * U;
-        const int col = colg0 + 4 * cu < NCp ? colg0 + 4 * cu : 0;
-        *src = ((int64_t)row * NCp + col) * 4;
-        *dst = (row * Ly::b_row + 4 * cu) * 4;
-      } else {
-        constexpr int U = kMixCols * CB;  // per part, rows (chunk, parity)
-        const int q = e / (8 * U), rem = e - q * (8 * U);
-        const int row = rem / U, cu = rem - row * U;
-        const int col = colg0 + cu < NCp ? colg0 + cu : 0;
-        *src = ((int64_t)q * part_stride + ((int64_t)row * NCp + col) * 8) * 2;
-        *dst = q * Ly::b_part + rem * 16;
-      }
+      constexpr int U = 8 * CB;  // rows (chunk, parity, i) of 32 CB floats
+      const int row = e / U, cu = e - row * U;
+      const int col = colg0 + 4 * cu < NCp ? colg0 + 4 * cu : 0;
+      *src = ((int64_t)row * NCp + col) * 4;
+      *dst = (row * Ly::b_row + 4 * cu) * 4;
     };
-    // bytes between the B rows of consecutive chunks
-    const int64_t bchunk = (S == 0) ? (int64_t)16 * NCp * 4 : (int64_t)2 * NCp * 8 * 2;
-    const char *bsc0 = (const char *)bmat + (offm / kChunk + (l0 - m) / kChunk) * bchunk;
+    const int64_t bchunk = (int64_t)16 * NCp * 4;  // bytes between chunks
+    const char *bsc0 = (const char *)bmat + chunk0 * bchunk;
 #define WM_LOAD_B(breg, sc)                                                                 \
   do {                                                                                      \
     const char *p_ = bsc0 + (int64_t)(sc) * 4 * bchunk;                                     \
@@ -493,12 +549,13 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
       zero_acc();
     };
     const int cl = (cw >> 1) * kMixCols;  // first column of this consumer in the group
-    auto consume = [&](int buf) {
+    // fresh: the first product overwrites acc (the previous sums were
+    // flushed), so no separate zeroing
+    auto consume = [&](int buf, const BFrag &bfr, auto fresh) {
       const unsigned char *abase = lds + Ly::a_off + buf * Ly::a_buf + pw * Ly::a_par;
-      const unsigned char *bbase = lds + Ly::b_off + buf * Ly::b_buf;
       if constexpr (S == 0) {
         const float *af = (const float *)abase;
-        const float *bf = (const float *)bbase;
+        const float *bf = (const float *)(lds + Ly::b_off + buf * Ly::b_buf);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
           float a[4], bv[2];
@@ -520,28 +577,34 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 #pragma unroll
         for (int q = 0; q < P; ++q)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct)
-            bv[q][ct] = *(const bf16x8 *)(bbase + q * Ly::b_part +
-                                          ((g * 2 + pw) * kMixCols * CB + cl + 16 * ct + i16) *
-                                              16);
+          for (int ct = 0; ct < 2; ++ct) bv[q][ct] = __builtin_bit_cast(bf16x8, bfr[q][ct]);
+        // row tiles in two halves: the A fragments of a half first (one LDS
+        // wait), then its products with the 4 accumulators interleaved, so
+        // that consecutive MFMAs are independent
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-          bf16x8 a[P];
+        for (int h2 = 0; h2 < 4; h2 += 2) {
+        bf16x8 a[4][P];
+#pragma unroll
+        for (int rt = h2; rt < h2 + 2; ++rt)
 #pragma unroll
           for (int q = 0; q < P; ++q)
-            a[q] = *(const bf16x8 *)(abase + q * Ly::a_part +
-                                     ((16 * rt + i16) * Ly::a_row + 8 * g) * 2);
+            a[rt][q] = *(const bf16x8 *)(abase + q * Ly::a_part +
+                                         ((16 * rt + i16) * Ly::a_row + 8 * g) * 2);
 #pragma unroll
-          for (int pi = 0; pi < Sc::nprod; ++pi)
+        for (int pi = 0; pi < Sc::nprod; ++pi)
+#pragma unroll
+          for (int rt = h2; rt < h2 + 2; ++rt)
 #pragma unroll
             for (int ct = 0; ct < 2; ++ct) {
+              const f32x4 c = (decltype(fresh)::value && pi == 0) ? (f32x4){0.f, 0.f, 0.f, 0.f}
+                                                                  : acc[rt][ct];
               if constexpr (S == 3)
                 acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    __builtin_bit_cast(f16x8, a[Sc::pa(pi)]),
-                    __builtin_bit_cast(f16x8, bv[Sc::pb(pi)][ct]), acc[rt][ct], 0, 0, 0);
+                    __builtin_bit_cast(f16x8, a[rt][Sc::pa(pi)]),
+                    __builtin_bit_cast(f16x8, bv[Sc::pb(pi)][ct]), c, 0, 0, 0);
               else
                 acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    a[Sc::pa(pi)], bv[Sc::pb(pi)][ct], acc[rt][ct], 0, 0, 0);
+                    a[rt][Sc::pa(pi)], bv[Sc::pb(pi)][ct], c, 0, 0, 0);
             }
         }
       }
@@ -565,7 +628,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
         }
         __syncthreads();
       }
-    } else {
+    } else if constexpr (S == 0) {
       int64_t bso[BU];
       int bdo[BU];
 #pragma unroll
@@ -577,6 +640,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
       zero_acc();
       __syncthreads();
       int nfl = 0;
+      BFrag none;
       for (int sc = 0; sc < nsc; ++sc) {
         // B of super-chunk sc + 1 into the buffer read last in sc - 1
         if (sc + 1 < nsc) {
@@ -584,7 +648,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
           if (sc + 2 < nsc) WM_LOAD_B(breg, sc + 2);
         }
         if (cons_on) {
-          consume(sc & 1);
+          consume(sc & 1, none, std::false_type());
           if (++nfl == kFlush) {
             flush();
             nfl = 0;
@@ -593,14 +657,48 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
         __syncthreads();
       }
       if (cons_on) flush();
+    } else {
+      // super-chunks in pairs: B fragments of even / odd super-chunks in
+      // b0 / b1, each reloaded (two super-chunks ahead) right after its
+      // use; the pair's FP32 sums are flushed once (kFlush = 2)
+      static_assert(kFlush == 2, "the 16-bit consumer loop flushes every two super-chunks");
+      BFrag b0, b1;
+      if (cons_on) {
+        load_b(b0, 0);
+        if (nsc > 1) load_b(b1, 1);
+      }
+      __syncthreads();
+      for (int sc = 0; sc < nsc; sc += 2) {
+        if (cons_on) {
+          consume(0, b0, std::true_type());
+          if (sc + 2 < nsc) load_b(b0, sc + 2);
+        }
+        __syncthreads();
+        if (sc + 1 < nsc) {
+          if (cons_on) {
+            consume(1, b1, std::false_type());
+            if (sc + 3 < nsc) load_b(b1, sc + 3);
+          }
+          __syncthreads();
+        }
+        if (cons_on) {
+#pragma unroll
+          for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                qd[rt][ct][r] += unscale[ct] * (double)acc[rt][ct][r];
+        }
+      }
     }
 #undef WM_LOAD_B
 #undef WM_STORE_B
   }
 
-  // ---- store (consumers): column col = 2 map + (Re, Im) of q(ring, pw, map, m)
+  // ---- store (consumers): column col = 2 map + (Re, Im) of q(ring, pw, map, m),
+  //      rounded to FP32 once after the FP64 l-chunk reduction when t.q32
   if (cons_on) {
-    double *qd_out = (double *)t.qbuf;
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -610,8 +708,13 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 #pragma unroll
           for (int ct = 0; ct < 2; ++ct) {
             const int col = cb * kMixCols + 16 * ct + i16;
-            if (col < 2 * K)
-              qd_out[2 * q_index(ring, pw, col >> 1, m, K, Mp1) + (col & 1)] = qd[rt][ct][r];
+            if (col < 2 * K) {
+              const int64_t at = 2 * q_index(ring, pw, col >> 1, m, K, Mp1) + (col & 1);
+              if (t.q32)
+                ((float *)t.qbuf)[at] = (float)qd[rt][ct][r];
+              else
+                ((double *)t.qbuf)[at] = qd[rt][ct][r];
+            }
           }
         }
       }
@@ -635,10 +738,26 @@ hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, i
     hipLaunchKernelGGL(k_mix_colmax, dim3(1024), dim3(256), (size_t)C2 * 8, s,
                        (const double *)alm, nalm, C2, colmax);
   }
-  const int64_t total = t.nlpad * NCp;
-  const int64_t nb = (total + 255) / 256;
-  hipLaunchKernelGGL(k_mix_prep, dim3((unsigned)(nb < 1048576 ? nb : 1048576)), dim3(256), 0,
-                     s, t, alm, bmat, NCp, scheme, part_stride, (const unsigned long long *)colmax);
+  const int64_t nchunks = t.nlpad / kChunk;
+  if (scheme == 0) {
+    hipLaunchKernelGGL(k_mix_prep_f32, dim3((unsigned)nchunks), dim3(256), 0, s, t, alm,
+                       (float *)bmat, NCp);
+  } else {
+    const int per = NCp;  // (parity, map) threads per chunk
+    const int cpb = 256 / per > 0 ? 256 / per : 1;
+    const dim3 grid((unsigned)((nchunks + cpb - 1) / cpb));
+    const auto cm = (const unsigned long long *)colmax;
+    unsigned short *bq = (unsigned short *)bmat;
+    if (scheme == 1)
+      hipLaunchKernelGGL(k_mix_prep_16<1>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
+                         cm, nchunks);
+    else if (scheme == 2)
+      hipLaunchKernelGGL(k_mix_prep_16<2>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
+                         cm, nchunks);
+    else
+      hipLaunchKernelGGL(k_mix_prep_16<3>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
+                         cm, nchunks);
+  }
   return hipGetLastError();
 }
 

@@ -268,6 +268,9 @@ static bool build_legendre_tables(wavemoth_plan p) {
   } else if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil")) {
     return false;
   }
+  // mixed-precision plans store q in FP32 (float2 elements at the same
+  // indices); the buffer keeps the FP64 size, which the adjoint uses
+  t.q32 = p->mix >= 0 ? 1 : 0;
   if (!dalloc(p->arena, &t.qbuf, (size_t)q_alloc_elems(t.R, p->K, p->M + 1), "qbuf")) return false;
   t.npix = 12 * (int64_t)p->N * p->N;
   return true;
@@ -759,8 +762,17 @@ int wavemoth_get_work(wavemoth_plan p, double *out) {
   }
   const int R = p->lt.R, M = p->M, K = p->K;
   std::vector<double2> q((size_t)q_elems(R, K, M + 1));
-  if (hipMemcpy(q.data(), p->lt.qbuf, q.size() * sizeof(double2), hipMemcpyDeviceToHost) !=
-      hipSuccess) {
+  if (p->lt.q32) {
+    // mixed-precision plans keep q in FP32 at the same indices
+    std::vector<float2> qf(q.size());
+    if (hipMemcpy(qf.data(), p->lt.qbuf, qf.size() * sizeof(float2), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      set_err("work download failed");
+      return -1;
+    }
+    for (size_t u = 0; u < q.size(); ++u) q[u] = make_double2(qf[u].x, qf[u].y);
+  } else if (hipMemcpy(q.data(), p->lt.qbuf, q.size() * sizeof(double2),
+                       hipMemcpyDeviceToHost) != hipSuccess) {
     set_err("work download failed");
     return -1;
   }

@@ -46,7 +46,9 @@ __device__ __forceinline__ void store_pix2(double *out, int p, double2 z, int os
 // holds lds_slots complex slots, the last kTwSlots of them the twiddle
 // table (already loaded).  Pixel p of the ring goes to outk[p * ostride]
 // (outk = map + k, ostride = nmaps: the (npix, nmaps) layout).
-// Used by k_ring_pairs and the batched kernel.
+// Used by k_ring_pairs and the batched kernel.  Q: the element type of q
+// (float2 for the mixed-precision plans, widened to double on load).
+template <typename Q>
 __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair, int k,
                                double2 *lds, int lds_slots, double *__restrict__ outk,
                                int ostride) {
@@ -55,8 +57,8 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
   const int n = rp.n, h = n / 2, B = h + 1, K = t.K, M = t.M, P = rp.P;
   const bool south = rp.pix_s >= 0;
   // q of map k: element m at qe[m * qs] (maps of a group interleaved)
-  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
-  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const Q *qe = (const Q *)t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const Q *qo = (const Q *)t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
   const int64_t qs = q_group(K);
   const double2 *chirp = rt.chirp + rp.chirp_off;
   const double2 *vhat = rt.vhat + rp.vhat_off;
@@ -89,7 +91,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
         for (int i = 0; i < kStage; ++i) {
           const int m = mb + tid + NT * i;
           if (m < me) {
-            const double2 e = qe[m * qs], o = qo[m * qs];
+            const double2 e = to_d(qe[m * qs]), o = to_d(qo[m * qs]);
             lds[pidx(m - mb)] = make_double2(fma(hs, o.x, e.x), fma(hs, o.y, e.y));
           }
         }
@@ -218,6 +220,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
 // dispatches block b to XCD b % 8) at consecutive dispatch slots: block
 // b = 8 (sigma G + j) + x runs map j of group gamma = 8 sigma + x, where
 // gamma enumerates (ring pair i, map group).
+template <typename Q>
 __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, int first,
                                                    int count, int lds_slots,
                                                    double *__restrict__ map) {
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
   const int i = gamma / nG, k = (gamma - i * nG) * G + j;
   if (i >= count || k >= K) return;
   load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
-  ring_pair_body(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, K);
+  ring_pair_body<Q>(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, K);
 }
 
 // Persistent ring stage of a batch: each block takes ring pairs of any
@@ -247,8 +250,8 @@ __global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int c
     if (it >= count) break;
     const BatchRingItem item = b.ritems[first + it];
     const LegTables &t = b.tabs[item.req];
-    ring_pair_body(t, b.rtabs[item.req], item.pair, item.k, lds, slots, b.map[item.req] + item.k,
-                   t.K);
+    ring_pair_body<double2>(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
+                            b.map[item.req] + item.k, t.K);
   }
 }
 
@@ -340,8 +343,11 @@ hipError_t ring_set_lds_limit(int slots) {
   std::lock_guard<std::mutex> lock(mu);
   if (slots <= cur) return hipSuccess;
   const int bytes = (int)((size_t)slots * sizeof(double2));
-  const hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs<double2>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_ring_pairs<float2>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess) cur = slots;
   return e;
 }
@@ -357,8 +363,12 @@ hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *m
       const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
       const int64_t groups = (int64_t)rt.count[i] * q_groups(t.K);
       const int64_t nb = (groups + 7) / 8 * 8 * q_group(t.K);
-      hipLaunchKernelGGL(k_ring_pairs, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
-                         rt.first[i], rt.count[i], rt.lds[i], map);
+      if (t.q32)
+        hipLaunchKernelGGL(k_ring_pairs<float2>, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
+                           rt.first[i], rt.count[i], rt.lds[i], map);
+      else
+        hipLaunchKernelGGL(k_ring_pairs<double2>, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
+                           rt.first[i], rt.count[i], rt.lds[i], map);
       e = hipGetLastError();
     }
     if (e != hipSuccess) return e;

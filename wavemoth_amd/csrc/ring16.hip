@@ -20,7 +20,14 @@
 //   i at plan time).  Splitting keeps every transform within 4096 points.
 //
 // One job = T = P/16 threads holding 16 points each (wm_fft16.h); a block of
-// 256 threads runs 256/T jobs.  Jobs of the G maps of one q group (they
+// 512 threads runs 512/T belt jobs or 512/(2T) cap jobs.
+//
+// Every kernel is instantiated for two point types C: double2 (the FP64
+// plans) and float2 (the mixed-precision plans, whose contraction stores q
+// in FP32).  With float2, q, the folds, LDS and the FFT arithmetic are FP32
+// (half the LDS, two blocks per CU); phase factors, twiddles and their
+// powers are formed in double and rounded once; pixels are stored as
+// double.  Jobs of the G maps of one q group (they
 // share 128-byte lines of q) and the blocks holding them are placed on one
 // XCD at consecutive dispatch slots.
 #include <mutex>
@@ -78,21 +85,28 @@ int64_t ring16_blocks(int kind, int P, int K, int count) {
   return (units + UB - 1) / UB;
 }
 
-__host__ __device__ constexpr int ring16_slots(int kind, int P) {
-  return (kind == kRingBelt16 ? 1 : 2) * ring16_jobs(kind, P) * job_slots(P) + kTwSlots;
+// LDS of one block: the twiddle table (double2) first, then the jobs'
+// buffers of cs-byte points
+__host__ __device__ constexpr size_t ring16_bytes(int kind, int P, int cs) {
+  return (size_t)kTwSlots * sizeof(double2) +
+         (size_t)(kind == kRingBelt16 ? 1 : 2) * ring16_jobs(kind, P) * job_slots(P) * cs;
 }
-int ring16_lds_slots(int kind, int P) { return ring16_slots(kind, P); }
+int ring16_lds_slots(int kind, int P) {
+  return (int)(ring16_bytes(kind, P, sizeof(double2)) / sizeof(double2));
+}
 
 // Z_a = (X_a + conj X_c) + i e^{2 pi i a/n} (X_a - conj X_c), c = h - a
-__device__ __forceinline__ double2 zpair(double2 xa, double2 xc, double2 om) {
-  const double2 cc = cconj(xc);
-  return cadd(cadd(xa, cc), cmul_i(cmul(om, csub(xa, cc))));
+template <typename C>
+__device__ __forceinline__ C zpair(C xa, C xc, double2 om) {
+  const C cc = cconj(xc);
+  return cadd(cadd(xa, cc), cmul_i(cmul(cx<C>::of(om), csub(xa, cc))));
 }
 
 // pixels 2p, 2p + 1 of one map of the (npix, K) map
-__device__ __forceinline__ void put2(double *out, int p, double2 z, int K) {
+template <typename C>
+__device__ __forceinline__ void put2(double *out, int p, C z, int K) {
   if (K == 1) {
-    *reinterpret_cast<double2 *>(out + 2 * p) = z;
+    *reinterpret_cast<double2 *>(out + 2 * p) = to_d(z);
   } else {
     out[(int64_t)(2 * p) * K] = z.x;
     out[(int64_t)(2 * p + 1) * K] = z.y;
@@ -100,12 +114,12 @@ __device__ __forceinline__ void put2(double *out, int p, double2 z, int K) {
 }
 
 // pixels 2p, 2p + 1 (p = j + T rho) of the 16 values of one thread
-template <int T>
-__device__ __forceinline__ void put16(double *out, int j, const double2 v[16], int K) {
+template <int T, typename C>
+__device__ __forceinline__ void put16(double *out, int j, const C v[16], int K) {
   if (K == 1) {
     double2 *o = reinterpret_cast<double2 *>(out) + j;
 #pragma unroll
-    for (int rho = 0; rho < 16; ++rho) o[T * rho] = v[rho];
+    for (int rho = 0; rho < 16; ++rho) o[T * rho] = to_d(v[rho]);
   } else {
     double *o = out + (int64_t)(2 * j) * K;
     const int64_t st = (int64_t)(2 * T) * K;
@@ -119,11 +133,11 @@ __device__ __forceinline__ void put16(double *out, int j, const double2 v[16], i
 }
 
 // Z pairs of the X values in buf (bins 0 .. h), the inverse FFT, the pixels.
-template <int P>
-__device__ __forceinline__ void belt_fft_out(double2 *buf, const double2 *twl, int j, int n,
+template <int P, typename C>
+__device__ __forceinline__ void belt_fft_out(C *buf, const double2 *twl, int j, int n,
                                              bool on, int K, double *__restrict__ out) {
   constexpr int T = P / 16, h = P;
-  double2 v[16];
+  C v[16];
   {
     const double2 ostep = expipi(2 * T, n);
     double2 om = expipi(2 * j, n);
@@ -143,23 +157,24 @@ __device__ __forceinline__ void belt_fft_out(double2 *buf, const double2 *twl, i
 // waits in registers.  The q rows past mmax are read (the q buffer has a
 // tail of kQTail rows) and discarded, so that all loads are unconditional
 // and in flight together.
-template <int P>
-__device__ __forceinline__ void belt_pair(const RingPair &rp, const double2 *__restrict__ qe,
-                                          const double2 *__restrict__ qo, int64_t qs, int M,
-                                          int K, double2 *buf, const double2 *twl, int j,
+template <int P, typename C>
+__device__ __forceinline__ void belt_pair(const RingPair &rp, const C *__restrict__ qe,
+                                          const C *__restrict__ qo, int64_t qs, int M,
+                                          int K, C *buf, const double2 *twl, int j,
                                           bool act, bool south, double *__restrict__ outn,
                                           double *__restrict__ outs) {
   constexpr int T = P / 16, h = P;
+  using X = cx<C>;
   const int n = rp.n;
   // 1. X_a = w_a (E_a +/- O_a), w_a = e^{i pi s a/n}, a = j + T rho; DC
   //    keeps its real part
-  double2 xs[16];
+  C xs[16];
   {
     const double2 wstep = expipi((int64_t)rp.s * T, n);
     double2 w = expipi((int64_t)rp.s * j, n);
-    const double2 *pe = qe + j * qs, *po = qo + j * qs;
+    const C *pe = qe + j * qs, *po = qo + j * qs;
     const int64_t st = T * qs;
-    double2 e[16], o[16];
+    C e[16], o[16];
 #pragma unroll
     for (int rho = 0; rho < 16; ++rho) {
       e[rho] = pe[rho * st];
@@ -169,26 +184,26 @@ __device__ __forceinline__ void belt_pair(const RingPair &rp, const double2 *__r
     for (int rho = 0; rho < 16; ++rho) {
       const int a = j + T * rho;
       const bool in = act && a <= M;
-      double2 xn = in ? cmul(cadd(e[rho], o[rho]), w) : make_double2(0.0, 0.0);
-      xs[rho] = in ? cmul(csub(e[rho], o[rho]), w) : make_double2(0.0, 0.0);
+      C xn = in ? cmul(cadd(e[rho], o[rho]), X::of(w)) : X::mk(0.0, 0.0);
+      xs[rho] = in ? cmul(csub(e[rho], o[rho]), X::of(w)) : X::mk(0.0, 0.0);
       if (a == 0) xn.y = xs[rho].y = 0.0;
       buf[pad(a)] = xn;
       w = cmul(w, wstep);
     }
   }
   // Nyquist bin h: e^{i pi s/2} (g_h + (-1)^s conj g_h), real part
-  double2 xsh = make_double2(0.0, 0.0);
+  C xsh = X::mk(0.0, 0.0);
   if (j == 0) {
-    double2 gn = make_double2(0.0, 0.0), gs = gn;
+    C gn = X::mk(0.0, 0.0), gs = gn;
     if (act && h <= M) {
-      const double2 e = qe[h * qs], o = qo[h * qs];
+      const C e = qe[h * qs], o = qo[h * qs];
       gn = cadd(e, o);
       gs = csub(e, o);
     }
     const double sg = rp.s ? -1.0 : 1.0;
-    const double2 wh = rp.s ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
-    double2 xn = cmul(make_double2(fma(sg, gn.x, gn.x), fma(-sg, gn.y, gn.y)), wh);
-    xsh = cmul(make_double2(fma(sg, gs.x, gs.x), fma(-sg, gs.y, gs.y)), wh);
+    const C wh = rp.s ? X::mk(0.0, 1.0) : X::mk(1.0, 0.0);
+    C xn = cmul(X::mk(fma(sg, (double)gn.x, (double)gn.x), fma(-sg, (double)gn.y, (double)gn.y)), wh);
+    xsh = cmul(X::mk(fma(sg, (double)gs.x, (double)gs.x), fma(-sg, (double)gs.y, (double)gs.y)), wh);
     xn.y = xsh.y = 0.0;
     buf[pad(h)] = xn;
   }
@@ -226,14 +241,14 @@ __device__ __forceinline__ void job_lane(int tid, int *jb, int *j) {
 }
 
 // Block b of a belt launch over pairs order[first .. first + count).
-template <int P>
+template <int P, typename C>
 __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables &rt, int first,
                                            int count, double *__restrict__ map, int b,
-                                           double2 *lds, const double2 *twl) {
+                                           C *lds, const double2 *twl) {
   constexpr int T = P / 16, J = RB / T, S = job_slots(P);
   int jb, j;
   job_lane<T>(threadIdx.x, &jb, &j);
-  double2 *buf = lds + jb * S;
+  C *buf = lds + jb * S;
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
   if (!act) {
@@ -243,8 +258,8 @@ __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables 
   const RingPair rp = rt.pairs[rt.order[first + pi]];
   const int K = t.K, M = t.M;
   const bool south = rp.pix_s >= 0;
-  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
-  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const C *qe = (const C *)t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const C *qo = (const C *)t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
   const int64_t qs = q_group(K);
   belt_pair<P>(rp, qe, qo, qs, M, K, buf, twl, j, act, south, map + k + rp.pix_n * K,
                map + k + (south ? rp.pix_s : 0) * K);
@@ -259,16 +274,20 @@ __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables 
 
 // F_b = P_b + (-1)^s conj P_{n-b} (0 < b <= h), F_0 = P_0 + conj(P_0 - g0),
 // in place over the P_rho (rho < n) in buf
-template <int P>
-__device__ __forceinline__ void wrap_fstep(double2 *buf, int n, int s, int tt, double2 g0) {
+template <int P, typename C>
+__device__ __forceinline__ void wrap_fstep(C *buf, int n, int s, int tt, C g0) {
+  using Rt = typename cx<C>::R;
   constexpr int T2 = P / 8;
   const int h = n / 2;
-  const double sg = s ? -1.0 : 1.0;
+  const Rt sg = s ? -1 : 1;
   for (int b = tt; b <= h; b += T2) {
-    const double2 pb = buf[pad(b)];
-    const double2 pc = b == 0 ? csub(pb, g0) : buf[pad(n - b)];
-    const double c = b == 0 ? 1.0 : sg;
-    buf[pad(b)] = make_double2(fma(c, pc.x, pb.x), fma(-c, pc.y, pb.y));
+    const C pb = buf[pad(b)];
+    const C pc = b == 0 ? csub(pb, g0) : buf[pad(n - b)];
+    const Rt c = b == 0 ? 1 : sg;
+    C f;
+    f.x = fma(c, pc.x, pb.x);
+    f.y = fma(-c, pc.y, pb.y);
+    buf[pad(b)] = f;
   }
 }
 
@@ -283,23 +302,24 @@ __device__ __forceinline__ void wrap_fstep(double2 *buf, int n, int s, int tt, d
 // g = E + O (north, folded into buf here) and g = E - O (south, returned
 // in ps[] and *g0s for wrap_fold_south).  Rows of bins rho >= n are not
 // read: those lanes load the row of bin tt again.
-template <int P>
-__device__ __forceinline__ void wrap_fold2(const double2 *__restrict__ qe,
-                                           const double2 *__restrict__ qo, int64_t qs, int M,
-                                           int n, int s, double2 *buf, int tt, bool on_n,
-                                           bool on_s, double2 ps[16], double2 *g0s) {
+template <int P, typename C>
+__device__ __forceinline__ void wrap_fold2(const C *__restrict__ qe, const C *__restrict__ qo,
+                                           int64_t qs, int M, int n, int s, C *buf, int tt,
+                                           bool on_n, bool on_s, C ps[16], C *g0s) {
+  using Rt = typename cx<C>::R;
+  using X = cx<C>;
   constexpr int T2 = P / 8;
-  double2 an[16];
+  C an[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) an[q] = ps[q] = make_double2(0.0, 0.0);
-  double2 g0n = make_double2(0.0, 0.0);
-  *g0s = g0n;
+  for (int q = 0; q < 16; ++q) an[q] = ps[q] = X::mk(0.0, 0.0);
+  C g0n = X::mk(0.0, 0.0);
+  *g0s = X::mk(0.0, 0.0);
   for (int w0 = 0; w0 <= M; w0 += n) {
-    const double se = (s && ((w0 / n) & 1)) ? -1.0 : 1.0;
+    const Rt se = (s && ((w0 / n) & 1)) ? -1 : 1;
     const int64_t m0 = w0 + tt <= M ? w0 + tt : M;
 #pragma unroll
     for (int h8 = 0; h8 < 16; h8 += 8) {
-      double2 e[8], o[8];
+      C e[8], o[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int rho = tt + T2 * (h8 + q), m = w0 + rho;
@@ -311,7 +331,7 @@ __device__ __forceinline__ void wrap_fold2(const double2 *__restrict__ qe,
       for (int q = 0; q < 8; ++q) {
         const int rho = tt + T2 * (h8 + q), m = w0 + rho;
         if (rho < n && m <= M) {
-          const double ex = se * e[q].x, ey = se * e[q].y;
+          const Rt ex = se * e[q].x, ey = se * e[q].y;
           an[h8 + q].x += fma(se, o[q].x, ex);
           an[h8 + q].y += fma(se, o[q].y, ey);
           ps[h8 + q].x += fma(-se, o[q].x, ex);
@@ -327,19 +347,19 @@ __device__ __forceinline__ void wrap_fold2(const double2 *__restrict__ qe,
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int rho = tt + T2 * q;
-    if (rho < n) buf[pad(rho)] = on_n ? an[q] : make_double2(0.0, 0.0);
-    if (!on_s) ps[q] = make_double2(0.0, 0.0);
+    if (rho < n) buf[pad(rho)] = on_n ? an[q] : X::mk(0.0, 0.0);
+    if (!on_s) ps[q] = X::mk(0.0, 0.0);
   }
   __syncthreads();
-  wrap_fstep<P>(buf, n, s, tt, on_n ? g0n : make_double2(0.0, 0.0));
+  wrap_fstep<P>(buf, n, s, tt, on_n ? g0n : X::mk(0.0, 0.0));
   __syncthreads();
 }
 
 // The southern fold of wrap_fold2 into buf.  The caller has passed a
 // barrier after its last read of buf.
-template <int P>
-__device__ __forceinline__ void wrap_fold_south(double2 *buf, int n, int s, int tt,
-                                                const double2 ps[16], double2 g0s) {
+template <int P, typename C>
+__device__ __forceinline__ void wrap_fold_south(C *buf, int n, int s, int tt, const C ps[16],
+                                                C g0s) {
   constexpr int T2 = P / 8;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
@@ -356,15 +376,15 @@ __device__ __forceinline__ void wrap_fold_south(double2 *buf, int n, int s, int 
 // (one writer per bin in each pass); h < P and n > mmax give mmax - h < h,
 // so 8 rows per thread cover each pass.  g = E + O (north) into bufN,
 // g = E - O (south) into bufS.
-template <int P>
-__device__ __forceinline__ void cap_fold2(const double2 *__restrict__ qe,
-                                          const double2 *__restrict__ qo, int64_t qs, int M,
-                                          int n, int s, double2 *bufN, double2 *bufS, int tt,
-                                          bool on_n, bool on_s) {
+template <int P, typename C>
+__device__ __forceinline__ void cap_fold2(const C *__restrict__ qe, const C *__restrict__ qo,
+                                          int64_t qs, int M, int n, int s, C *bufN, C *bufS,
+                                          int tt, bool on_n, bool on_s) {
+  using Rt = typename cx<C>::R;
   constexpr int T2 = P / 8;
   const int h = n / 2;
-  const double sg = s ? -1.0 : 1.0;
-  double2 e[16], o[16];
+  const Rt sg = s ? -1 : 1;
+  C e[16], o[16];
 #pragma unroll
   for (int rho = 0; rho < 8; ++rho) {
     const int b = tt + T2 * rho, m = h + b;
@@ -374,28 +394,33 @@ __device__ __forceinline__ void cap_fold2(const double2 *__restrict__ qe,
     e[8 + rho] = qe[mc * qs];
     o[8 + rho] = qo[mc * qs];
   }
-  double2 cn[8], cs[8];
+  using X = cx<C>;
+  C cn[8], cs[8];
 #pragma unroll
   for (int rho = 0; rho < 8; ++rho) {
     const int b = tt + T2 * rho, m = h + b;
     const bool db = b <= M, dm = m <= M;
-    const double2 zero = make_double2(0.0, 0.0);
     if (b <= h) {
-      bufN[pad(b)] = (on_n && db) ? cadd(e[rho], o[rho]) : zero;
-      bufS[pad(b)] = (on_s && db) ? csub(e[rho], o[rho]) : zero;
+      bufN[pad(b)] = (on_n && db) ? cadd(e[rho], o[rho]) : X::mk(0.0, 0.0);
+      bufS[pad(b)] = (on_s && db) ? csub(e[rho], o[rho]) : X::mk(0.0, 0.0);
     }
-    const double2 c = cadd(e[8 + rho], o[8 + rho]), d = csub(e[8 + rho], o[8 + rho]);
-    cn[rho] = (on_n && dm) ? make_double2(sg * c.x, -sg * c.y) : zero;
-    cs[rho] = (on_s && dm) ? make_double2(sg * d.x, -sg * d.y) : zero;
+    const C c = cadd(e[8 + rho], o[8 + rho]), d = csub(e[8 + rho], o[8 + rho]);
+    C cc, dd;
+    cc.x = sg * c.x;
+    cc.y = -sg * c.y;
+    dd.x = sg * d.x;
+    dd.y = -sg * d.y;
+    cn[rho] = (on_n && dm) ? cc : X::mk(0.0, 0.0);
+    cs[rho] = (on_s && dm) ? dd : X::mk(0.0, 0.0);
   }
   __syncthreads();
 #pragma unroll
   for (int rho = 0; rho < 8; ++rho) {
     const int m = h + tt + T2 * rho;
     if (m <= M) {
-      double2 &fn = bufN[pad(n - m)];
+      C &fn = bufN[pad(n - m)];
       fn = cadd(fn, cn[rho]);
-      double2 &fs = bufS[pad(n - m)];
+      C &fs = bufS[pad(n - m)];
       fs = cadd(fs, cs[rho]);
     }
   }
@@ -406,9 +431,10 @@ __device__ __forceinline__ void cap_fold2(const double2 *__restrict__ qe,
 // b = 2 kappa + half (kappa = j + T rho < i < P/2, so rho < 8) times the
 // chirp c_kappa; phase w_b = e^{i pi s b/n}, partner phase
 // e^{i pi s/2} conj(w_b), twiddle e^{2 pi i b/n}; DC / Nyquist real.
-template <int P>
+template <int P, typename C>
 __device__ __forceinline__ void cap_z(const RingPair &rp, const double2 *__restrict__ chirp,
-                                      const double2 *bufF, int tt, double2 z[8]) {
+                                      const C *bufF, int tt, C z[8]) {
+  using X = cx<C>;
   constexpr int T = P / 16;
   const int n = rp.n, h = n / 2, i = h / 2;
   const int half = tt / T, j = tt - half * T;
@@ -419,12 +445,12 @@ __device__ __forceinline__ void cap_z(const RingPair &rp, const double2 *__restr
   for (int rho = 0; rho < 8; ++rho) {
     const int kap = j + T * rho;
     const bool in = kap < i;
-    const double2 c = chirp[in ? kap : 0];
+    const C c = X::of(chirp[in ? kap : 0]);
     const int b = in ? 2 * kap + half : 0;
-    double2 xa = cmul(bufF[pad(b)], w);
-    double2 xc = cmul(bufF[pad(h - b)], b == 0 ? wh : cmul(wh, cconj(w)));
-    if (b == 0) xa.y = xc.y = 0.0;
-    z[rho] = in ? cmul(zpair(xa, xc, om), c) : make_double2(0.0, 0.0);
+    C xa = cmul(bufF[pad(b)], X::of(w));
+    C xc = cmul(bufF[pad(h - b)], X::of(b == 0 ? wh : cmul(wh, cconj(w))));
+    if (b == 0) xa.y = xc.y = 0;
+    z[rho] = in ? cmul(zpair(xa, xc, om), c) : X::mk(0.0, 0.0);
     w = cmul(w, wstep);
     om = cmul(om, ostep);
   }
@@ -433,27 +459,28 @@ __device__ __forceinline__ void cap_z(const RingPair &rp, const double2 *__restr
 // One cap ring of one map from its Z entries: the two Bluestein
 // convolutions, the pixels.  tt = thread in the job (0 .. 2T), bufA / bufB
 // the job's two buffers (free: the transform starts with a barrier).
-template <int P>
+template <int P, typename C>
 __device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
                                           const double2 *__restrict__ chirp,
-                                          const double2 *__restrict__ vhat, const double2 z[8],
-                                          double2 *bufA, double2 *bufB, const double2 *twl,
-                                          int tt, bool on, double *__restrict__ out) {
+                                          const double2 *__restrict__ vhat, const C z[8],
+                                          C *bufA, C *bufB, const double2 *twl, int tt, bool on,
+                                          double *__restrict__ out) {
+  using X = cx<C>;
   constexpr int T = P / 16, T2 = 2 * T;
   const int n = rp.n, h = n / 2, i = h / 2;
   const int half = tt / T, j = tt - half * T;
-  double2 v[16];
+  C v[16];
 #pragma unroll
   for (int rho = 0; rho < 8; ++rho) {
     v[rho] = z[rho];
-    v[8 + rho] = make_double2(0.0, 0.0);
+    v[8 + rho] = X::mk(0.0, 0.0);
   }
   // Bluestein: y = IFFT(vhat . FFT(c . Z)) in the half's own buffer
-  double2 *buf = half ? bufB : bufA;
+  C *buf = half ? bufB : bufA;
   fft16::fft<P, false>(v, buf, twl, j);
   asm volatile("" : "+v"(tt));
 #pragma unroll
-  for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], vhat[j + T * rho]);
+  for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], X::of(vhat[j + T * rho]));
   fft16::fft<P, true>(v, buf, twl, j);
   asm volatile("" : "+v"(tt));
   // A_p = c_p y_p into bufA, e^{2 pi i p/h} c_p y_p (B) into bufB, p < i
@@ -465,7 +492,7 @@ __device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
 #pragma unroll
     for (int rho = 0; rho < 8; ++rho) {
       const int p = j + T * rho;
-      if (p < i) buf[pad(p)] = cmul(cmul(v[rho], chirp[p]), om);
+      if (p < i) buf[pad(p)] = cmul(v[rho], X::of(cmul(chirp[p], om)));
       om = cmul(om, st);
     }
   }
@@ -476,7 +503,7 @@ __device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
     for (int rho = 0; rho < 4; ++rho) {
       const int p = tt + T2 * rho;
       if (p < i) {
-        const double2 a = bufA[pad(p)], b = bufB[pad(p)];
+        const C a = bufA[pad(p)], b = bufB[pad(p)];
         put2(out, p, cadd(a, b), K);
         put2(out, p + i, csub(a, b), K);
       }
@@ -486,14 +513,14 @@ __device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
 
 // Block b of a cap (WRAP: wrapping-ring) launch over pairs
 // order[first .. first + count).
-template <int P, bool WRAP>
+template <int P, bool WRAP, typename C>
 __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &rt, int first,
                                           int count, double *__restrict__ map, int b,
-                                          double2 *lds, const double2 *twl) {
+                                          C *lds, const double2 *twl) {
   constexpr int T = P / 16, J = RB / (2 * T), S = job_slots(P);
   const int jb = threadIdx.x / (2 * T);
   int tt = threadIdx.x % (2 * T);
-  double2 *bufA = lds + 2 * jb * S, *bufB = bufA + S;
+  C *bufA = lds + 2 * jb * S, *bufB = bufA + S;
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
   if (!act) {
@@ -503,15 +530,15 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
   const RingPair rp = rt.pairs[rt.order[first + pi]];
   const int K = t.K, M = t.M;
   const bool south = rp.pix_s >= 0;
-  const double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
-  const double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const C *qe = (const C *)t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const C *qo = (const C *)t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
   const int64_t qs = q_group(K);
   // every pair of a cap launch (also the padding jobs' stand-in) has tables
   const double2 *chirp = rt.chirp2 + rp.chirp2_off;  // [i]
   const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
-  double2 zn[8], zs[8];
+  C zn[8], zs[8];
   if constexpr (WRAP) {
-    double2 ps[16], g0s;
+    C ps[16], g0s;
     wrap_fold2<P>(qe, qo, qs, M, rp.n, rp.s, bufA, tt, act, act && south, ps, &g0s);
     cap_z<P>(rp, chirp, bufA, tt, zn);
     __syncthreads();  // reads of the northern fold done
@@ -530,10 +557,10 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
                map + k + (south ? rp.pix_s : 0) * K);
 }
 
-template <int KIND, int P>
+template <int KIND, int P, typename C>
 __device__ __forceinline__ void ring16_block(const LegTables &t, const RingTables &rt, int first,
                                              int count, double *__restrict__ map, int b,
-                                             double2 *lds, const double2 *twl) {
+                                             C *lds, const double2 *twl) {
   if constexpr (KIND == kRingBelt16)
     belt_block<P>(t, rt, first, count, map, b, lds, twl);
   else
@@ -545,13 +572,20 @@ __device__ __forceinline__ void load_twiddles16(double2 *twl, const double2 *__r
     twl[u] = (u < kTwC) ? tw[u * 64] : tw[u - kTwC];
 }
 
-template <int KIND, int P>
-__global__ __launch_bounds__(RB, 1) void k_ring16(LegTables t, RingTables rt, int first,
-                                                  int count, double *__restrict__ map) {
+// Waves per SIMD the kernel is built for (the second launch-bounds
+// argument): one 512-thread block per CU.  FP32 points would fit the LDS of
+// two blocks, but 4 waves per SIMD (<= 128 VGPRs) spill and measured slower.
+template <int KIND, typename C> constexpr int ring16_waves_per_eu() { return 2; }
+
+template <int KIND, int P, typename C>
+__global__ __launch_bounds__(RB, (ring16_waves_per_eu<KIND, C>())) void k_ring16(LegTables t,
+                                                                        RingTables rt, int first,
+                                                                        int count,
+                                                                        double *__restrict__ map) {
   extern __shared__ double2 lds[];
-  double2 *twl = lds + ring16_slots(KIND, P) - kTwSlots;
+  double2 *twl = lds;
   load_twiddles16(twl, rt.twiddle);  // first read after a barrier
-  ring16_block<KIND, P>(t, rt, first, count, map, blockIdx.x, lds, twl);
+  ring16_block<KIND, P>(t, rt, first, count, map, blockIdx.x, (C *)(lds + kTwSlots), twl);
 }
 
 // Persistent ring kernel of a batch: blocks pull (request, launch group,
@@ -562,7 +596,7 @@ __global__ __launch_bounds__(RB, 1) void k_batch_ring16(BatchDev d, int first, i
                                                         int ctr) {
   extern __shared__ double2 lds[];
   __shared__ int s_item;
-  double2 *twl = lds + ring16_slots(KIND, P) - kTwSlots;
+  double2 *twl = lds;
   load_twiddles16(twl, d.twiddle);
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(&d.counters[ctr], 1);
@@ -573,7 +607,8 @@ __global__ __launch_bounds__(RB, 1) void k_batch_ring16(BatchDev d, int first, i
     const BatchRing16Item x = d.r16items[first + it];
     const LegTables &t = d.tabs[x.req];
     const RingTables &rt = d.rtabs[x.req];
-    ring16_block<KIND, P>(t, rt, rt.first[x.g], rt.count[x.g], d.map[x.req], x.b, lds, twl);
+    ring16_block<KIND, P>(t, rt, rt.first[x.g], rt.count[x.g], d.map[x.req], x.b,
+                          lds + kTwSlots, twl);
     __syncthreads();  // LDS reused by the next item
   }
 }
@@ -587,10 +622,15 @@ hipError_t ring16_set_lds_limits() {
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
 #define WM_SET_K(KIND, P)                                                                  \
-  for (const void *f : {(const void *)k_ring16<KIND, P>, (const void *)k_batch_ring16<KIND, P>}) \
+  for (const void *f : {(const void *)k_ring16<KIND, P, double2>,                          \
+                        (const void *)k_batch_ring16<KIND, P>})                            \
     if (err == hipSuccess)                                                                 \
       err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,             \
-                                (int)(ring16_slots(KIND, P) * sizeof(double2)));
+                                (int)ring16_bytes(KIND, P, sizeof(double2)));              \
+  if (err == hipSuccess)                                                                   \
+    err = hipFuncSetAttribute((const void *)k_ring16<KIND, P, float2>,                     \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,                  \
+                              (int)ring16_bytes(KIND, P, sizeof(float2)));
 #define WM_SET(P) WM_SET_K(kRingBelt16, P) WM_SET_K(kRingCap16, P) WM_SET_K(kRingWrap16, P)
     WM_RING16_SIZES(WM_SET)
 #undef WM_SET
@@ -599,16 +639,21 @@ hipError_t ring16_set_lds_limits() {
   return err;
 }
 
-// pairs order[first .. first + count) of one launch group (kind, FFT size P)
+// pairs order[first .. first + count) of one launch group (kind, FFT size
+// P); t.q32: q in FP32 and the FP32 kernels
 hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &rt, int first,
                          int count, double *map, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const int64_t nb = ring16_blocks(kind, P, t.K, count);
-  const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
+  const size_t lds = ring16_bytes(kind, P, t.q32 ? sizeof(float2) : sizeof(double2));
 #define WM_CASE_K(KIND, PP)                                                                \
   if (kind == KIND && P == PP) {                                                           \
-    hipLaunchKernelGGL((k_ring16<KIND, PP>), dim3((unsigned)nb), dim3(RB), lds, s, t, rt,  \
-                       first, count, map);                                                 \
+    if (t.q32)                                                                             \
+      hipLaunchKernelGGL((k_ring16<KIND, PP, float2>), dim3((unsigned)nb), dim3(RB), lds,  \
+                         s, t, rt, first, count, map);                                     \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_ring16<KIND, PP, double2>), dim3((unsigned)nb), dim3(RB), lds, \
+                         s, t, rt, first, count, map);                                     \
     return hipGetLastError();                                                              \
   }
 #define WM_CASE(PP) WM_CASE_K(kRingBelt16, PP) WM_CASE_K(kRingCap16, PP) WM_CASE_K(kRingWrap16, PP)
@@ -623,7 +668,7 @@ hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &
 hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count, int ctr,
                                int nblocks, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
+  const size_t lds = ring16_bytes(kind, P, sizeof(double2));
 #define WM_CASE_K(KIND, PP)                                                                \
   if (kind == KIND && P == PP) {                                                           \
     hipLaunchKernelGGL((k_batch_ring16<KIND, PP>), dim3(nblocks), dim3(RB), lds, s, d,     \
@@ -638,7 +683,7 @@ hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, in
 }
 
 hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu) {
-  const size_t lds = (size_t)ring16_slots(kind, P) * sizeof(double2);
+  const size_t lds = ring16_bytes(kind, P, sizeof(double2));
 #define WM_CASE_K(KIND, PP)                                                                \
   if (kind == KIND && P == PP)                                                             \
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                     \

@@ -38,9 +38,42 @@ static __device__ __forceinline__ double2 cscale(double2 a, double s) { return m
 static __device__ __forceinline__ double2 cmul_i(double2 a) { return make_double2(-a.y, a.x); }
 static __device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
 
-template <bool INV>
-static __device__ __forceinline__ void dft4(double2 &a, double2 &b, double2 &c, double2 &d) {
-  const double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
+// The same operations on FP32 complex values (the ring stage of the
+// mixed-precision plans).
+static __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+static __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+static __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+static __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+static __device__ __forceinline__ float2 cmul_i(float2 a) { return make_float2(-a.y, a.x); }
+static __device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// Complex value type C (double2 or float2): its real type, a constant from
+// doubles, conversions from and to double2.
+template <typename C> struct cx;
+template <> struct cx<double2> {
+  using R = double;
+  static __device__ __forceinline__ double2 mk(double a, double b) { return make_double2(a, b); }
+  static __device__ __forceinline__ double2 of(double2 a) { return a; }
+};
+template <> struct cx<float2> {
+  using R = float;
+  static __device__ __forceinline__ float2 mk(double a, double b) {
+    return make_float2((float)a, (float)b);
+  }
+  static __device__ __forceinline__ float2 of(double2 a) {
+    return make_float2((float)a.x, (float)a.y);
+  }
+};
+static __device__ __forceinline__ double2 to_d(double2 a) { return a; }
+static __device__ __forceinline__ double2 to_d(float2 a) {
+  return make_double2((double)a.x, (double)a.y);
+}
+
+template <bool INV, typename C>
+static __device__ __forceinline__ void dft4(C &a, C &b, C &c, C &d) {
+  const C t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
   a = cadd(t0, t2);
   c = csub(t0, t2);
   if (INV) {
@@ -53,18 +86,18 @@ static __device__ __forceinline__ void dft4(double2 &a, double2 &b, double2 &c, 
 }
 
 // In-register 8-point DFT, y_k = sum_r v_r e^{-/+ 2 pi i r k / 8}.
-template <bool INV>
-static __device__ __forceinline__ void dft8(double2 v[8]) {
+template <bool INV, typename C>
+static __device__ __forceinline__ void dft8(C v[8]) {
   constexpr double c = 0.70710678118654752440;
   dft4<INV>(v[0], v[2], v[4], v[6]);
   dft4<INV>(v[1], v[3], v[5], v[7]);
   // after dft4: e_k in v[0], v[2], v[4], v[6] (k = 0..3), o_k likewise odd
-  const double2 w1 = INV ? make_double2(c, c) : make_double2(c, -c);
-  const double2 w3 = INV ? make_double2(-c, c) : make_double2(-c, -c);
-  const double2 o1 = cmul(v[3], w1);
-  const double2 o2 = INV ? cmul_i(v[5]) : cmul_mi(v[5]);
-  const double2 o3 = cmul(v[7], w3);
-  const double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  const C w1 = INV ? cx<C>::mk(c, c) : cx<C>::mk(c, -c);
+  const C w3 = INV ? cx<C>::mk(-c, c) : cx<C>::mk(-c, -c);
+  const C o1 = cmul(v[3], w1);
+  const C o2 = INV ? cmul_i(v[5]) : cmul_mi(v[5]);
+  const C o3 = cmul(v[7], w3);
+  const C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
   v[0] = cadd(e0, o0);
   v[4] = csub(e0, o0);
   v[1] = cadd(e1, o1);

@@ -2,6 +2,9 @@
 // job holds 16 points in registers, passes of radix 16 (the last one radix
 // 16, 8, 4 or 2) exchange through one padded LDS buffer per job.
 //
+// The point type C is double2 or float2 (FP32 data, half the LDS); the
+// twiddle table and the twiddle powers stay in double either way.
+//
 // Canonical layout: thread j of a job holds positions j + T rho in v[rho],
 // rho = 0..15.  A Stockham pass of radix R over stride Ns reads butterfly
 // j' = j + T q (q < 16/R) inputs x[j' + (P/R) r], twiddles input r by
@@ -33,18 +36,18 @@ __host__ __device__ constexpr int job_slots(int P) { return pad(P) + 2; }
 
 // 16-point DFT in registers, y_k = sum_r v_r e^{-/+ 2 pi i r k/16} (INV: +).
 // Radix 4 x 4: r = r0 + 4 r1, k = k1 + 4 k2; y_k ends in v[4 (k & 3) + (k >> 2)].
-template <bool INV>
-__device__ __forceinline__ void dft16(double2 v[16]) {
+template <bool INV, typename C>
+__device__ __forceinline__ void dft16(C v[16]) {
   constexpr double c1 = 0.92387953251128675613;  // cos(pi/8)
   constexpr double s1 = 0.38268343236508977173;  // sin(pi/8)
   constexpr double r2 = 0.70710678118654752440;
   constexpr double sg = INV ? 1.0 : -1.0;
+  using X = fft::cx<C>;
 #pragma unroll
   for (int r0 = 0; r0 < 4; ++r0) fft::dft4<INV>(v[r0], v[r0 + 4], v[r0 + 8], v[r0 + 12]);
   // slot r0 + 4 k1 *= w^{r0 k1}, w = e^{-/+ 2 pi i/16}
-  const double2 w1 = make_double2(c1, sg * s1), w2 = make_double2(r2, sg * r2),
-                w3 = make_double2(s1, sg * c1), w6 = make_double2(-r2, sg * r2),
-                w9 = make_double2(-c1, -sg * s1);
+  const C w1 = X::mk(c1, sg * s1), w2 = X::mk(r2, sg * r2), w3 = X::mk(s1, sg * c1),
+          w6 = X::mk(-r2, sg * r2), w9 = X::mk(-c1, -sg * s1);
   v[5] = cmul(v[5], w1);
   v[9] = cmul(v[9], w2);
   v[13] = cmul(v[13], w3);
@@ -60,10 +63,10 @@ __device__ __forceinline__ void dft16(double2 v[16]) {
 }
 
 // Radix-R DFT of x[0..R) in place, natural output order.
-template <int R, bool INV>
-__device__ __forceinline__ void dft_r(double2 *x) {
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void dft_r(C *x) {
   if constexpr (R == 16) {
-    double2 t[16];
+    C t[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) t[r] = x[r];
     dft16<INV>(t);
@@ -74,7 +77,7 @@ __device__ __forceinline__ void dft_r(double2 *x) {
   } else if constexpr (R == 4) {
     fft::dft4<INV>(x[0], x[1], x[2], x[3]);
   } else {
-    const double2 a = x[0];
+    const C a = x[0];
     x[0] = cadd(a, x[1]);
     x[1] = csub(a, x[1]);
   }
@@ -82,13 +85,13 @@ __device__ __forceinline__ void dft_r(double2 *x) {
 
 // One pass (Ns, R) of a P-point transform; buf is this job's LDS buffer,
 // twl the two-level twiddle table, j the thread's index in its job.
-template <int P, int Ns, bool INV>
-__device__ __forceinline__ void pass(double2 v[16], double2 *buf, const double2 *twl, int j) {
+template <int P, int Ns, bool INV, typename C>
+__device__ __forceinline__ void pass(C v[16], C *buf, const double2 *twl, int j) {
   constexpr int T = P / 16;
   constexpr int R = (P / Ns >= 16) ? 16 : P / Ns;
   constexpr int Q = 16 / R;   // butterflies per thread
   constexpr int NB = P / R;   // butterflies per job
-  double2 x[Q][R];
+  C x[Q][R];
   if constexpr (Ns == 1) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) x[0][r] = v[r];
@@ -105,7 +108,7 @@ __device__ __forceinline__ void pass(double2 v[16], double2 *buf, const double2 
       double2 p = w;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        x[q][r] = cmul(x[q][r], p);
+        x[q][r] = cmul(x[q][r], fft::cx<C>::of(p));
         if (r + 1 < R) p = cmul(p, w);
       }
     }
@@ -137,8 +140,8 @@ __device__ __forceinline__ void pass(double2 v[16], double2 *buf, const double2 
 // unnormalised.  Starts with a barrier before its first LDS write, so
 // the caller may still read buf up to the call.  Every thread of the
 // block must call it.
-template <int P, bool INV>
-__device__ __forceinline__ void fft(double2 v[16], double2 *buf, const double2 *twl, int j) {
+template <int P, bool INV, typename C>
+__device__ __forceinline__ void fft(C v[16], C *buf, const double2 *twl, int j) {
   static_assert(P >= 16 && P <= 4096 && (P & (P - 1)) == 0, "P");
   pass<P, 1, INV>(v, buf, twl, j);
 }

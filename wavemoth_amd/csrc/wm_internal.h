@@ -43,6 +43,7 @@ struct LegTables {
   int *wseedmax = nullptr;    // [(M+1)*W] max seed_l (<= lmax) over a wave
   double2 *atil = nullptr;    // [ceil(K/KB)][nlpad][KB] prescaled a_lm (per execute)
   double2 *qbuf = nullptr;    // [R][2][K][M+1] even/odd Legendre sums (q_index)
+  int q32 = 0;                // q stored as float2 at the same indices (mixed plans)
   int64_t npix = 0;
 };
 
