@@ -38,7 +38,8 @@ def _err(x, ref):
 @pytest.mark.parametrize("scheme", [F32, BF3, BF6, F16])
 @pytest.mark.parametrize("N,L,M,K,flat", [(32, 64, 64, 16, False), (64, 128, 128, 32, False),
                                           (64, 128, 128, 32, True), (16, 40, 20, 3, False),
-                                          (128, 256, 256, 17, False)])
+                                          (128, 256, 256, 17, False),
+                                          (128, 512, 512, 8, False)])
 def test_scheme_vs_fp64(scheme, N, L, M, K, flat):
     import torch
     from wavemoth_amd.torch_sht import random_alm

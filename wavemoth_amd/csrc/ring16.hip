@@ -48,25 +48,27 @@ __host__ __device__ inline int pow2_ceil(int v) {
 }
 
 // Job (pair index, map) of job slot jb of block b.  U = pow2_ceil(q_group(K))
-// jobs per unit (pair, q group), J jobs per block.  Returns false for a
-// padding job.
+// jobs per unit (pair, q group), J jobs per block.  Block b runs on XCD
+// x = b mod 8; the units of XCD x are pairs x, x + 8, ..., each with all its
+// q groups, so that every line of a pair's pixel rows (all maps of a pixel)
+// is written through one L2.  Returns false for a padding job.
 __device__ __forceinline__ bool ring16_job(int b, int J, int jb, int K, int count, int *pi,
                                            int *k) {
   const int G = q_group(K), nG = q_groups(K), U = pow2_ceil(G);
-  int gamma, u;
+  const int x = b & 7, s = b >> 3;
+  int sg, u;  // unit index on XCD x, job in unit
   if (J <= U) {
-    const int BU = U / J;  // blocks per unit, same XCD
-    const int x = b & 7, s = b >> 3;
-    const int sigma = s / BU, r = s - sigma * BU;
-    gamma = sigma * 8 + x;
-    u = r * J + jb;
+    const int BU = U / J;  // blocks per unit, consecutive on XCD x
+    sg = s / BU;
+    u = (s - sg * BU) * J + jb;
   } else {
     const int UB = J / U;  // units per block
-    gamma = b * UB + jb / U;
+    sg = s * UB + jb / U;
     u = jb & (U - 1);
   }
-  *pi = gamma / nG;
-  *k = (gamma - *pi * nG) * G + u;
+  const int q = sg / nG;
+  *pi = q * 8 + x;
+  *k = (sg - q * nG) * G + u;
   return *pi < count && u < G && *k < K;
 }
 
@@ -79,17 +81,22 @@ __host__ __device__ constexpr int ring16_jobs(int kind, int P) {
 int64_t ring16_blocks(int kind, int P, int K, int count) {
   const int J = ring16_jobs(kind, P);
   const int G = q_group(K), U = pow2_ceil(G);
-  const int64_t units = (int64_t)count * q_groups(K);
-  if (J <= U) return (units + 7) / 8 * 8 * (U / J);
+  const int64_t ux = (int64_t)(count + 7) / 8 * q_groups(K);  // units per XCD
+  if (J <= U) return 8 * ux * (U / J);
   const int UB = J / U;
-  return (units + UB - 1) / UB;
+  return 8 * ((ux + UB - 1) / UB);
 }
 
 // LDS of one block: the twiddle table (double2) first, then the jobs'
 // buffers of cs-byte points
+// buffers per job: FP64 belt 1, FP64 cap 2 (A, B); FP32 twice as many, so
+// that both rings of a pair are transformed together (same bytes)
+__host__ __device__ constexpr int ring16_bufs(int kind, int cs) {
+  return (kind == kRingBelt16 ? 1 : 2) * (cs == 8 ? 2 : 1);
+}
 __host__ __device__ constexpr size_t ring16_bytes(int kind, int P, int cs) {
   return (size_t)kTwSlots * sizeof(double2) +
-         (size_t)(kind == kRingBelt16 ? 1 : 2) * ring16_jobs(kind, P) * job_slots(P) * cs;
+         (size_t)ring16_bufs(kind, cs) * ring16_jobs(kind, P) * job_slots(P) * cs;
 }
 int ring16_lds_slots(int kind, int P) {
   return (int)(ring16_bytes(kind, P, sizeof(double2)) / sizeof(double2));
@@ -207,6 +214,30 @@ __device__ __forceinline__ void belt_pair(const RingPair &rp, const C *__restric
     xn.y = xsh.y = 0.0;
     buf[pad(h)] = xn;
   }
+  if constexpr (sizeof(C) == 8) {
+    // FP32: X^S into the job's second buffer, both rings transformed together
+    C *bufs = buf + job_slots(P);
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) bufs[pad(j + T * rho)] = xs[rho];
+    if (j == 0) bufs[pad(h)] = xsh;
+    __syncthreads();
+    C v[2][16];
+    {
+      const double2 ostep = expipi(2 * T, n);
+      double2 om = expipi(2 * j, n);
+#pragma unroll
+      for (int rho = 0; rho < 16; ++rho) {
+        const int a = j + T * rho;
+        v[0][rho] = zpair(buf[pad(a)], buf[pad(h - a)], om);
+        v[1][rho] = zpair(bufs[pad(a)], bufs[pad(h - a)], om);
+        om = cmul(om, ostep);
+      }
+    }
+    fft16::fft2<P, true>(v, buf, bufs, twl, j);
+    if (act) put16<T>(outn, j, v[0], K);
+    if (act && south) put16<T>(outs, j, v[1], K);
+    return;
+  }
   __syncthreads();
   // 2. northern ring
   belt_fft_out<P>(buf, twl, j, n, act, K, outn);
@@ -248,7 +279,7 @@ __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables 
   constexpr int T = P / 16, J = RB / T, S = job_slots(P);
   int jb, j;
   job_lane<T>(threadIdx.x, &jb, &j);
-  C *buf = lds + jb * S;
+  C *buf = lds + jb * S * ring16_bufs(kRingBelt16, sizeof(C));
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
   if (!act) {
@@ -511,6 +542,74 @@ __device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
   }
 }
 
+// Both cap rings of a pair from their Z entries together (FP32): half 0
+// transforms the A (even) entries of both rings in AN, AS, half 1 the B
+// entries in BN, BS; then the pixels of both rings.
+template <int P, typename C>
+__device__ __forceinline__ void cap_xform2(const RingPair &rp, int K,
+                                           const double2 *__restrict__ chirp,
+                                           const double2 *__restrict__ vhat, const C zn[8],
+                                           const C zs[8], C *AN, C *BN, C *AS, C *BS,
+                                           const double2 *twl, int tt, bool onn, bool ons,
+                                           double *__restrict__ outn,
+                                           double *__restrict__ outs) {
+  using X = cx<C>;
+  constexpr int T = P / 16, T2 = 2 * T;
+  const int n = rp.n, h = n / 2, i = h / 2;
+  const int half = tt / T, j = tt - half * T;
+  C v[2][16];
+#pragma unroll
+  for (int rho = 0; rho < 8; ++rho) {
+    v[0][rho] = zn[rho];
+    v[1][rho] = zs[rho];
+    v[0][8 + rho] = v[1][8 + rho] = X::mk(0.0, 0.0);
+  }
+  C *b0 = half ? BN : AN, *b1 = half ? BS : AS;
+  fft16::fft2<P, false>(v, b0, b1, twl, j);
+  asm volatile("" : "+v"(tt));
+#pragma unroll
+  for (int rho = 0; rho < 16; ++rho) {
+    const C vh = X::of(vhat[j + T * rho]);
+    v[0][rho] = cmul(v[0][rho], vh);
+    v[1][rho] = cmul(v[1][rho], vh);
+  }
+  fft16::fft2<P, true>(v, b0, b1, twl, j);
+  asm volatile("" : "+v"(tt));
+  __syncthreads();  // last-pass reads of the buffers done
+  {
+    const double2 ostep = expipi(4 * T, n);
+    double2 om = half ? expipi(4 * j, n) : make_double2(1.0, 0.0);
+    const double2 st = half ? ostep : make_double2(1.0, 0.0);
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int p = j + T * rho;
+      if (p < i) {
+        const C f = X::of(cmul(chirp[p], om));
+        b0[pad(p)] = cmul(v[0][rho], f);
+        b1[pad(p)] = cmul(v[1][rho], f);
+      }
+      om = cmul(om, st);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rho = 0; rho < 4; ++rho) {
+    const int p = tt + T2 * rho;
+    if (p < i) {
+      if (onn) {
+        const C a = AN[pad(p)], b = BN[pad(p)];
+        put2(outn, p, cadd(a, b), K);
+        put2(outn, p + i, csub(a, b), K);
+      }
+      if (ons) {
+        const C a = AS[pad(p)], b = BS[pad(p)];
+        put2(outs, p, cadd(a, b), K);
+        put2(outs, p + i, csub(a, b), K);
+      }
+    }
+  }
+}
+
 // Block b of a cap (WRAP: wrapping-ring) launch over pairs
 // order[first .. first + count).
 template <int P, bool WRAP, typename C>
@@ -518,9 +617,11 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
                                           int count, double *__restrict__ map, int b,
                                           C *lds, const double2 *twl) {
   constexpr int T = P / 16, J = RB / (2 * T), S = job_slots(P);
+  constexpr bool dual = sizeof(C) == 8;  // FP32: both rings together
   const int jb = threadIdx.x / (2 * T);
   int tt = threadIdx.x % (2 * T);
-  C *bufA = lds + 2 * jb * S, *bufB = bufA + S;
+  C *bufA = lds + ring16_bufs(kRingCap16, sizeof(C)) * jb * S, *bufB = bufA + S;
+  C *bufA2 = bufB + S, *bufB2 = bufA2 + S;  // dual only
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
   if (!act) {
@@ -537,6 +638,21 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
   const double2 *chirp = rt.chirp2 + rp.chirp2_off;  // [i]
   const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
   C zn[8], zs[8];
+  if constexpr (dual) {
+    if constexpr (WRAP) {
+      C ps[16], g0s;
+      wrap_fold2<P>(qe, qo, qs, M, rp.n, rp.s, bufA, tt, act, act && south, ps, &g0s);
+      wrap_fold_south<P>(bufA2, rp.n, rp.s, tt, ps, g0s);
+    } else {
+      cap_fold2<P>(qe, qo, qs, M, rp.n, rp.s, bufA, bufA2, tt, act, act && south);
+    }
+    cap_z<P>(rp, chirp, bufA, tt, zn);
+    cap_z<P>(rp, chirp, bufA2, tt, zs);
+    asm volatile("" : "+v"(tt));
+    cap_xform2<P>(rp, K, chirp, vhat, zn, zs, bufA, bufB, bufA2, bufB2, twl, tt, act,
+                  act && south, map + k + rp.pix_n * K, map + k + (south ? rp.pix_s : 0) * K);
+    return;
+  }
   if constexpr (WRAP) {
     C ps[16], g0s;
     wrap_fold2<P>(qe, qo, qs, M, rp.n, rp.s, bufA, tt, act, act && south, ps, &g0s);

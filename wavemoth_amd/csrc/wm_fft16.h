@@ -83,23 +83,30 @@ __device__ __forceinline__ void dft_r(C *x) {
   }
 }
 
-// One pass (Ns, R) of a P-point transform; buf is this job's LDS buffer,
-// twl the two-level twiddle table, j the thread's index in its job.
-template <int P, int Ns, bool INV, typename C>
-__device__ __forceinline__ void pass(C v[16], C *buf, const double2 *twl, int j) {
+// One pass (Ns, R) of NV independent P-point transforms of the same
+// thread layout: transform t in v[t] with LDS buffer buf[t]; twl the
+// two-level twiddle table, j the thread's index in its job.  The twiddle
+// powers are formed once for all NV transforms.
+template <int P, int Ns, bool INV, int NV, typename C>
+__device__ __forceinline__ void pass(C (&v)[NV][16], C *const (&buf)[NV], const double2 *twl,
+                                     int j) {
   constexpr int T = P / 16;
   constexpr int R = (P / Ns >= 16) ? 16 : P / Ns;
   constexpr int Q = 16 / R;   // butterflies per thread
   constexpr int NB = P / R;   // butterflies per job
-  C x[Q][R];
+  C x[NV][Q][R];
   if constexpr (Ns == 1) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) x[0][r] = v[r];
+    for (int t = 0; t < NV; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[t][0][r] = v[t][r];
   } else {
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
+    for (int t = 0; t < NV; ++t)
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[q][r] = buf[pad(j + T * q + NB * r)];
+      for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[t][q][r] = buf[t][pad(j + T * q + NB * r)];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = (j + T * q) & (Ns - 1);
@@ -108,18 +115,24 @@ __device__ __forceinline__ void pass(C v[16], C *buf, const double2 *twl, int j)
       double2 p = w;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        x[q][r] = cmul(x[q][r], fft::cx<C>::of(p));
+        const C pc = fft::cx<C>::of(p);
+#pragma unroll
+        for (int t = 0; t < NV; ++t) x[t][q][r] = cmul(x[t][q][r], pc);
         if (r + 1 < R) p = cmul(p, w);
       }
     }
   }
 #pragma unroll
-  for (int q = 0; q < Q; ++q) dft_r<R, INV>(x[q]);
+  for (int t = 0; t < NV; ++t)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) dft_r<R, INV>(x[t][q]);
   if constexpr (Ns * R == P) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
+    for (int t = 0; t < NV; ++t)
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q + Q * r] = x[q][r];
+      for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[t][q + Q * r] = x[t][q][r];
   } else {
     __syncthreads();  // every read of buf (this pass or earlier users) done
 #pragma unroll
@@ -128,10 +141,12 @@ __device__ __forceinline__ void pass(C v[16], C *buf, const double2 *twl, int j)
       const int k = jq & (Ns - 1);
       const int d = (jq - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[pad(d + Ns * r)] = x[q][r];
+      for (int t = 0; t < NV; ++t)
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[t][pad(d + Ns * r)] = x[t][q][r];
     }
     __syncthreads();
-    pass<P, Ns * R, INV>(v, buf, twl, j);
+    pass<P, Ns * R, INV, NV>(v, buf, twl, j);
   }
 }
 
@@ -143,7 +158,19 @@ __device__ __forceinline__ void pass(C v[16], C *buf, const double2 *twl, int j)
 template <int P, bool INV, typename C>
 __device__ __forceinline__ void fft(C v[16], C *buf, const double2 *twl, int j) {
   static_assert(P >= 16 && P <= 4096 && (P & (P - 1)) == 0, "P");
-  pass<P, 1, INV>(v, buf, twl, j);
+  C(&vv)[1][16] = *reinterpret_cast<C(*)[1][16]>(v);
+  C *const bb[1] = {buf};
+  pass<P, 1, INV, 1>(vv, bb, twl, j);
+}
+
+// Two independent transforms of the same size and direction together
+// (buffers buf0, buf1): half the barriers of two calls of fft.
+template <int P, bool INV, typename C>
+__device__ __forceinline__ void fft2(C (&v)[2][16], C *buf0, C *buf1, const double2 *twl,
+                                     int j) {
+  static_assert(P >= 16 && P <= 4096 && (P & (P - 1)) == 0, "P");
+  C *const bb[2] = {buf0, buf1};
+  pass<P, 1, INV, 2>(v, bb, twl, j);
 }
 
 }  // namespace fft16
