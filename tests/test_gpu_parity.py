@@ -144,6 +144,20 @@ def test_wrapping_rings_many_maps(oracle, N, L, K):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("N,L,K", [(512, 512, 2), (512, 512, 10)])
+def test_belt_paired_maps(oracle, N, L, K):
+    """Belt rings with FFT length >= 1024 at an even map count: the register
+    kernel's paired jobs exchange lanes so that each store covers two maps
+    (K = 10 also leaves a half-padded group of maps)."""
+    print("case N=%d L=%d K=%d" % (N, L, K), flush=True)
+    plan, alm = _plan(N, L, L, K, seed=31 + K)
+    gpu = plan.execute().copy()
+    ref = oracle.OraclePlan(N, L, L, K).synthesize(alm)
+    err = rel_rms(gpu, ref)
+    print("rel rms", err)
+    assert err <= 1e-12
+
+
 @pytest.mark.parametrize("N,L,M,K", [(16, 32, 32, 10), (8, 24, 12, 5), (32, 60, 60, 8)])
 def test_many_maps_and_mmax(oracle, N, L, M, K):
     print("case N=%d L=%d M=%d K=%d" % (N, L, M, K), flush=True)

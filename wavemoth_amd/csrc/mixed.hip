@@ -291,7 +291,7 @@ struct MixLds {
 template <int S, int CB>
 __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
     LegTables t, const void *bmat, int NCp, int ncb, int bpm, int64_t part_stride,
-    double lam_scale, const unsigned long long *colmax, const double *dtil, const double *etil) {
+    double lam_scale, const unsigned long long *colmax, const double *dtil, const double *etil, const double *gtil) {
   using Sc = MixScheme<S>;
   using Ly = MixLds<S, CB>;
   constexpr int P = Sc::parts;
@@ -350,6 +350,14 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
       const double t3 = fma(x, t2, -(ct_[sl + 2] * t1));
       us = pw ? t1 : sv.y;
       vs = pw ? t3 : t2;
+      if constexpr (S == 3) {
+        // FP16: the producer steps lam_scale Lambda_l itself (normalised x^2
+        // form), so the monic start values take their chunk-relative A_l
+        // (sl is chunk-aligned, so sl .. sl + 3 share sl's chunk)
+        const int64_t j0 = offm - m + sl + pw;
+        us *= lam_scale * t.ascale[j0];
+        vs *= lam_scale * t.ascale[j0 + 2];
+      }
     }
 
     // 16-bit schemes: B fragments of a consumer (its 32 columns, its
@@ -412,8 +420,10 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
       const int k = lane & 31;
       const int64_t j = offm - m + lsc + pw + 2 * k;  // table index of l
       const int64_t jd = j + 2 < t.nlpad + 15 ? j + 2 : t.nlpad + 15;
+      // FP16: alpha_{l+2}, beta_{l+2} (lanes < 32 / >= 32) and gamma_{l+2}
+      // of the normalised step, else d_{l+2}, e_{l+2} of the monic one
       creg0 = lane < 32 ? dtil[jd] : etil[jd];
-      if constexpr (S == 3) creg1 = lam_scale * t.ascale[j < t.nlpad ? j : t.nlpad - 1];
+      if constexpr (S == 3) creg1 = gtil[jd];
       // chunk rescale factors of the 4 chunks (lanes 0 .. 3)
       const int64_t jc = (offm - m + lsc) / kChunk + (lane & 3);
       creg2 = t.cscale[jc < t.nlpad / kChunk ? jc : t.nlpad / kChunk - 1];
@@ -461,16 +471,18 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
           }
 #pragma unroll
           for (int i = 0; i < 8; ++i) {  // l = lc + pw + 2 i
-            if constexpr (S == 3)
-              f[i] = (float)(u * ca[x][i]);  // lam_scale A_l (exact power-of-two scale)
-            else
-              f[i] = (float)u;
-            const double w = fma(x2 - cd[x][i], v, -(ce[x][i] * u));
+            f[i] = (float)u;
+            // FP16: w = (alpha x^2 + beta) v - gamma u on lam_scale Lambda;
+            // else w = (x^2 - d) v - e u on the monic L~
+            const double w = (S == 3) ? fma(fma(cd[x][i], x2, ce[x][i]), v, -(ca[x][i] * u))
+                                      : fma(x2 - cd[x][i], v, -(ce[x][i] * u));
             u = v;
             v = w;
           }
-          u *= cs[x];
-          v *= cs[x];
+          if constexpr (S != 3) {
+            u *= cs[x];
+            v *= cs[x];
+          }
         } else {
 #pragma unroll
           for (int i = 0; i < 8; ++i) f[i] = 0.f;
@@ -489,13 +501,19 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
           for (int i = 0; i < 8; i += 2) {
             if constexpr (S == 3) {
               // packed conversions (round toward zero for the high part; the
-              // low part takes the exact residual, < 1 ulp of the high part)
+              // low part takes the exact residual, < 1 ulp of the high part,
+              // each formed by one mixed-precision FMA f - h * 1.0 reading
+              // its fp16 half of h directly)
               typedef __fp16 h16x2 __attribute__((ext_vector_type(2)));
               const h16x2 h = __builtin_amdgcn_cvt_pkrtz(f[i], f[i + 1]);
-              const h16x2 r =
-                  __builtin_amdgcn_cvt_pkrtz(f[i] - (float)h[0], f[i + 1] - (float)h[1]);
-              wh[i >> 1] = __builtin_bit_cast(unsigned, h);
-              wm[i >> 1] = __builtin_bit_cast(unsigned, r);
+              const unsigned hb = __builtin_bit_cast(unsigned, h);
+              float r0, r1;
+              asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]"
+                  : "=v"(r0) : "v"(hb), "v"(f[i]));
+              asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+                  : "=v"(r1) : "v"(hb), "v"(f[i + 1]));
+              wh[i >> 1] = hb;
+              wm[i >> 1] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(r0, r1));
             } else {
               const __bf16 h0 = (__bf16)f[i], h1 = (__bf16)f[i + 1];
               const float r0 = f[i] - (float)h0, r1 = f[i + 1] - (float)h1;
@@ -765,7 +783,7 @@ template <int S, int CB>
 static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
                                int64_t part_stride, double lam_scale,
                                const unsigned long long *colmax, const double *dtil,
-                               const double *etil, hipStream_t s) {
+                               const double *etil, const double *gtil, hipStream_t s) {
   const int ncb = NCp / kMixCols;
   const int ncg = (ncb + CB - 1) / CB;
   const int bpm = t.W * ncg;
@@ -773,7 +791,7 @@ static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
   const unsigned grid = (unsigned)(8 * mgroups * bpm);
   constexpr size_t lds = MixLds<S, CB>::bytes;
   hipLaunchKernelGGL((k_mix<S, CB>), dim3(grid), dim3(64 * (2 + 2 * CB)), lds, s, t, bmat, NCp,
-                     ncb, bpm, part_stride, lam_scale, colmax, dtil, etil);
+                     ncb, bpm, part_stride, lam_scale, colmax, dtil, etil, gtil);
   return hipGetLastError();
 }
 
@@ -781,13 +799,13 @@ template <int S>
 static hipError_t launch_mix_s(const LegTables &t, const void *bmat, int NCp,
                                int64_t part_stride, double lam_scale,
                                const unsigned long long *colmax, const double *dtil,
-                               const double *etil, hipStream_t s) {
+                               const double *etil, const double *gtil, hipStream_t s) {
   switch (NCp / kMixCols) {
-    case 1: return launch_mix_t<S, 1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
-    case 2: return launch_mix_t<S, 2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
-    case 3: return launch_mix_t<S, 3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 1: return launch_mix_t<S, 1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+    case 2: return launch_mix_t<S, 2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+    case 3: return launch_mix_t<S, 3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
     default:
-      return launch_mix_t<S, kMixCB>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+      return launch_mix_t<S, kMixCB>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
   }
 }
 
@@ -807,13 +825,13 @@ hipError_t mix_set_lds_limits() {
 
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
                       int64_t part_stride, double lam_scale, const unsigned long long *colmax,
-                      const double *dtil, const double *etil, hipStream_t s) {
+                      const double *dtil, const double *etil, const double *gtil, hipStream_t s) {
   switch (scheme) {
-    case 0: return launch_mix_s<0>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
-    case 1: return launch_mix_s<1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
-    case 2: return launch_mix_s<2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+    case 0: return launch_mix_s<0>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+    case 1: return launch_mix_s<1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+    case 2: return launch_mix_s<2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
     default:
-      return launch_mix_s<3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, s);
+      return launch_mix_s<3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
   }
 }
 

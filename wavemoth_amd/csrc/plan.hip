@@ -54,7 +54,9 @@ struct _wavemoth_plan {
   int64_t mix_pstride = 0;  // elements per B part
   void *mix_b = nullptr;
   unsigned long long *mix_colmax = nullptr;  // FP16 scheme: per-column max |a|
-  double *mix_dtil = nullptr, *mix_etil = nullptr;  // x^2-form recurrence coefficients
+  // x^2-form recurrence coefficients: d, e of the monic step (FP32 / BF16
+  // schemes) or alpha, beta, gamma of the normalised one (FP16)
+  double *mix_dtil = nullptr, *mix_etil = nullptr, *mix_gtil = nullptr;
   double mix_lam_scale = 1.0;
   LegCounts cnt;
   int64_t T_exec = 0;
@@ -248,19 +250,39 @@ static bool build_legendre_tables(wavemoth_plan p) {
       std::vector<int64_t> offh(p->M + 1);
       WM_TRY(hipMemcpy(offh.data(), t.off, offh.size() * sizeof(int64_t), hipMemcpyDeviceToHost),
              "off download");
-      std::vector<double> dt(t.nlpad + 16, 0.0), et(t.nlpad + 16, 0.0);
+      std::vector<double> dt(t.nlpad + 16, 0.0), et(t.nlpad + 16, 0.0), gt;
       auto ctl = [](double l, double m) { return (l * l - m * m) / (4.0 * l * l - 1.0); };
+      // A_{l+1} / A_l of Lambda_l = A_l L~_l: r0(l) = sqrt((4(l+1)^2 - 1) / ((l+1)^2 - m^2))
+      auto r0 = [](double l, double m) {
+        return std::sqrt((4.0 * (l + 1.0) * (l + 1.0) - 1.0) / ((l + 1.0) * (l + 1.0) - m * m));
+      };
+      const bool norm = p->mix == 3;
+      if (norm) gt.assign(t.nlpad + 16, 0.0);
       for (int m = 0; m <= p->M; ++m) {
         const int64_t lp = (p->L - m + kChunk) / kChunk * kChunk;
         const double dm = (double)m;
         for (int64_t jj = 0; jj < lp; ++jj) {
           const double l = dm + (double)jj;
-          dt[offh[m] + jj] = ctl(l, dm) + ctl(l + 1.0, dm);
-          et[offh[m] + jj] = (jj == 0) ? 0.0 : ctl(l, dm) * ctl(l - 1.0, dm);
+          const double d = ctl(l, dm) + ctl(l + 1.0, dm);
+          const double e = (jj == 0) ? 0.0 : ctl(l, dm) * ctl(l - 1.0, dm);
+          if (!norm) {
+            dt[offh[m] + jj] = d;
+            et[offh[m] + jj] = e;
+          } else {
+            // Lambda_{l+2} = (alpha x^2 + beta) Lambda_l - gamma Lambda_{l-2}:
+            // alpha = A_{l+2}/A_l, beta = -alpha d, gamma = e A_{l+2}/A_{l-2}
+            // (e = 0 for l < m + 2, where A_{l-2} is undefined)
+            const double alpha = r0(l + 1.0, dm) * r0(l, dm);
+            dt[offh[m] + jj] = alpha;
+            et[offh[m] + jj] = -alpha * d;
+            gt[offh[m] + jj] =
+                (jj < 2) ? 0.0 : e * alpha * r0(l - 1.0, dm) * r0(l - 2.0, dm);
+          }
         }
       }
       if (!upload(p->arena, &p->mix_dtil, dt, "mixed d table")) return false;
       if (!upload(p->arena, &p->mix_etil, et, "mixed e table")) return false;
+      if (norm && !upload(p->arena, &p->mix_gtil, gt, "mixed gamma table")) return false;
     }
     p->mix_lam_scale = mix_lambda_scale(p->L);
     WM_TRY(mix_set_lds_limits(), "mixed kernel LDS limit");
@@ -539,7 +561,7 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
            "mixed prescale kernel");
     if (!mark(p, 1, s)) return false;
     WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, p->mix_lam_scale,
-                      p->mix_colmax, p->mix_dtil, p->mix_etil, s),
+                      p->mix_colmax, p->mix_dtil, p->mix_etil, p->mix_gtil, s),
            "mixed contraction kernel");
   } else {
     WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");

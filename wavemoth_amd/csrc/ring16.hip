@@ -120,6 +120,17 @@ __device__ __forceinline__ void put2(double *out, int p, C z, int K) {
   }
 }
 
+// Lanes 32 .. 63 of x trade places with lanes 0 .. 31 of y.
+__device__ __forceinline__ void swap_halves(double &x, double &y) {
+  const unsigned long long xb = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long yb = __builtin_bit_cast(unsigned long long, y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xb, (unsigned)yb, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane32_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+  x = __builtin_bit_cast(double, (unsigned long long)lo[0] | (unsigned long long)hi[0] << 32);
+  y = __builtin_bit_cast(double, (unsigned long long)lo[1] | (unsigned long long)hi[1] << 32);
+}
+
 // pixels 2p, 2p + 1 (p = j + T rho) of the 16 values of one thread
 template <int T, typename C>
 __device__ __forceinline__ void put16(double *out, int j, const C v[16], int K) {
@@ -127,6 +138,22 @@ __device__ __forceinline__ void put16(double *out, int j, const C v[16], int K) 
     double2 *o = reinterpret_cast<double2 *>(out) + j;
 #pragma unroll
     for (int rho = 0; rho < 16; ++rho) o[T * rho] = to_d(v[rho]);
+  } else if (T >= 64 && (K & 1) == 0) {
+    // paired jobs (job_lane): lane l and lane l + 32 hold the same pixels of
+    // maps k (even) and k + 1 (out = map + k resp. map + k + 1); after the
+    // exchange lane l holds pixel 2p of both maps and lane l + 32 pixel
+    // 2p + 1, one 16-byte store each.  Every lane of the wave takes part
+    // (both jobs of a pair are live or padding together when K is even).
+    const int hi = (threadIdx.x >> 5) & 1;
+    double *o = out - hi + (int64_t)(2 * j + hi) * K;
+    const int64_t st = (int64_t)(2 * T) * K;
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      double x = v[rho].x, y = v[rho].y;
+      swap_halves(x, y);
+      *reinterpret_cast<double2 *>(o) = make_double2(x, y);
+      o += st;
+    }
   } else {
     double *o = out + (int64_t)(2 * j) * K;
     const int64_t st = (int64_t)(2 * T) * K;

@@ -181,7 +181,8 @@ hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, i
 // l-grid of ctil, with at least 4 trailing entries past nlpad.
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
                       int64_t part_stride, double lam_scale, const unsigned long long *colmax,
-                      const double *dtil, const double *etil, hipStream_t s);
+                      const double *dtil, const double *etil, const double *gtil,
+                      hipStream_t s);
 double mix_lambda_scale(int lmax);
 
 // Shape of the Legendre sweep for nmaps = K.
