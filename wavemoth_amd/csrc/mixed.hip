@@ -91,49 +91,6 @@ struct MixScheme<3> {
 
 int mix_parts(int scheme) { return scheme == 0 ? 1 : (scheme == 2 ? 3 : 2); }
 
-// Scale of a_lm column c for the FP16 scheme: 2^(13 - floor(log2 max|a|)),
-// so the column's largest value lands in [2^13, 2^14).
-__device__ __forceinline__ double col_scale(const unsigned long long *colmax, int c) {
-  const double mx = __builtin_bit_cast(double, colmax[c]);
-  return mx > 0.0 ? ldexp(1.0, 13 - ilogb(mx)) : 1.0;
-}
-
-// Per-column max |a| (bit patterns of non-negative doubles order as
-// integers): block-level max in LDS, then one global atomic per column.
-__global__ __launch_bounds__(256) void k_mix_colmax_clear(unsigned long long *colmax, int NCp) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < NCp) colmax[c] = 0ull;
-}
-
-// Block b covers rows [r0, r1) of the (nrows, C2) array; thread tid owns
-// column cb + tid % cols of every RP-th row and keeps its max in a register.
-__global__ __launch_bounds__(256) void k_mix_colmax(const double *__restrict__ alm,
-                                                    int64_t nrows, int C2,
-                                                    unsigned long long *colmax) {
-  extern __shared__ unsigned long long smax[];
-  for (int c = threadIdx.x; c < C2; c += blockDim.x) smax[c] = 0ull;
-  __syncthreads();
-  const int64_t per = (nrows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = blockIdx.x * per;
-  const int64_t r1 = r0 + per < nrows ? r0 + per : nrows;
-  const int cols = C2 < 256 ? C2 : 256, RP = 256 / cols;
-  const int rr = threadIdx.x / cols, cc = threadIdx.x - rr * cols;
-  for (int cb = 0; cb < C2; cb += cols) {
-    const int c = cb + cc;
-    if (rr < RP && c < C2) {
-      unsigned long long mx = 0ull;
-      for (int64_t r = r0 + rr; r < r1; r += RP) {
-        const unsigned long long v = __builtin_bit_cast(unsigned long long, fabs(alm[r * C2 + c]));
-        mx = v > mx ? v : mx;
-      }
-      if (mx) atomicMax(&smax[c], mx);
-    }
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c < C2; c += blockDim.x)
-    if (smax[c]) atomicMax(&colmax[c], smax[c]);
-}
-
 // ------------------------------------------------------------- prescale
 
 // B operand of chunk c (16 l from m + 16 (c - off[m]/16)), parity p, the
@@ -208,12 +165,11 @@ __global__ __launch_bounds__(256) void k_mix_prep_f32(LegTables t, const double2
 // 16-bit parts: thread u -> (chunk c = blockIdx.x * CPB + u / (2 NK), parity
 // p, map k) reads the 8 a_lm of (c, p, k) and writes 16 bytes per part and
 // column.
+// BF16 schemes (S = 1, 2).
 template <int S>
 __global__ __launch_bounds__(256) void k_mix_prep_16(LegTables t, const double2 *__restrict__ alm,
                                                      unsigned short *__restrict__ bq, int NCp,
-                                                     int64_t part_stride,
-                                                     const unsigned long long *colmax,
-                                                     int64_t nchunks) {
+                                                     int64_t part_stride, int64_t nchunks) {
   const int NK = NCp / 2, per = 2 * NK;
   const int CPB = blockDim.x / per > 0 ? blockDim.x / per : 1;
   for (int u = threadIdx.x; u < CPB * per; u += blockDim.x) {
@@ -222,14 +178,11 @@ __global__ __launch_bounds__(256) void k_mix_prep_16(LegTables t, const double2 
     const int64_t c = (int64_t)blockIdx.x * CPB + cl;
     if (c >= nchunks) continue;
     const int m = t.chunk_m[c];
-    const int kc = k < t.K ? k : 0;
-    const double csr = S == 3 ? col_scale(colmax, 2 * kc) : 1.0;
-    const double csi = S == 3 ? col_scale(colmax, 2 * kc + 1) : 1.0;
     unsigned short hr[3][8], hi[3][8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       double vr, vi;
-      mix_prep_pair(t, alm, c * kChunk + 2 * i + p, m, k, S, csr, csi, &vr, &vi);
+      mix_prep_pair(t, alm, c * kChunk + 2 * i + p, m, k, S, 1.0, 1.0, &vr, &vi);
       mix_split16(vr, S, &hr[0][i], &hr[1][i], &hr[2][i]);
       mix_split16(vi, S, &hi[0][i], &hi[1][i], &hi[2][i]);
     }
@@ -237,6 +190,81 @@ __global__ __launch_bounds__(256) void k_mix_prep_16(LegTables t, const double2 
     const int64_t at = ((c * 2 + p) * NCp + 2 * k) * 8;
 #pragma unroll
     for (int q = 0; q < nparts; ++q) {
+      uint4 wr, wi;
+      wr.x = hr[q][0] | (unsigned)hr[q][1] << 16;
+      wr.y = hr[q][2] | (unsigned)hr[q][3] << 16;
+      wr.z = hr[q][4] | (unsigned)hr[q][5] << 16;
+      wr.w = hr[q][6] | (unsigned)hr[q][7] << 16;
+      wi.x = hi[q][0] | (unsigned)hi[q][1] << 16;
+      wi.y = hi[q][2] | (unsigned)hi[q][3] << 16;
+      wi.z = hi[q][4] | (unsigned)hi[q][5] << 16;
+      wi.w = hi[q][6] | (unsigned)hi[q][7] << 16;
+      uint4 *o = reinterpret_cast<uint4 *>(bq + q * part_stride + at);
+      o[0] = wr;
+      o[1] = wi;
+    }
+  }
+}
+
+// FP16: one block per m.  The block first finds max |a_lm| of each column
+// (map, Re/Im) over m's l-range, picks the power of two 2^e that puts it in
+// [2^13, 2^14) (fp16's range with headroom; e = 0 for an all-zero column),
+// records e in cexp[m][col] for the contraction to undo, and writes the
+// scaled a_lm's two fp16 parts for m's chunks as k_mix_prep_16 does.  The
+// second read of m's rows mostly hits the L2.
+__global__ __launch_bounds__(256) void k_mix_prep_f16(LegTables t, const double2 *__restrict__ alm,
+                                                      unsigned short *__restrict__ bq, int NCp,
+                                                      int64_t part_stride, int *__restrict__ cexp) {
+  extern __shared__ unsigned long long smx[];  // [NCp]: max |a| bits, then the scale
+  const int m = blockIdx.x, K = t.K, L = t.L, NK = NCp / 2;
+  const int nl = L - m + 1;
+  const int64_t row0 = (int64_t)m * (2 * L - m + 3) / 2;  // a_lm row of (l = m, m)
+  for (int c = threadIdx.x; c < NCp; c += blockDim.x) smx[c] = 0ull;
+  __syncthreads();
+  const int cols = NK < 256 ? NK : 256, RP = 256 / cols;
+  const int rr = threadIdx.x / cols, cc = threadIdx.x - rr * cols;
+  for (int kb = 0; kb < NK; kb += cols) {
+    const int k = kb + cc;
+    if (rr < RP && k < K) {
+      unsigned long long mr = 0ull, mi = 0ull;
+      for (int l = rr; l < nl; l += RP) {
+        const double2 a = alm[(row0 + l) * K + k];
+        const unsigned long long br = __builtin_bit_cast(unsigned long long, fabs(a.x));
+        const unsigned long long bi = __builtin_bit_cast(unsigned long long, fabs(a.y));
+        mr = br > mr ? br : mr;
+        mi = bi > mi ? bi : mi;
+      }
+      if (mr) atomicMax(&smx[2 * k], mr);
+      if (mi) atomicMax(&smx[2 * k + 1], mi);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < NCp; c += blockDim.x) {
+    const double mx = __builtin_bit_cast(double, smx[c]);
+    const int e = mx > 0.0 ? 13 - ilogb(mx) : 0;
+    cexp[(int64_t)m * NCp + c] = e;
+    smx[c] = __builtin_bit_cast(unsigned long long, ldexp(1.0, e));
+  }
+  __syncthreads();
+  const int nch = (L - m + kChunk) / kChunk, per = 2 * NK;
+  const int64_t c0 = t.off[m] / kChunk;
+  for (int u = threadIdx.x; u < nch * per; u += blockDim.x) {
+    const int cl = u / per, r = u - cl * per;
+    const int p = r / NK, k = r - p * NK;
+    const int64_t c = c0 + cl;
+    const double csr = __builtin_bit_cast(double, smx[2 * k]);
+    const double csi = __builtin_bit_cast(double, smx[2 * k + 1]);
+    unsigned short hr[3][8], hi[3][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      double vr, vi;
+      mix_prep_pair(t, alm, c * kChunk + 2 * i + p, m, k, 3, csr, csi, &vr, &vi);
+      mix_split16(vr, 3, &hr[0][i], &hr[1][i], &hr[2][i]);
+      mix_split16(vi, 3, &hi[0][i], &hi[1][i], &hi[2][i]);
+    }
+    const int64_t at = ((c * 2 + p) * NCp + 2 * k) * 8;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
       uint4 wr, wi;
       wr.x = hr[q][0] | (unsigned)hr[q][1] << 16;
       wr.y = hr[q][2] | (unsigned)hr[q][3] << 16;
@@ -291,7 +319,7 @@ struct MixLds {
 template <int S, int CB>
 __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
     LegTables t, const void *bmat, int NCp, int ncb, int bpm, int64_t part_stride,
-    double lam_scale, const unsigned long long *colmax, const double *dtil, const double *etil, const double *gtil) {
+    double lam_scale, const int *cexp, const double *dtil, const double *etil, const double *gtil) {
   using Sc = MixScheme<S>;
   using Ly = MixLds<S, CB>;
   constexpr int P = Sc::parts;
@@ -546,7 +574,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
           const int col = cb * kMixCols + 16 * ct + i16;
-          unscale[ct] = 1.0 / (lam_scale * col_scale(colmax, col < NCp ? col : 0));
+          unscale[ct] = ldexp(1.0 / lam_scale, -cexp[(int64_t)m * NCp + (col < NCp ? col : 0)]);
         }
       }
     }
@@ -746,35 +774,25 @@ int64_t mix_bmat_elems(const LegTables &t, int NCp) {
 }
 
 hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, int NCp,
-                           int scheme, int64_t part_stride, unsigned long long *colmax,
-                           hipStream_t s) {
-  if (scheme == 3) {
-    hipLaunchKernelGGL(k_mix_colmax_clear, dim3((NCp + 255) / 256), dim3(256), 0, s, colmax,
-                       NCp);
-    const int64_t nalm = (int64_t)(t.M + 1) * (t.L + 1) - (int64_t)t.M * (t.M + 1) / 2;
-    const int C2 = 2 * t.K;
-    hipLaunchKernelGGL(k_mix_colmax, dim3(1024), dim3(256), (size_t)C2 * 8, s,
-                       (const double *)alm, nalm, C2, colmax);
-  }
+                           int scheme, int64_t part_stride, int *cexp, hipStream_t s) {
   const int64_t nchunks = t.nlpad / kChunk;
+  unsigned short *bq = (unsigned short *)bmat;
   if (scheme == 0) {
     hipLaunchKernelGGL(k_mix_prep_f32, dim3((unsigned)nchunks), dim3(256), 0, s, t, alm,
                        (float *)bmat, NCp);
+  } else if (scheme == 3) {
+    hipLaunchKernelGGL(k_mix_prep_f16, dim3(t.M + 1), dim3(256), (size_t)NCp * 8, s, t, alm, bq,
+                       NCp, part_stride, cexp);
   } else {
     const int per = NCp;  // (parity, map) threads per chunk
     const int cpb = 256 / per > 0 ? 256 / per : 1;
     const dim3 grid((unsigned)((nchunks + cpb - 1) / cpb));
-    const auto cm = (const unsigned long long *)colmax;
-    unsigned short *bq = (unsigned short *)bmat;
     if (scheme == 1)
       hipLaunchKernelGGL(k_mix_prep_16<1>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
-                         cm, nchunks);
-    else if (scheme == 2)
-      hipLaunchKernelGGL(k_mix_prep_16<2>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
-                         cm, nchunks);
+                         nchunks);
     else
-      hipLaunchKernelGGL(k_mix_prep_16<3>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
-                         cm, nchunks);
+      hipLaunchKernelGGL(k_mix_prep_16<2>, grid, dim3(256), 0, s, t, alm, bq, NCp, part_stride,
+                         nchunks);
   }
   return hipGetLastError();
 }
@@ -782,7 +800,7 @@ hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, i
 template <int S, int CB>
 static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
                                int64_t part_stride, double lam_scale,
-                               const unsigned long long *colmax, const double *dtil,
+                               const int *cexp, const double *dtil,
                                const double *etil, const double *gtil, hipStream_t s) {
   const int ncb = NCp / kMixCols;
   const int ncg = (ncb + CB - 1) / CB;
@@ -791,26 +809,28 @@ static hipError_t launch_mix_t(const LegTables &t, const void *bmat, int NCp,
   const unsigned grid = (unsigned)(8 * mgroups * bpm);
   constexpr size_t lds = MixLds<S, CB>::bytes;
   hipLaunchKernelGGL((k_mix<S, CB>), dim3(grid), dim3(64 * (2 + 2 * CB)), lds, s, t, bmat, NCp,
-                     ncb, bpm, part_stride, lam_scale, colmax, dtil, etil, gtil);
+                     ncb, bpm, part_stride, lam_scale, cexp, dtil, etil, gtil);
   return hipGetLastError();
 }
 
 template <int S>
 static hipError_t launch_mix_s(const LegTables &t, const void *bmat, int NCp,
                                int64_t part_stride, double lam_scale,
-                               const unsigned long long *colmax, const double *dtil,
+                               const int *cexp, const double *dtil,
                                const double *etil, const double *gtil, hipStream_t s) {
   switch (NCp / kMixCols) {
-    case 1: return launch_mix_t<S, 1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
-    case 2: return launch_mix_t<S, 2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
-    case 3: return launch_mix_t<S, 3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+    case 1: return launch_mix_t<S, 1>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
+    case 2: return launch_mix_t<S, 2>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
+    case 3: return launch_mix_t<S, 3>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
     default:
-      return launch_mix_t<S, kMixCB>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+      return launch_mix_t<S, kMixCB>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
   }
 }
 
 hipError_t mix_set_lds_limits() {
-  hipError_t e = hipSuccess;
+  // the FP16 prescale's column maxima: NCp <= 8192 (nmaps <= 4096)
+  hipError_t e = hipFuncSetAttribute((const void *)k_mix_prep_f16,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8);
 #define WM_SET(S, CB)                                                                      \
   if (e == hipSuccess)                                                                     \
     e = hipFuncSetAttribute((const void *)k_mix<S, CB>,                                    \
@@ -824,14 +844,14 @@ hipError_t mix_set_lds_limits() {
 }
 
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
-                      int64_t part_stride, double lam_scale, const unsigned long long *colmax,
+                      int64_t part_stride, double lam_scale, const int *cexp,
                       const double *dtil, const double *etil, const double *gtil, hipStream_t s) {
   switch (scheme) {
-    case 0: return launch_mix_s<0>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
-    case 1: return launch_mix_s<1>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
-    case 2: return launch_mix_s<2>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+    case 0: return launch_mix_s<0>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
+    case 1: return launch_mix_s<1>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
+    case 2: return launch_mix_s<2>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
     default:
-      return launch_mix_s<3>(t, bmat, NCp, part_stride, lam_scale, colmax, dtil, etil, gtil, s);
+      return launch_mix_s<3>(t, bmat, NCp, part_stride, lam_scale, cexp, dtil, etil, gtil, s);
   }
 }
 

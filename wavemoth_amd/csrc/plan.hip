@@ -53,7 +53,7 @@ struct _wavemoth_plan {
   int mix_ncp = 0;          // padded real columns
   int64_t mix_pstride = 0;  // elements per B part
   void *mix_b = nullptr;
-  unsigned long long *mix_colmax = nullptr;  // FP16 scheme: per-column max |a|
+  int *mix_cexp = nullptr;  // FP16 scheme: a_lm scale exponents per (m, column)
   // x^2-form recurrence coefficients: d, e of the monic step (FP32 / BF16
   // schemes) or alpha, beta, gamma of the normalised one (FP16)
   double *mix_dtil = nullptr, *mix_etil = nullptr, *mix_gtil = nullptr;
@@ -244,7 +244,8 @@ static bool build_legendre_tables(wavemoth_plan p) {
     WM_TRY(hipMemsetAsync(b, 0, bytes, p->stream), "mixed B clear");
     WM_TRY(hipStreamSynchronize(p->stream), "mixed B clear");
     p->mix_b = b;
-    if (!dalloc(p->arena, &p->mix_colmax, (size_t)p->mix_ncp, "column maxima")) return false;
+    if (!dalloc(p->arena, &p->mix_cexp, (size_t)(p->M + 1) * p->mix_ncp, "scale exponents"))
+      return false;
     {
       // d_l = c~_l + c~_{l+1}, e_l = c~_l c~_{l-1}, c~_l = (l^2 - m^2)/(4 l^2 - 1)
       std::vector<int64_t> offh(p->M + 1);
@@ -557,11 +558,11 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
   }
   if (p->mix >= 0) {
     WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride,
-                           p->mix_colmax, s),
+                           p->mix_cexp, s),
            "mixed prescale kernel");
     if (!mark(p, 1, s)) return false;
     WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, p->mix_lam_scale,
-                      p->mix_colmax, p->mix_dtil, p->mix_etil, p->mix_gtil, s),
+                      p->mix_cexp, p->mix_dtil, p->mix_etil, p->mix_gtil, s),
            "mixed contraction kernel");
   } else {
     WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");

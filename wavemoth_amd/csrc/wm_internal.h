@@ -173,14 +173,16 @@ int mix_cols_per_block();
 int mix_rings_per_lane_table();
 int64_t mix_bmat_elems(const LegTables &t, int NCp);
 hipError_t mix_set_lds_limits();
-// colmax: [NCp] scratch of the FP16 scheme (scheme 3), unused otherwise.
+// cexp: [(M+1) NCp] power-of-two exponents of the FP16 scheme's a_lm
+// scales per (m, column), written by the prescale (scheme 3), unused
+// otherwise.
 hipError_t launch_mix_prep(const LegTables &t, const double2 *alm, void *bmat, int NCp,
-                           int scheme, int64_t part_stride, unsigned long long *colmax,
-                           hipStream_t s);
+                           int scheme, int64_t part_stride, int *cexp, hipStream_t s);
 // dtil / etil: d_l = c~_l + c~_{l+1}, e_l = c~_l c~_{l-1} on the padded
-// l-grid of ctil, with at least 4 trailing entries past nlpad.
+// l-grid of ctil (FP16: alpha, beta, and gamma in gtil of the normalised
+// step), with at least 4 trailing entries past nlpad.
 hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
-                      int64_t part_stride, double lam_scale, const unsigned long long *colmax,
+                      int64_t part_stride, double lam_scale, const int *cexp,
                       const double *dtil, const double *etil, const double *gtil,
                       hipStream_t s);
 double mix_lambda_scale(int lmax);
