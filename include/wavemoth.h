@@ -52,7 +52,9 @@ typedef struct _wavemoth_plan *wavemoth_plan;
                             three products (hi*hi + hi*lo + lo*hi);
      WAVEMOTH_MIXED_BF16X6  three bf16 parts each, six products;
      WAVEMOTH_MIXED_F16X3   two fp16 parts each, three products, with
-                            power-of-two scales keeping fp16's range. */
+                            power-of-two scales keeping fp16's range.
+   These plans keep the Legendre-stage output in FP32 and run the ring FFTs
+   in FP32 (the map is FP64); wavemoth_get_work widens it to complex128. */
 #define WAVEMOTH_MIXED_F32 0x1000
 #define WAVEMOTH_MIXED_BF16X3 0x2000
 #define WAVEMOTH_MIXED_BF16X6 0x4000
