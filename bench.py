@@ -16,6 +16,10 @@ Workloads (--workload):
          through the library's device-pointer execute.  Nside = 2048,
          lmax = mmax = 4096, nmaps = 1, seeded power-law a_lm
          (C_l = (l+1)^-2).
+  grad   one step = one forward synthesis and one adjoint (the gradient of
+         the weighted least-squares loss of the differentiable-synthesis
+         configuration: Nside 512, lmax = mmax 1024, nmaps 3, weights 0 for
+         |z| < 0.2 else uniform in [0.5, 2]), FP64, device buffers.
   batch  one step = one execute of a batch of 96 uneven requests (Nside
          uniform over {128 .. 2048}, lmax = mmax = 2 Nside, nmaps over
          {1, 3, 8}, seeded), FP64, replayed from the batch's HIP graph.
@@ -49,7 +53,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["fp64", "batch", "mixed"], default="mixed")
+    ap.add_argument("--workload", choices=["fp64", "batch", "mixed", "grad"], default="mixed")
     ap.add_argument("--mixed-nmaps", type=int, default=64)
     ap.add_argument("--schemes", default="f32,f16x3,bf16x3,bf16x6")
     ap.add_argument("--fp64-steps", type=int, default=3,
@@ -241,6 +245,85 @@ def run_fp64(args, dev, res):
     })
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(N, L, K, alm_h, args.cpu_m_stride)
+
+
+def run_grad(args, dev, res):
+    """Forward + adjoint of the differentiable-synthesis configuration.  The
+    adjoint's Legendre kernel does the forward sweep's flops, so it is priced
+    with the same (5 + 4 nmaps) convention."""
+    import numpy as np
+    import torch
+
+    from wavemoth_amd.alm import power_law_alm
+    from wavemoth_amd.healpix import get_ring_offsets
+    from wavemoth_amd.torch_sht import DevicePlan
+
+    N, L, K = 512, 1024, 3
+    npix = 12 * N * N
+    alm = torch.from_numpy(power_law_alm(L, L, K, args.seed)).to(dev)
+    d = torch.from_numpy(power_law_alm(L, L, K, args.seed + 1)).to(dev)
+    plan = DevicePlan(N, L, L, K, 0)
+    info = plan.info()
+    data = torch.empty((npix, K), dtype=torch.float64, device=dev)
+    plan.synthesize_into(d, data)
+    g = torch.Generator(device="cpu").manual_seed(args.seed)
+    data += torch.randn((npix, K), generator=g, dtype=torch.float64).to(dev)
+    # weights: 0 for |z| < 0.2, else uniform in [0.5, 2] (per pixel)
+    offs = get_ring_offsets(N)
+    z = np.empty(npix)
+    for i in range(4 * N - 1):
+        ii = i + 1
+        zi = 1 - ii * ii / (3.0 * N * N) if ii < N else (
+            4.0 / 3 - 2.0 * ii / (3 * N) if ii <= 3 * N else -(1 - (4 * N - ii) ** 2 / (3.0 * N * N)))
+        z[offs[i]:offs[i + 1]] = zi
+    w = torch.from_numpy(np.where(np.abs(z) < 0.2, 0.0,
+                                  0.5 + 1.5 * np.random.default_rng(args.seed).random(npix)))
+    w = w.to(dev).unsqueeze(1)
+    out = torch.empty((npix, K), dtype=torch.float64, device=dev)
+    grad = torch.empty_like(alm)
+
+    def step():
+        plan.synthesize_into(alm, out)
+        r = w * (out - data)
+        plan.adjoint_into(r, grad)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd = adj = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev[0].record()
+        plan.synthesize_into(alm, out)
+        ev[1].record()
+        r = w * (out - data)
+        plan.adjoint_into(r, grad)
+        ev[2].record()
+        ev[2].synchronize()
+        fwd += ev[0].elapsed_time(ev[1])
+        adj += ev[1].elapsed_time(ev[2])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    T_eps = info["triples_eps"]
+    adj_ms = adj / args.steps
+    res.update({
+        "metric": "maps/s differentiable synthesis (forward + adjoint) Nside=%d lmax=%d "
+                  "nmaps=%d: FP64" % (N, L, K),
+        "value": K * args.steps / (t1 - t0),
+        "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+        "dtype": "f64",
+        "data": "synthetic (seeded power-law a_lm and data, N(0,1) noise, the configuration's "
+                "weights)",
+        "config": {"workload": "gradient of the weighted least-squares loss: synthesis + "
+                               "adjoint (unweighted analysis)",
+                   "nside": N, "lmax": L, "mmax": L, "nmaps": K, "npix": npix},
+        "roofline": roofline((5 + 4 * K) * T_eps, adj_ms, "adjoint (k_ring_adjoint + "
+                             "k_legendre_adj + k_adj_finish)", {
+            "note": "whole adjoint execute timed with events (its kernels together)"}),
+        "stages_ms": {"forward": fwd / args.steps, "residual_and_adjoint": adj_ms},
+        "grad_finite": bool(torch.isfinite(grad).all().item()),
+    })
 
 
 def run_batch(args, dev, res):
@@ -454,6 +537,8 @@ def main():
         run_fp64(args, dev, res)
     elif args.workload == "mixed":
         run_mixed(args, dev, res)
+    elif args.workload == "grad":
+        run_grad(args, dev, res)
     else:
         run_batch(args, dev, res)
     res.setdefault("cpu_baseline", None)

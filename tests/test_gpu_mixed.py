@@ -96,3 +96,23 @@ def test_fp16_scheme_scale_invariant(amp):
     rms, mx = _err(got, ref)
     print("amp %g rel rms %.3e" % (amp, rms), flush=True)
     assert rms <= 1e-6
+
+
+@pytest.mark.parametrize("scheme", [F32, F16])
+def test_mixed_get_work(scheme):
+    """A mixed plan keeps the Legendre-stage output in FP32; get_work widens
+    it into the reference's complex128 layout, which matches the FP64 plan's
+    to the scheme's accuracy."""
+    import wavemoth_amd as wm
+    N, L, K = 32, 64, 16
+    alm = wm.power_law_alm(L, L, K, 17)
+    works = []
+    for flags in (0, scheme):
+        p = wm.ShtPlan(N, L, L, alm, np.zeros((12 * N * N, K)), 'mmajor', flags=flags)
+        p.perform_legendre_transform()
+        works.append(p.get_work())
+    ref, got = works
+    assert got.dtype == np.complex128 and got.shape == ref.shape
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print("%s work rel rms %.3e" % (NAME[scheme], err), flush=True)
+    assert err <= 1e-6
