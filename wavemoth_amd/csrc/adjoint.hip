@@ -18,7 +18,7 @@
 //     F_south (the equator ring feeds both), into the [R][2][M+1][K] buffer.
 //  2. k_legendre_adj: b~_l = sum_rings L~_l(x_r) q_{parity(l-m)}(r) with the
 //     same monic recurrence and seeds as the synthesis; lanes evaluate rings,
-//     a 64 x 64 LDS tile transposes them so lanes then own l.
+//     a 16 x 64 LDS tile per wave transposes them so lanes then own l.
 //  3. k_adj_finish: b_lm = c_m A_l sum_groups b~ into the m-major layout.
 #include <mutex>
 
@@ -141,18 +141,39 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
   }
 }
 
-// b~ partial sums.  One 64-lane block per (m, ring group g); the group's
-// ring chunks c = g, g + G, ... (64 rings each) run one after the other,
-// and for each the l-range runs in 64-l windows: lanes evaluate their ring
-// (monic recurrence, 16-l chunk rescale), a padded 64 x 65 LDS tile
-// transposes, lanes then own l and sum over the 64 rings, accumulating into
-// partial[g][off_m + l - m][k] (this block's row only).
-__global__ __launch_bounds__(64) void k_legendre_adj(LegTables t, int G, double2 *partial) {
-  __shared__ double tile[64 * 65];
-  __shared__ double2 qs[2 * 64 * 8];
-  const int lane = threadIdx.x;
-  const int m = blockIdx.x / G, g = blockIdx.x - (blockIdx.x / G) * G;
+// b~ partial sums.  Each wave is one work item (m, ring group g) and runs
+// alone (no workgroup barrier; 4 waves per workgroup only share the launch):
+// the group's 64-ring chunks c = g, g + G, ... one after the other, for each
+// the l-range in 16-l windows.  Lanes evaluate their ring (monic
+// recurrence, 16-l chunk rescale) into the wave's 16 x 65 LDS tile; then
+// lane (li, rs) = (lane mod 16, lane / 16) sums rings 16 rs .. 16 rs + 15 of
+// l = window + li against the chunk's q (staged once per chunk and map
+// block in the wave's LDS), and two butterfly shuffles add the four ring
+// quarters.  Rows partial[g][off_m + l - m][k] belong to this wave only.
+constexpr int kAdjWaves = 4;  // waves per workgroup of k_legendre_adj
+
+__host__ __device__ inline int adj_wave_lds_doubles(int K) {
+  const int KB = K < 8 ? K : 8;
+  return 16 * 65 + 2 * 64 * KB * 2;  // tile + q (double2) of one wave
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, int G,
+                                                                 double2 *partial) {
+  extern __shared__ double adj_lds[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int L = t.L, R = t.R, M = t.M, K = t.K;
+  const int KB = K < 8 ? K : 8;
+  const int64_t item = (int64_t)blockIdx.x * kAdjWaves + wv;
+  const int m = (int)(item / G), g = (int)(item - (int64_t)(item / G) * G);
+  if (m > M) return;  // the whole wave
+  double *tile = adj_lds + (size_t)wv * adj_wave_lds_doubles(K);  // [16][65]
+  double2 *qs = (double2 *)(tile + 16 * 65);                       // [2][64][KB]
   const int64_t offm = t.off[m];
   const int len = L - m + 1;
   const int lp = (len + kChunk - 1) / kChunk * kChunk;
@@ -160,84 +181,87 @@ __global__ __launch_bounds__(64) void k_legendre_adj(LegTables t, int G, double2
   const double *cs = t.cscale + offm / kChunk;
   double2 *prow = partial + ((int64_t)g * t.nlpad + offm) * K;
   for (int j = lane; j < lp * K; j += 64) prow[j] = make_double2(0.0, 0.0);
-  __syncthreads();
+  const int li = lane & 15, rs = lane >> 4;
   const int nchunks = (R + 63) / 64;
-  for (int c = g; c < nchunks; c += G) {
-    const int r = c * 64 + lane;
-    const bool ok = r < R;
-    const double x = ok ? t.ring_x[r] : 0.0;
-    const int sl = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
-    int lmin = sl;
-    for (int d = 32; d >= 1; d >>= 1) {
-      const int o = __shfl_xor(lmin, d);
-      lmin = o < lmin ? o : lmin;
-    }
-    lmin = __builtin_amdgcn_readfirstlane(lmin);
-    if (lmin > L) continue;
-    double prev = 0.0, cur = 0.0;
-    // windows of 64 l on the grid l = m + 64 w; seeds sit on the 16-grid
-    const int w0 = (lmin - m) / 64;
-    for (int wl = m + 64 * w0; wl <= L; wl += 64) {
-      for (int kb = 0; kb < K; kb += 8) {
-        const int kn = (K - kb) < 8 ? (K - kb) : 8;
-        // stage q for this window's chunk of rings: qs[par][ring][kk]
-        for (int j = lane; j < 2 * 64 * kn; j += 64) {
-          const int par = j / (64 * kn), rem = j - par * 64 * kn;
-          const int rr = rem / kn, kk = rem - rr * kn;
-          const int rg = c * 64 + rr;
-          qs[(par * 64 + rr) * 8 + kk] =
-              (rg < R) ? t.qbuf[q_index(rg, par, kb + kk, m, K, M + 1)]
-                       : make_double2(0.0, 0.0);
+  for (int kb = 0; kb < K; kb += KB) {
+    const int kn = (K - kb) < KB ? (K - kb) : KB;
+    for (int c = g; c < nchunks; c += G) {
+      const int r = c * 64 + lane;
+      const bool ok = r < R;
+      const double x = ok ? t.ring_x[r] : 0.0;
+      const int sl = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
+      int lmin = sl;
+      for (int d = 32; d >= 1; d >>= 1) {
+        const int o = __shfl_xor(lmin, d);
+        lmin = o < lmin ? o : lmin;
+      }
+      lmin = __builtin_amdgcn_readfirstlane(lmin);
+      if (lmin > L) continue;
+      // the chunk's q for maps kb .. kb + kn - 1: qs[par][ring][kk]
+      wave_sync_lds();  // earlier reads of qs done
+      for (int jq = lane; jq < 2 * 64 * kn; jq += 64) {
+        const int par = jq / (64 * kn), rem = jq - par * 64 * kn;
+        const int rr = rem / kn, kk = rem - rr * kn;
+        const int rg = c * 64 + rr;
+        qs[(par * 64 + rr) * KB + kk] =
+            (rg < R) ? t.qbuf[q_index(rg, par, kb + kk, m, K, M + 1)] : make_double2(0.0, 0.0);
+      }
+      double prev = 0.0, cur = 0.0;
+      for (int l0 = lmin; l0 <= L; l0 += kChunk) {
+        // lanes = rings: L~ at l0 .. l0 + 15 into the tile (seeds sit on
+        // the 16-l grid of m)
+        if (sl == l0) {
+          const double2 v = t.seed_v[(int64_t)m * R + r];
+          prev = v.x;
+          cur = v.y;
         }
-        // lanes = rings: L~ for the window (only on the first map block)
-        if (kb == 0) {
-          for (int sub = 0; sub < 4; ++sub) {
-            const int l0 = wl + 16 * sub;
-            if (l0 > L) {
-              for (int j = 0; j < 16; ++j) tile[(16 * sub + j) * 65 + lane] = 0.0;
-              continue;
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          tile[j * 65 + lane] = cur;
+          const double nx = fma(x, cur, -(ct[l0 + j] * prev));
+          prev = cur;
+          cur = nx;
+        }
+        const double S = cs[(l0 - m) / kChunk];
+        prev *= S;
+        cur *= S;
+        wave_sync_lds();
+        // lane (li, rs): l = l0 + li against rings 16 rs .. 16 rs + 15
+        const int l = l0 + li;
+        const int par = (l - m) & 1;
+        double2 acc[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) acc[kk] = make_double2(0.0, 0.0);
+#pragma unroll 4
+        for (int u = 0; u < 16; ++u) {
+          const int rr = 16 * rs + u;
+          const double v = tile[li * 65 + rr];
+          const double2 *qr = qs + (par * 64 + rr) * KB;
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) {
+            if (kk < kn) {
+              const double2 q = qr[kk];
+              acc[kk].x = fma(v, q.x, acc[kk].x);
+              acc[kk].y = fma(v, q.y, acc[kk].y);
             }
-            if (sl == l0) {
-              const double2 v = t.seed_v[(int64_t)m * R + r];
-              prev = v.x;
-              cur = v.y;
-            }
-            for (int j = 0; j < kChunk; ++j) {
-              tile[(16 * sub + j) * 65 + lane] = cur;
-              const double nx = fma(x, cur, -(ct[l0 + j] * prev));
-              prev = cur;
-              cur = nx;
-            }
-            const double S = cs[(l0 - m) / kChunk];
-            prev *= S;
-            cur *= S;
           }
         }
-        __syncthreads();
-        // lanes = l: sum over the 64 rings
-        const int l = wl + lane;
-        if (l <= L && l < m + lp) {
-          const int par = (l - m) & 1;
-          double2 acc[8];
 #pragma unroll
-          for (int kk = 0; kk < 8; ++kk) acc[kk] = make_double2(0.0, 0.0);
-          for (int rr = 0; rr < 64; ++rr) {
-            const double v = tile[lane * 65 + rr];
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-              if (kk < kn) {
-                const double2 q = qs[(par * 64 + rr) * 8 + kk];
-                acc[kk].x = fma(v, q.x, acc[kk].x);
-                acc[kk].y = fma(v, q.y, acc[kk].y);
-              }
-            }
+        for (int kk = 0; kk < 8; ++kk) {
+          if (kk < kn) {
+            acc[kk].x += __shfl_xor(acc[kk].x, 16);
+            acc[kk].y += __shfl_xor(acc[kk].y, 16);
+            acc[kk].x += __shfl_xor(acc[kk].x, 32);
+            acc[kk].y += __shfl_xor(acc[kk].y, 32);
           }
+        }
+        if (rs == 0 && l <= L) {
           double2 *dst = prow + (int64_t)(l - m) * K + kb;
 #pragma unroll
           for (int kk = 0; kk < 8; ++kk)
             if (kk < kn) dst[kk] = cadd(dst[kk], acc[kk]);
         }
-        __syncthreads();
+        wave_sync_lds();  // tile reads done before the next window
       }
     }
   }
@@ -266,11 +290,15 @@ __global__ __launch_bounds__(256) void k_adj_finish(LegTables t, int G,
 }
 
 int adjoint_groups(const LegTables &t) {
-  // enough (m, group) blocks to fill the GPU, at most one group per 64-ring chunk
+  // one wave per (m, group): groups up to one per 64-ring chunk, while the
+  // waves stay below ~64 per CU and the partial sums below 2 GiB
   const int nchunks = (t.R + 63) / 64;
+  const double row_bytes = (double)t.nlpad * t.K * sizeof(double2);
   int G = 1;
-  while (G < nchunks && (int64_t)(t.M + 1) * G < 4096) G *= 2;
-  return G < nchunks ? G : nchunks;
+  while (2 * G <= nchunks && (int64_t)(t.M + 1) * 2 * G <= 16384 &&
+         row_bytes * 2 * G <= 2.0 * 1024 * 1024 * 1024)
+    G *= 2;
+  return G;
 }
 
 hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double *map,
@@ -283,8 +311,22 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_legendre_adj, dim3((unsigned)((t.M + 1) * G)), dim3(64), 0, s, t, G,
-                     partial);
+  const int64_t waves = (int64_t)(t.M + 1) * G;
+  const size_t alds = (size_t)kAdjWaves * adj_wave_lds_doubles(t.K) * sizeof(double);
+  {
+    // largest LDS of any map count (map blocks of 8)
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+      err = hipFuncSetAttribute((const void *)k_legendre_adj,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((size_t)kAdjWaves * adj_wave_lds_doubles(8) *
+                                      sizeof(double)));
+    });
+    if (err != hipSuccess) return err;
+  }
+  hipLaunchKernelGGL(k_legendre_adj, dim3((unsigned)((waves + kAdjWaves - 1) / kAdjWaves)),
+                     dim3(64 * kAdjWaves), alds, s, t, G, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_adj_finish, dim3((unsigned)((t.nlpad + 255) / 256)), dim3(256), 0, s, t,
