@@ -6,18 +6,17 @@
 // replaces the reference's register-blocked dgemm-like kernels
 // (src/legendre_transform.c.in:389-451, 453-618) for many maps.
 //
-// Workgroup = 4 waves = 128 ring pairs x 32 columns (16 maps) of one m.
-// Waves 2j and 2j+1 take rings 64j .. 64j+63 (one per lane) and the even /
-// odd l, stepping the FP64 recurrence two l at a time (the x^2 form; same
-// monic scaling, seeds and chunk rescale as the FP64 kernel).  Every 64 l
-// (a super-chunk) each lane writes its 32 values, converted to the scheme's
-// input format, into a wave-private LDS tile; the wave then reads MFMA
-// A-fragments from it.  The B operand (prescaled a_lm, pre-split at
-// prescale time) is read by each wave straight into MFMA B-fragments, so
-// the waves of a workgroup never wait for each other.  MFMA accumulates in
-// FP32; every kFlush super-chunks the FP32 sums are added into FP64
-// accumulators (the FP64 reduction across l-chunks), which are written to
-// the same q buffer the FP64 path fills.
+// A workgroup covers 64 ring pairs of one m and up to 4 column blocks of 32
+// columns (16 maps each).  Two producer waves (one per parity, one ring per
+// lane) step the FP64 recurrence two l at a time (the x^2 form) and write
+// each 64-l super-chunk of Lambda, in the scheme's input format, into a
+// double-buffered LDS tile that every column block reads; consumer waves
+// (column block, parity) run the MFMAs against the B operand (the prescaled
+// a_lm, pre-split at prescale time).  One workgroup barrier per
+// super-chunk.  MFMA accumulates in FP32; every kFlush super-chunks the
+// FP32 sums are added into FP64 accumulators (the FP64 reduction across
+// l-chunks), and q is stored once at the end, rounded to FP32 for the
+// mixed plans' FP32 ring stage.
 //
 // Schemes (products of input parts, FP32 accumulation):
 //   0 FP32   v_mfma_f32_16x16x4_f32 on fp32 Lambda and a.
@@ -25,10 +24,11 @@
 //   2 BF16x6 three bf16 parts each: hh + hm + mh + hl + mm + lh.
 //   3 FP16x3 Lambda and a split into two fp16 parts each (~22 bits):
 //            hh + hl + lh.  fp16's range is kept by power-of-two scales:
-//            Lambda (the true value, A_l times the monic one) by 2^e with
-//            |Lambda| <= sqrt((2 lmax + 1) / 4 pi), each a_lm column by a
-//            power of two from its maximum (found per execute), both undone
-//            when the FP32 sums are added into FP64.
+//            Lambda (the true value, stepped by the normalised recurrence)
+//            by 2^e with |Lambda| <= sqrt((2 lmax + 1) / 4 pi), a_lm per
+//            (m, column) by a power of two from its maximum over m's
+//            l-range (found per execute), both undone when the FP32 sums
+//            are added into FP64.
 // BF16 schemes use v_mfma_f32_16x16x32_bf16, FP16x3 v_mfma_f32_16x16x32_f16.
 #include <cmath>
 #include <type_traits>
@@ -102,10 +102,9 @@ int mix_parts(int scheme) { return scheme == 0 ? 1 : (scheme == 2 ? 3 : 2); }
 // m read past its end.
 
 // Value of B at (l = t-index j of chunk c, map k, Re/Im): A_l a_lm (FP16:
-// the column-scaled a_lm, Lambda carries A_l), zero past lmax and for
-// padded maps.
-// csr / csi: the FP16 column scales of (k, Re) and (k, Im), used when
-// scheme == 3.
+// the scaled a_lm, the producer steps the normalised Lambda), zero past
+// lmax and for padded maps.  csr / csi: the FP16 scales of (m, k, Re) and
+// (m, k, Im), used when scheme == 3.
 __device__ __forceinline__ void mix_prep_pair(const LegTables &t, const double2 *__restrict__ alm,
                                               int64_t j, int m, int k, int scheme, double csr,
                                               double csi, double *vr, double *vi) {
@@ -210,8 +209,8 @@ __global__ __launch_bounds__(256) void k_mix_prep_16(LegTables t, const double2 
 // (map, Re/Im) over m's l-range, picks the power of two 2^e that puts it in
 // [2^13, 2^14) (fp16's range with headroom; e = 0 for an all-zero column),
 // records e in cexp[m][col] for the contraction to undo, and writes the
-// scaled a_lm's two fp16 parts for m's chunks as k_mix_prep_16 does.  The
-// second read of m's rows mostly hits the L2.
+// scaled a_lm's two fp16 parts for m's chunks as k_mix_prep_16 does (m's
+// rows are read twice; the second read mostly comes from HBM again).
 __global__ __launch_bounds__(256) void k_mix_prep_f16(LegTables t, const double2 *__restrict__ alm,
                                                       unsigned short *__restrict__ bq, int NCp,
                                                       int64_t part_stride, int *__restrict__ cexp) {
@@ -389,7 +388,7 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
     }
 
     // 16-bit schemes: B fragments of a consumer (its 32 columns, its
-    // parity) straight from global memory into registers, one super-chunk
+    // parity) straight from global memory into registers, two super-chunks
     // ahead of their use (each B element is used by one wave of the
     // workgroup, so it gains nothing from LDS): lane (i16, g) holds the 8 l
     // of chunk g for column 16 ct + i16, 16 bytes per part and ct.  Columns
@@ -439,10 +438,11 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
   } while (0)
 
     // Coefficients of the 32 l of this wave's parity in a super-chunk: lane
-    // k < 32 holds d_{l+2}, lane 32 + k e_{l+2} (l = lsc + pw + 2k), and for
-    // FP16 lane k also lam_scale A_l; loaded one super-chunk ahead, then put in the
-    // wave's LDS slot so that the step loop reads them without waiting on
-    // memory.  Indices past the tables are clamped (those chunks are unused).
+    // k < 32 holds d_{l+2}, lane 32 + k e_{l+2} (l = lsc + pw + 2k) (FP16:
+    // alpha_{l+2}, beta_{l+2}, and lane k also gamma_{l+2}); loaded one
+    // super-chunk ahead, then put in the wave's LDS slot so that the step
+    // loop reads them without waiting on memory.  Indices past the tables
+    // are clamped (those chunks are unused).
     double creg0 = 0.0, creg1 = 0.0, creg2 = 0.0;
     auto load_coef = [&](int lsc) {
       const int k = lane & 31;

@@ -1,35 +1,34 @@
-// Ring stage, register-FFT kernels for the rings whose m-range does not
-// wrap around the ring (n > mmax).  Same semantics as ring.hip
+// Ring stage, register-FFT kernels.  Same semantics as ring.hip
 // (src/wavemoth.c:1083-1191): G_m = E_m +/- O_m, phase e^{i m phi0},
 // bins (m mod n) and conjugates ((n - m) mod n) up to n/2, FFTW-style c2r of
 // length n (imaginary parts of DC and Nyquist ignored, unnormalised), with
 // the same Z-pair reduction of the c2r to a complex DFT of h = n/2.
 //
-// No wrap means bin b receives at most g_b (b <= mmax) and the conjugate
-// term (-1)^s conj g_{n-b} (n - b <= mmax), so the fold is two passes over
-// LDS with one writer per bin and no reduction.
-//
-// k_ring16_belt: h a power of two and mmax <= h (the equatorial belt): one
+// kind 1, belt: h a power of two and mmax <= h (the equatorial belt): one
 //   job per (ring pair, map), both rings from one read of E and O, one
 //   P = h point inverse FFT per ring.
-// k_ring16_cap: h = 2i not a power of two (polar caps with 4i > mmax):
-//   the length-h inverse DFT splits into the DFTs of the even and odd Z
-//   entries, z_p = A_p + e^{2 pi i p/h} B_p, z_{p+i} = A_p - e^{2 pi i p/h} B_p,
-//   each a length-i Bluestein convolution of P = next_pow2(2i - 1) <= 4096
-//   points (chirp c_k = e^{i pi k^2/i}, kernel spectrum vhat tabulated per
-//   i at plan time).  Splitting keeps every transform within 4096 points.
+// kind 2, caps: h = 2i not a power of two, 4i > mmax (no wrap: bin b
+//   receives at most g_b and the conjugate term (-1)^s conj g_{n-b}, so the
+//   fold is two passes over LDS with one writer per bin): the length-h
+//   inverse DFT splits into the DFTs of the even and odd Z entries,
+//   z_p = A_p + e^{2 pi i p/h} B_p, z_{p+i} = A_p - e^{2 pi i p/h} B_p, each a
+//   length-i Bluestein convolution of P = next_pow2(2i - 1) <= 4096 points
+//   (chirp c_k = e^{i pi k^2/i}, kernel spectrum vhat tabulated per i at
+//   plan time).  Splitting keeps every transform within 4096 points.
+// kind 3, wrapping rings (n <= mmax): the fold sums the row blocks of every
+//   bin in registers, then as kind 2.
 //
 // One job = T = P/16 threads holding 16 points each (wm_fft16.h); a block of
 // 512 threads runs 512/T belt jobs or 512/(2T) cap jobs.
 //
 // Every kernel is instantiated for two point types C: double2 (the FP64
 // plans) and float2 (the mixed-precision plans, whose contraction stores q
-// in FP32).  With float2, q, the folds, LDS and the FFT arithmetic are FP32
-// (half the LDS, two blocks per CU); phase factors, twiddles and their
-// powers are formed in double and rounded once; pixels are stored as
-// double.  Jobs of the G maps of one q group (they
-// share 128-byte lines of q) and the blocks holding them are placed on one
-// XCD at consecutive dispatch slots.
+// in FP32).  With float2, q, the folds, LDS and the FFT arithmetic are FP32,
+// and the LDS one FP64 ring takes holds both rings of the pair, which are
+// transformed together; phase factors, twiddles and their powers are formed
+// in double and rounded once; pixels are stored as double.  Jobs of the G
+// maps of one q group (they share the lines of q) and the blocks holding
+// them are placed on one XCD at consecutive dispatch slots.
 #include <mutex>
 
 #include "wm_fft16.h"
