@@ -54,6 +54,28 @@ def test_scheme_vs_fp64(scheme, N, L, M, K, flat):
     assert rms <= BOUND[scheme]
 
 
+@pytest.mark.parametrize("scheme", [F32, F16])
+@pytest.mark.parametrize("N,L,K", [(512, 1024, 8), (256, 512, 5), (1024, 2048, 2)])
+def test_cap_block_stores(scheme, N, L, K):
+    """FP32 cap and wrapping-ring blocks whose jobs are consecutive maps of
+    one ring pair store their pixels together (Bluestein lengths 512 to
+    2048, 2 to 8 jobs per block); K = 5 leaves padding jobs in such blocks."""
+    import torch
+    from wavemoth_amd.torch_sht import random_alm
+    dev = torch.device("cuda:0")
+    print("case %s N=%d L=%d K=%d" % (NAME[scheme], N, L, K), flush=True)
+    alm = random_alm(L, L, K, 77 + K, dev)
+    ref, _ = _maps(N, L, L, K, 0, alm)
+    got, _ = _maps(N, L, L, K, scheme, alm)
+    rms, mx = _err(got, ref)
+    print("rel rms %.3e  max/rms %.3e" % (rms, mx), flush=True)
+    assert rms <= BOUND[scheme]
+    # every map is written (a skipped column would leave the empty() values)
+    assert bool(torch.isfinite(got).all())
+    per_map = torch.linalg.norm(got - ref, dim=0) / torch.linalg.norm(ref, dim=0)
+    assert float(per_map.max()) <= 10 * BOUND[scheme]
+
+
 def test_fp32_scheme_vs_oracle(oracle):
     import torch
     from wavemoth_amd import power_law_alm

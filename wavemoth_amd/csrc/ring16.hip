@@ -570,14 +570,15 @@ __device__ __forceinline__ void cap_xform(const RingPair &rp, int K,
 
 // Both cap rings of a pair from their Z entries together (FP32): half 0
 // transforms the A (even) entries of both rings in AN, AS, half 1 the B
-// entries in BN, BS; then the pixels of both rings.
+// entries in BN, BS; then (own) the pixels of both rings.  Without own the
+// results stay in the buffers for cap_store_block.
 template <int P, typename C>
 __device__ __forceinline__ void cap_xform2(const RingPair &rp, int K,
                                            const double2 *__restrict__ chirp,
                                            const double2 *__restrict__ vhat, const C zn[8],
                                            const C zs[8], C *AN, C *BN, C *AS, C *BS,
                                            const double2 *twl, int tt, bool onn, bool ons,
-                                           double *__restrict__ outn,
+                                           bool own, double *__restrict__ outn,
                                            double *__restrict__ outs) {
   using X = cx<C>;
   constexpr int T = P / 16, T2 = 2 * T;
@@ -618,6 +619,7 @@ __device__ __forceinline__ void cap_xform2(const RingPair &rp, int K,
     }
   }
   __syncthreads();
+  if (!own) return;
 #pragma unroll
   for (int rho = 0; rho < 4; ++rho) {
     const int p = tt + T2 * rho;
@@ -631,6 +633,46 @@ __device__ __forceinline__ void cap_xform2(const RingPair &rp, int K,
         const C a = AS[pad(p)], b = BS[pad(p)];
         put2(outs, p, cadd(a, b), K);
         put2(outs, p + i, csub(a, b), K);
+      }
+    }
+  }
+}
+
+// Pixels of both rings of every job of block b, stored by the whole block
+// (FP32 caps with 2 <= J <= U).  The J jobs of such a block are maps
+// k0 .. k0 + J - 1 of one pair (a job is padding only if every later job
+// is), so with the map fastest over the lanes each pixel row gets J
+// adjacent doubles per request instead of one per lane.  The caller has
+// passed the barrier after the buffers' last write.
+template <int P, typename C>
+__device__ __forceinline__ void cap_store_block(const LegTables &t, const RingTables &rt,
+                                                int first, int count, double *__restrict__ map,
+                                                int b, const C *lds) {
+  constexpr int T = P / 16, J = RB / (2 * T), S = job_slots(P), NP = RB / J;
+  const int K = t.K;
+  int pi0, k0;
+  if (!ring16_job(b, J, 0, K, count, &pi0, &k0)) return;
+  const int jj = threadIdx.x % J, pp = threadIdx.x / J;
+  int pj, kj;
+  if (!ring16_job(b, J, jj, K, count, &pj, &kj)) return;
+  const RingPair rp = rt.pairs[rt.order[first + pi0]];
+  const int i = rp.n / 4;
+  const bool south = rp.pix_s >= 0;
+  const C *AN = lds + ring16_bufs(kRingCap16, sizeof(C)) * jj * S, *BN = AN + S;
+  const C *AS = BN + S, *BS = AS + S;
+  double *outn = map + kj + (int64_t)rp.pix_n * K;
+  double *outs = map + kj + (int64_t)(south ? rp.pix_s : 0) * K;
+#pragma unroll
+  for (int rho = 0; rho < 4; ++rho) {  // i <= P/2 = 4 NP
+    const int p = pp + NP * rho;
+    if (p < i) {
+      const C a = AN[pad(p)], bb = BN[pad(p)];
+      put2(outn, p, cadd(a, bb), K);
+      put2(outn, p + i, csub(a, bb), K);
+      if (south) {
+        const C c = AS[pad(p)], d = BS[pad(p)];
+        put2(outs, p, cadd(c, d), K);
+        put2(outs, p + i, csub(c, d), K);
       }
     }
   }
@@ -675,8 +717,11 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
     cap_z<P>(rp, chirp, bufA, tt, zn);
     cap_z<P>(rp, chirp, bufA2, tt, zs);
     asm volatile("" : "+v"(tt));
+    const bool coop = J >= 2 && K > 1 && J <= pow2_ceil(q_group(K));  // block-uniform
     cap_xform2<P>(rp, K, chirp, vhat, zn, zs, bufA, bufB, bufA2, bufB2, twl, tt, act,
-                  act && south, map + k + rp.pix_n * K, map + k + (south ? rp.pix_s : 0) * K);
+                  act && south, !coop, map + k + rp.pix_n * K,
+                  map + k + (south ? rp.pix_s : 0) * K);
+    if (coop) cap_store_block<P>(t, rt, first, count, map, b, lds);
     return;
   }
   if constexpr (WRAP) {
