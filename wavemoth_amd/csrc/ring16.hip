@@ -19,7 +19,10 @@
 //   bin in registers, then as kind 2.
 //
 // One job = T = P/16 threads holding 16 points each (wm_fft16.h); a block of
-// 512 threads runs 512/T belt jobs or 512/(2T) cap jobs.
+// 512 threads runs 512/T belt jobs or 512/(2T) cap jobs.  Pixel rows hold
+// all maps of a pixel (K doubles), so the stores of one map are strided;
+// paired belt jobs store two maps per lane, and FP32 cap blocks whose jobs
+// are consecutive maps of one pair store all of them together.
 //
 // Every kernel is instantiated for two point types C: double2 (the FP64
 // plans) and float2 (the mixed-precision plans, whose contraction stores q
