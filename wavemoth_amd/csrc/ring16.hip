@@ -300,12 +300,171 @@ __device__ __forceinline__ void job_lane(int tid, int *jb, int *j) {
   }
 }
 
+// pixels 2p, 2p + 1 (p = j + T rho) of maps k and (two) k + 1 of one ring;
+// out = map + k + ring offset.  With K even the two maps of a pixel row are
+// one aligned 16-byte store.
+template <int T, typename C>
+__device__ __forceinline__ void put16x2(double *out, int j, const C v0[16], const C v1[16],
+                                        int K, bool two) {
+  double *o = out + (int64_t)(2 * j) * K;
+  const int64_t st = (int64_t)(2 * T) * K;
+  if (two && (K & 1) == 0) {
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      *reinterpret_cast<double2 *>(o) = make_double2(v0[rho].x, v1[rho].x);
+      *reinterpret_cast<double2 *>(o + K) = make_double2(v0[rho].y, v1[rho].y);
+      o += st;
+    }
+  } else {
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      o[0] = v0[rho].x;
+      o[K] = v0[rho].y;
+      if (two) {
+        o[1] = v1[rho].x;
+        o[K + 1] = v1[rho].y;
+      }
+      o += st;
+    }
+  }
+}
+
+// FP32 belt at K >= 2: a job is (ring pair, maps k = 2 kp and k + 1); the two
+// maps of a ring are transformed together (fft2), the northern ring first,
+// while X^S of both maps waits in registers.  Paired jobs (maps k .. k + 3)
+// share each wave, so a store instruction writes 32 bytes of each of its
+// pixel rows.  The second map's q is the next entry of the q group (k is
+// even; past the last map it reads the group's padding or the q tail).
+template <int P, typename C>
+__device__ __forceinline__ void belt_block_maps2(const LegTables &t, const RingTables &rt,
+                                                 int first, int count,
+                                                 double *__restrict__ map, int b, C *lds,
+                                                 const double2 *twl) {
+  constexpr int T = P / 16, J = RB / T, S = job_slots(P), h = P;
+  using X = cx<C>;
+  int jb, j;
+  job_lane<T>(threadIdx.x, &jb, &j);
+  C *buf0 = lds + jb * S * ring16_bufs(kRingBelt16, sizeof(C)), *buf1 = buf0 + S;
+  const int K = t.K, M = t.M;
+  int pi, kp;
+  const bool act = ring16_job(b, J, jb, (K + 1) / 2, count, &pi, &kp);
+  if (!act) {
+    pi = 0;
+    kp = 0;
+  }
+  const int k = 2 * kp;
+  const bool two = act && k + 1 < K;
+  const RingPair rp = rt.pairs[rt.order[first + pi]];
+  const bool south = rp.pix_s >= 0;
+  const int n = rp.n;
+  const C *qe = (const C *)t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const C *qo = (const C *)t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+  // 1. X_a = w_a (E_a +/- O_a) of both maps; northern into buf0 / buf1
+  C xs0[16], xs1[16];
+  {
+    const double2 wstep = expipi((int64_t)rp.s * T, n);
+    double2 w = expipi((int64_t)rp.s * j, n);
+    const C *pe = qe + j * qs, *po = qo + j * qs;
+    const int64_t st = T * qs;
+#pragma unroll
+    for (int h8 = 0; h8 < 16; h8 += 8) {
+      C e0[8], e1[8], o0[8], o1[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t at = (h8 + q) * st;
+        e0[q] = pe[at];
+        e1[q] = pe[at + 1];
+        o0[q] = po[at];
+        o1[q] = po[at + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int rho = h8 + q, a = j + T * rho;
+        const bool in = act && a <= M, in1 = two && a <= M;
+        const C wc = X::of(w);
+        C xn0 = in ? cmul(cadd(e0[q], o0[q]), wc) : X::mk(0.0, 0.0);
+        C xn1 = in1 ? cmul(cadd(e1[q], o1[q]), wc) : X::mk(0.0, 0.0);
+        xs0[rho] = in ? cmul(csub(e0[q], o0[q]), wc) : X::mk(0.0, 0.0);
+        xs1[rho] = in1 ? cmul(csub(e1[q], o1[q]), wc) : X::mk(0.0, 0.0);
+        if (a == 0) xn0.y = xn1.y = xs0[rho].y = xs1[rho].y = 0.0;
+        buf0[pad(a)] = xn0;
+        buf1[pad(a)] = xn1;
+        w = cmul(w, wstep);
+      }
+    }
+  }
+  // Nyquist bin h: e^{i pi s/2} (g_h + (-1)^s conj g_h), real part
+  C xsh0 = X::mk(0.0, 0.0), xsh1 = xsh0;
+  if (j == 0) {
+    const double sg = rp.s ? -1.0 : 1.0;
+    const C wh = rp.s ? X::mk(0.0, 1.0) : X::mk(1.0, 0.0);
+    auto nyq = [&](C g) {
+      C x = cmul(X::mk(fma(sg, (double)g.x, (double)g.x), fma(-sg, (double)g.y, (double)g.y)), wh);
+      x.y = 0.0;
+      return x;
+    };
+    C e0 = X::mk(0.0, 0.0), e1 = e0, o0 = e0, o1 = e0;
+    if (act && h <= M) {
+      e0 = qe[h * qs];
+      o0 = qo[h * qs];
+      if (two) {
+        e1 = qe[h * qs + 1];
+        o1 = qo[h * qs + 1];
+      }
+    }
+    buf0[pad(h)] = nyq(cadd(e0, o0));
+    buf1[pad(h)] = nyq(cadd(e1, o1));
+    xsh0 = nyq(csub(e0, o0));
+    xsh1 = nyq(csub(e1, o1));
+  }
+  double *outn = map + k + (int64_t)rp.pix_n * K;
+  double *outs = map + k + (int64_t)(south ? rp.pix_s : 0) * K;
+#pragma unroll 1
+  for (int ring = 0; ring < 2; ++ring) {
+    if (ring == 1) {
+      // southern ring: X^S of both maps through the same buffers
+      __syncthreads();  // last FFT pass reads done
+#pragma unroll
+      for (int rho = 0; rho < 16; ++rho) {
+        buf0[pad(j + T * rho)] = xs0[rho];
+        buf1[pad(j + T * rho)] = xs1[rho];
+      }
+      if (j == 0) {
+        buf0[pad(h)] = xsh0;
+        buf1[pad(h)] = xsh1;
+      }
+    }
+    __syncthreads();
+    C v[2][16];
+    {
+      const double2 ostep = expipi(2 * T, n);
+      double2 om = expipi(2 * j, n);
+#pragma unroll
+      for (int rho = 0; rho < 16; ++rho) {
+        const int a = j + T * rho;
+        v[0][rho] = zpair(buf0[pad(a)], buf0[pad(h - a)], om);
+        v[1][rho] = zpair(buf1[pad(a)], buf1[pad(h - a)], om);
+        om = cmul(om, ostep);
+      }
+    }
+    fft16::fft2<P, true>(v, buf0, buf1, twl, j);
+    if (act && (ring == 0 || south)) put16x2<T>(ring ? outs : outn, j, v[0], v[1], K, two);
+  }
+}
+
 // Block b of a belt launch over pairs order[first .. first + count).
 template <int P, typename C>
 __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables &rt, int first,
                                            int count, double *__restrict__ map, int b,
                                            C *lds, const double2 *twl) {
   constexpr int T = P / 16, J = RB / T, S = job_slots(P);
+  if constexpr (sizeof(C) == 8) {
+    if (t.K >= 2) {  // block-uniform
+      belt_block_maps2<P>(t, rt, first, count, map, b, lds, twl);
+      return;
+    }
+  }
   int jb, j;
   job_lane<T>(threadIdx.x, &jb, &j);
   C *buf = lds + jb * S * ring16_bufs(kRingBelt16, sizeof(C));
@@ -834,7 +993,9 @@ hipError_t ring16_set_lds_limits() {
 hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &rt, int first,
                          int count, double *map, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  const int64_t nb = ring16_blocks(kind, P, t.K, count);
+  // FP32 belt jobs at K >= 2 hold two maps each
+  const bool maps2 = kind == kRingBelt16 && t.q32 && t.K >= 2;
+  const int64_t nb = ring16_blocks(kind, P, maps2 ? (t.K + 1) / 2 : t.K, count);
   const size_t lds = ring16_bytes(kind, P, t.q32 ? sizeof(float2) : sizeof(double2));
 #define WM_CASE_K(KIND, PP)                                                                \
   if (kind == KIND && P == PP) {                                                           \
