@@ -840,6 +840,92 @@ __device__ __forceinline__ void cap_store_block(const LegTables &t, const RingTa
   }
 }
 
+// pixels 2p, 2p + 1 of maps k (za) and, if two, k + 1 (zb); out = map + k +
+// ring offset.  With K even the two maps of a row are one 16-byte store.
+template <typename C>
+__device__ __forceinline__ void put2x2(double *out, int p, C za, C zb, int K, bool two) {
+  double *o = out + (int64_t)(2 * p) * K;
+  if (two && (K & 1) == 0) {
+    *reinterpret_cast<double2 *>(o) = make_double2(za.x, zb.x);
+    *reinterpret_cast<double2 *>(o + K) = make_double2(za.y, zb.y);
+  } else {
+    o[0] = za.x;
+    o[K] = za.y;
+    if (two) {
+      o[1] = zb.x;
+      o[K + 1] = zb.y;
+    }
+  }
+}
+
+// FP32 polar caps without wrap at P = 4096 (one job per block) and K >= 2:
+// a job is (ring pair, maps k = 2 kp and k + 1).  Per ring (north, then
+// south): both maps are folded into the four buffers (north k, south k,
+// north k + 1, south k + 1), the ring's Z entries of both maps are formed,
+// the two maps are transformed together (cap_xform2 with the maps in place
+// of the rings) and stored, the two maps of a pixel row in one 16-byte
+// store.  The second map's q is the next entry of the q group, as in
+// belt_block_maps2.
+template <int P, typename C>
+__device__ __forceinline__ void cap_block_maps2(const LegTables &t, const RingTables &rt,
+                                                int first, int count,
+                                                double *__restrict__ map, int b, C *lds,
+                                                const double2 *twl) {
+  constexpr int T = P / 16, T2 = 2 * T, S = job_slots(P);
+  static_assert(RB == T2, "one cap job per block");
+  int tt = threadIdx.x;
+  C *b0 = lds, *b1 = b0 + S, *b2 = b1 + S, *b3 = b2 + S;
+  const int K = t.K, M = t.M;
+  int pi, kp;
+  const bool act = ring16_job(b, 1, 0, (K + 1) / 2, count, &pi, &kp);
+  if (!act) {
+    pi = 0;
+    kp = 0;
+  }
+  const int k = 2 * kp;
+  const bool two = act && k + 1 < K;
+  const RingPair rp = rt.pairs[rt.order[first + pi]];
+  const bool south = rp.pix_s >= 0;
+  const int n = rp.n, i = n / 4;
+  const C *qe = (const C *)t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const C *qo = (const C *)t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+  const double2 *chirp = rt.chirp2 + rp.chirp2_off;  // [i]
+  const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
+  double *outn = map + k + (int64_t)rp.pix_n * K;
+  double *outs = map + k + (int64_t)(south ? rp.pix_s : 0) * K;
+#pragma unroll 1
+  for (int ring = 0; ring < 2; ++ring) {
+    if (ring == 1) {
+      if (!south) break;  // block-uniform
+      __syncthreads();    // pixel reads of the northern ring done
+    }
+    // the folds of both rings and maps (the southern ring's pass reads q
+    // again, so that only one ring's Z entries are held in registers)
+    cap_fold2<P>(qe, qo, qs, M, n, rp.s, b0, b1, tt, act, act && south);
+    cap_fold2<P>(qe + 1, qo + 1, qs, M, n, rp.s, b2, b3, tt, two, two && south);
+    C za[8], zb[8];
+    cap_z<P>(rp, chirp, ring ? b1 : b0, tt, za);
+    cap_z<P>(rp, chirp, ring ? b3 : b2, tt, zb);
+    __syncthreads();  // fold reads done before the transforms write the buffers
+    asm volatile("" : "+v"(tt));
+    cap_xform2<P>(rp, K, chirp, vhat, za, zb, b0, b1, b2, b3, twl, tt, false, false, false,
+                  nullptr, nullptr);
+    if (act) {
+      double *out = ring ? outs : outn;
+#pragma unroll
+      for (int rho = 0; rho < 4; ++rho) {
+        const int p = tt + T2 * rho;
+        if (p < i) {
+          const C a0 = b0[pad(p)], c0 = b1[pad(p)], a1 = b2[pad(p)], c1 = b3[pad(p)];
+          put2x2(out, p, cadd(a0, c0), cadd(a1, c1), K, two);
+          put2x2(out, p + i, csub(a0, c0), csub(a1, c1), K, two);
+        }
+      }
+    }
+  }
+}
+
 // Block b of a cap (WRAP: wrapping-ring) launch over pairs
 // order[first .. first + count).
 template <int P, bool WRAP, typename C>
@@ -848,6 +934,12 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
                                           C *lds, const double2 *twl) {
   constexpr int T = P / 16, J = RB / (2 * T), S = job_slots(P);
   constexpr bool dual = sizeof(C) == 8;  // FP32: both rings together
+  if constexpr (dual && !WRAP && J == 1) {
+    if (t.K >= 2) {  // block-uniform
+      cap_block_maps2<P>(t, rt, first, count, map, b, lds, twl);
+      return;
+    }
+  }
   const int jb = threadIdx.x / (2 * T);
   int tt = threadIdx.x % (2 * T);
   C *bufA = lds + ring16_bufs(kRingCap16, sizeof(C)) * jb * S, *bufB = bufA + S;
@@ -993,8 +1085,9 @@ hipError_t ring16_set_lds_limits() {
 hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &rt, int first,
                          int count, double *map, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  // FP32 belt jobs at K >= 2 hold two maps each
-  const bool maps2 = kind == kRingBelt16 && t.q32 && t.K >= 2;
+  // FP32 belt jobs, and FP32 cap jobs of a whole block, hold two maps each at K >= 2
+  const bool maps2 = (kind == kRingBelt16 || (kind == kRingCap16 && ring16_jobs(kind, P) == 1)) &&
+                     t.q32 && t.K >= 2;
   const int64_t nb = ring16_blocks(kind, P, maps2 ? (t.K + 1) / 2 : t.K, count);
   const size_t lds = ring16_bytes(kind, P, t.q32 ? sizeof(float2) : sizeof(double2));
 #define WM_CASE_K(KIND, PP)                                                                \
