@@ -20,9 +20,13 @@
 //
 // One job = T = P/16 threads holding 16 points each (wm_fft16.h); a block of
 // 512 threads runs 512/T belt jobs or 512/(2T) cap jobs.  Pixel rows hold
-// all maps of a pixel (K doubles), so the stores of one map are strided;
-// paired belt jobs store two maps per lane, and FP32 cap blocks whose jobs
-// are consecutive maps of one pair store all of them together.
+// all maps of a pixel (K doubles), so the stores of one map are strided and
+// the kernels put several maps of a row into each store instruction: FP64
+// belt jobs go in pairs that exchange half-waves (two maps per lane); FP32
+// belt jobs, and FP32 cap jobs of Bluestein length 4096, hold two maps each
+// at K >= 2 (one 16-byte store per row, the paired job writing the next two
+// maps of the same rows); other FP32 cap blocks whose jobs are consecutive
+// maps of one pair store all of them together from LDS.
 //
 // Every kernel is instantiated for two point types C: double2 (the FP64
 // plans) and float2 (the mixed-precision plans, whose contraction stores q
