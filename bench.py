@@ -27,9 +27,12 @@ Workloads (--workload):
 Prints one JSON line.  `roofline` prices the dominant kernel: for fp64 and
 batch the Legendre sweep against the FP64 peak with the reference's flop
 convention (5 + 4 nmaps flops per evaluated (l, m, ring-pair) triple); for
-mixed the headline scheme's contraction kernel: the MFMA flops it executes
-(4 nmaps flops per triple times the scheme's products per MAC: 1 for FP32,
-3 for FP16x3 / BF16x3, 6 for BF16x6) against that MFMA type's dense peak.  Kernel durations are HIP events recorded on the execute stream
+mixed the headline scheme's contraction kernel: its algorithmic flops
+(4 nmaps per triple) against the raw dense peak of the MFMA type it runs on
+(FP32 157.3, FP16 / BF16 2500 TFLOP/s); the rate of the split products it
+executes (x1 FP32, x3 FP16x3 / BF16x3, x6 BF16x6) is a separate field, and
+`ring_stage` prices the ring FFT stage against HBM bandwidth.  Kernel
+durations are HIP events recorded on the execute stream
 around the kernel.  `cpu_baseline` times the CPU oracle (a C restatement of
 the reference algorithm, OpenMP) on a bounded sample of the same workload.
 --calibrate-copy MIB runs a device copy of that size first (untimed), as
@@ -458,12 +461,19 @@ def run_mixed(args, dev, res):
             errs[name] = _err(out, refs[name])
         del p
         flops = 4.0 * K * T_eps
+        # FP32 q read by the ring stage + the FP64 map it writes
+        ring_bytes = (2 * N) * 2 * (L + 1) * K * 8 + npix * K * 8
         schemes[key] = {
             "label": label, "maps_per_s": K / dt, "ms_per_step": dt * 1e3,
             "stages_ms": {"prescale": st[0], "contraction": st[1], "rings": st[2]},
             "contraction_tflops": flops / (st[1] / 1e3) / 1e12,
             "mfma_peak_tflops": peak,
-            "mfma_frac": nprod * flops / (st[1] / 1e3) / 1e12 / peak,
+            "frac": flops / (st[1] / 1e3) / 1e12 / peak,
+            "executed_product_tflops": nprod * flops / (st[1] / 1e3) / 1e12,
+            "executed_product_frac": nprod * flops / (st[1] / 1e3) / 1e12 / peak,
+            "ring_stage": {"bytes": ring_bytes,
+                           "GB_per_s": ring_bytes / (st[2] / 1e3) / 1e9,
+                           "frac_hbm": ring_bytes / (st[2] / 1e3) / 1e9 / HBM_PEAK_GBS},
             "errors_vs_fp64": errs,
             "meets_1e-6": all(e["rms"] <= 1e-6 for e in errs.values()),
         }
@@ -476,8 +486,8 @@ def run_mixed(args, dev, res):
     nprod = SCHEMES[main_key][3]
     kidx = ["f32", "bf16x3", "bf16x6", "f16x3"].index(main_key)
     res.update({
-        "metric": "maps/s synthesis Nside=%d lmax=%d: FP64 vs batched FP32-MFMA @<=1e-6 err"
-                  % (N, L),
+        "metric": "maps/s synthesis Nside=%d lmax=%d: FP64 vs batched FP32-MFMA @<=1e-6 err "
+                  "(value: %s contraction, the fastest scheme within 1e-6)" % (N, L, m["label"]),
         "value": m["maps_per_s"],
         "ms_per_step": m["ms_per_step"],
         "dtype": {"f32": "f32 MFMA", "bf16x3": "bf16x3 split products on bf16 MFMA",
@@ -492,18 +502,22 @@ def run_mixed(args, dev, res):
                                "contraction vs FP64" % K,
                    "nside": N, "lmax": L, "mmax": L, "nmaps": K, "npix": npix},
         "roofline": {
-            "bound": "mfma", "achieved": nprod * m["contraction_tflops"],
+            "bound": "mfma", "achieved": m["contraction_tflops"],
             "peak": SCHEMES[main_key][2], "unit": "TFLOP/s",
-            "frac": m["mfma_frac"],
+            "frac": m["frac"],
+            "executed_product_tflops": m["executed_product_tflops"],
+            "executed_product_frac": m["executed_product_frac"],
             "traffic": measured_traffic("k_mix<%d> nside=%d lmax=%d nmaps=%d" % (kidx, N, L, K)),
             "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                               "x2.0 / x1.0 from a 1 GiB copy)",
             "kernel": "k_mix<%d>" % kidx,
-            "flops_per_launch": nprod * 4.0 * K * T_eps,
-            "flop_convention": "%d x 4*nmaps per (l, m, ring-pair) triple at or after first l "
-                               "(the MFMA products of the contraction; the FP64 recurrence is "
-                               "not counted)" % nprod,
+            "flops_per_launch": 4.0 * K * T_eps,
+            "flop_convention": "4*nmaps per (l, m, ring-pair) triple at or after first l (the "
+                               "contraction's algorithmic flops, priced against the raw dense "
+                               "peak of the MFMA type; the FP64 recurrence is not counted; "
+                               "executed_product_* count the %d split products per MAC)" % nprod,
             "kernel_ms": m["stages_ms"]["contraction"]},
+        "ring_stage": m["ring_stage"],
         "fp64": {"nmaps": K, "maps_per_s": K / dt64, "ms_per_step": dt64 * 1e3,
                  "stages_ms": {"prescale": st64[0], "legendre": st64[1], "rings": st64[2]},
                  "nmaps1_maps_per_s": 1.0 / dt1, "nmaps1_stages_ms": st1},

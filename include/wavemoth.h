@@ -27,6 +27,7 @@
 #ifndef WAVEMOTH_H_
 #define WAVEMOTH_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -95,8 +96,35 @@ void wavemoth_perform_legendre_transforms(wavemoth_plan plan);
 /* replaces wavemoth_perform_backward_ffts (src/wavemoth.c:1193-1195): runs
    the ring stage on the current work buffer and copies the map to host */
 void wavemoth_perform_backward_ffts(wavemoth_plan plan);
-/* replaces wavemoth_disable_phase_shifting (src/wavemoth.c:1240-1248) */
+/* replaces wavemoth_disable_phase_shifting (src/wavemoth.c:1240-1248);
+   waits for all work queued on the current device before it changes the
+   plan's ring table (executes issued later use phi0 = 0) */
 void wavemoth_disable_phase_shifting(wavemoth_plan plan);
+
+/* ---- per-strip Legendre transform (replaces src/legendre_transform.h:14-25,
+        the calls wavemoth/lib.pyx:220-292 wraps and
+        wavemoth/test/test_legendre_transforms.py:22-72 drives) ------------ */
+
+/* replaces wavemoth_legendre_transform_auxdata (src/legendre_transform.h:14-15,
+   impl src/legendre_transform.c.in:723-747): for rows l = lmin + 2k,
+   auxdata[3(k-2) + {0,1,2}] = {alpha, beta, gamma} of
+   P_k = (x^2 + alpha) beta P_{k-1} + gamma P_{k-2}; 3 (nk - 2) entries.
+   Host arithmetic (plan-time data, as in the reference). */
+void wavemoth_legendre_transform_auxdata(size_t m, size_t lmin, size_t nk, double *auxdata);
+/* replaces wavemoth_legendre_transform (src/legendre_transform.h:17-23, impl
+   src/legendre_transform.c.in:680-716; its SSE twin :640-678 computes the
+   same sums): y[i][j] = sum_{k<nk} P_k(x_i) a[2 k nvecs + j] for nx rings,
+   P_0 = P[i], P_1 = Pp1[i].  Host arrays; the sums run on the current HIP
+   device, synchronously.  nk >= 2 and even nvecs as the reference asserts;
+   errors go to wavemoth_last_error() instead of assert(). */
+void wavemoth_legendre_transform(size_t nx, size_t nk, size_t nvecs, double *a, double *y,
+                                 double *x_squared, double *auxdata, double *P, double *Pp1);
+/* Extension: the same on device arrays, asynchronous on `stream` (a
+   hipStream_t, NULL = default).  Returns 0 on success. */
+int wavemoth_legendre_transform_device(size_t nx, size_t nk, size_t nvecs, const double *a,
+                                       double *y, const double *x_squared,
+                                       const double *auxdata, const double *P,
+                                       const double *Pp1, void *stream);
 
 /* ---- extensions (no counterpart in the reference) ---------------------- */
 

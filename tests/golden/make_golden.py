@@ -13,6 +13,13 @@ Fixtures:
   golden_n1024_l2048.npz  Nside=1024, lmax=mmax=2048, nmaps=1: every pixel
                         of 24 rings (poles, cap/belt boundaries, equator,
                         random), every 251st pixel of the map, per-ring rms
+  golden_n2048_l4096.npz  Nside=2048, lmax=mmax=4096, nmaps=3 (map k of the
+                        K=1 case is column 0 of the same a_lm): map 0 of
+                        14 whole rings (both poles, cap rings whose split
+                        Bluestein length is 4096, the cap/belt boundaries,
+                        belt rings of both phi0 parities, the equator),
+                        every 2003rd pixel of all 3 maps, per-ring and
+                        per-map rms
 """
 import os
 import sys
@@ -82,6 +89,33 @@ def config2():
                         alm_checksum=alm_checksum(alm), rings=rings, ring_pix=ring_pix,
                         ring_rms=ring_rms, stride_idx=stride_idx, stride_pix=mp[stride_idx, 0],
                         map_norm=np.array([np.linalg.norm(mp)]))
+
+
+def metric_size():
+    N, L, K, seed = 2048, 4096, 3, 20484096
+    alm = power_law_alm(L, L, K, seed)
+    t0 = time.time()
+    op = O.OraclePlan(N, L, L, K)
+    t1 = time.time()
+    mp = op.synthesize(alm)
+    t2 = time.time()
+    print("metric size precompute %.1fs execute %.1fs" % (t1 - t0, t2 - t1), flush=True)
+    offs = op.ring_offsets
+    nrings = 4 * N - 1
+    # ring index iring (0 = north pole): cap rings iring < N hold 4 (iring + 1)
+    # pixels; their split Bluestein length is next_pow2(2 (iring + 1) - 1),
+    # 4096 from iring = 1024 up.  Belt rings alternate phi0 = 0 / pi / (4N).
+    rings = np.array([0, 1, 511, 1024, 1500, 2046, N - 1, N, N + 1, 2 * N - 2, 2 * N - 1,
+                      3 * N - 1, 3 * N, nrings - 1])
+    ring_pix = np.concatenate([mp[offs[i]:offs[i + 1], 0] for i in rings])
+    ring_rms = np.array([np.sqrt(np.mean(mp[offs[i]:offs[i + 1]] ** 2, axis=0))
+                         for i in range(nrings)])
+    stride_idx = np.arange(0, mp.shape[0], 2003)
+    np.savez_compressed(os.path.join(HERE, "golden_n2048_l4096.npz"), seed=np.array([seed]),
+                        alm_checksum=alm_checksum(alm), alm1_checksum=alm_checksum(alm[:, :1]),
+                        rings=rings, ring_pix=ring_pix, ring_rms=ring_rms,
+                        stride_idx=stride_idx, stride_pix=mp[stride_idx],
+                        map_norm=np.linalg.norm(mp, axis=0))
 
 
 if __name__ == "__main__":

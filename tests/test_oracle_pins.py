@@ -180,6 +180,73 @@ def test_ring_z_matches_healpix_formula(oracle):
         assert abs(z[r] - want) < 1e-16
 
 
+def _all_ring_z(z_eq_to_pole):
+    """z of all 4N-1 rings, north to south, from the 2N northern values
+    ordered equator -> pole."""
+    north = np.asarray(z_eq_to_pole)[::-1]
+    return np.concatenate([north, -north[:-1][::-1]])
+
+
+def _equal_area_residual(N, z, counts):
+    """HEALPix pixels all have area 4 pi / (12 N^2); a ring's centre splits
+    its own pixels in half, so the spherical cap above ring i has area
+    2 pi (1 - z_i) = (pi / (3 N^2)) (pixels of rings 1..i-1 + n_i / 2).
+    Returns the largest deviation from that identity, in units of one
+    pixel's area."""
+    north_pix = np.concatenate([[0], np.cumsum(counts)[:-1]]) + counts / 2.0
+    cap_area = 2 * math.pi * (1 - np.asarray(z))
+    return float(np.abs(cap_area / (math.pi / (3.0 * N * N)) - north_pix).max())
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 8, 64, 2048])
+def test_ring_z_equal_area(oracle, N):
+    """The ring z of the oracle (and the product's helper) satisfy the
+    equal-area identity at every ring: this pins the cap and belt formulas
+    and the equator -> pole ordering without restating them."""
+    from wavemoth_amd import healpix as H
+    _, offs = oracle.grid(N)
+    counts = np.diff(offs).astype(np.float64)
+    z = _all_ring_z(oracle.ring_z(N))
+    assert _equal_area_residual(N, z, counts) < 1e-9 * N * N
+    zp = np.cos(H.get_ring_thetas(N))
+    assert _equal_area_residual(N, zp, counts) < 1e-9 * N * N
+
+
+def test_ring_z_equal_area_catches_planted_mistakes(oracle):
+    """Plausible mistakes in the ring formulas break the identity by at
+    least half a pixel: cap index off by one, belt slope 1/(3N) instead of
+    2/(3N), rings in pole -> equator order."""
+    N = 16
+    _, offs = oracle.grid(N)
+    counts = np.diff(offs).astype(np.float64)
+    i = np.arange(1, 2 * N + 1, dtype=np.float64)
+    good = np.where(i <= N, 1 - i ** 2 / (3 * N * N), 4 / 3 - 2 * i / (3 * N))
+    assert _equal_area_residual(N, _all_ring_z(good[::-1]), counts) < 1e-9
+    plants = [np.where(i <= N, 1 - (i + 1) ** 2 / (3 * N * N), 4 / 3 - 2 * i / (3 * N)),
+              np.where(i <= N, 1 - i ** 2 / (3 * N * N), 1 - (i - N) / (3 * N) - 1 / 3),
+              good[::-1]]
+    for bad in plants:
+        assert _equal_area_residual(N, _all_ring_z(bad[::-1]), counts) > 0.4
+
+
+@pytest.mark.parametrize("N", [3, 5, 7])
+def test_grid_phi0_odd_nside_follows_wavemoth_c(oracle, N):
+    """For odd Nside the reference's grid (src/wavemoth.c:1223, belt phi0
+    from iring % 2) and HEALPix ((i - N + 1) mod 2, the convention of
+    wavemoth/healpix.py) differ on every belt ring: this build follows
+    wavemoth.c, whose maps it reproduces.  Cap rings agree."""
+    phi0s, _ = oracle.grid(N)
+    idx = np.arange(4 * N - 1)
+    i = idx + 1
+    belt = (i > N) & (i <= 3 * N)  # rings N and 3N close the caps (4N pixels)
+    healpix = (math.pi / (4 * N)) * ((i - N + 1) % 2)
+    wavemoth_c = (math.pi / (4 * N)) * (idx % 2)
+    np.testing.assert_allclose(phi0s[belt], wavemoth_c[belt], atol=1e-16)
+    assert np.all(np.abs(phi0s[belt] - healpix[belt]) > 1e-3)
+    from wavemoth_amd import healpix as H
+    np.testing.assert_allclose(H.get_grid_phi0s(N), phi0s, atol=1e-16)
+
+
 def test_product_geometry_helpers_agree_with_oracle(oracle):
     from wavemoth_amd import healpix as H
     for N in [1, 2, 5, 16]:

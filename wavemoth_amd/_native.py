@@ -13,6 +13,7 @@ _LIB = None
 # Every symbol include/wavemoth.h declares, with its ctypes signature.
 _c_int, _c_u, _c_i64 = ctypes.c_int, ctypes.c_uint, ctypes.c_int64
 _vp, _dp, _cp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+_sz = ctypes.c_size_t
 SIGNATURES = {
     "wavemoth_configure": (None, [_cp]),
     "wavemoth_plan_to_healpix": (_vp, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
@@ -41,6 +42,10 @@ SIGNATURES = {
     "wavemoth_batch_set_profiling": (_c_int, [_vp, _c_int]),
     "wavemoth_batch_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int)]),
     "wavemoth_batch_info": (_c_i64, [_vp, _c_int, _c_int]),
+    "wavemoth_legendre_transform_auxdata": (None, [_sz, _sz, _sz, _vp]),
+    "wavemoth_legendre_transform": (None, [_sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wavemoth_legendre_transform_device": (_c_int, [_sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp,
+                                                    _vp, _vp]),
 }
 
 WAVEMOTH_MMAJOR = 0x0

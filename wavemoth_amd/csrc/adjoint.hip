@@ -305,9 +305,10 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
                           double2 *alm, double2 *partial, int G, hipStream_t s) {
   for (int i = 0; i < rt.nlaunch; ++i) {
     if (rt.count[i] == 0) continue;
-    const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
+    // every pair of the group fits: adj_lds[i] >= ring_lds_slots of each
+    const size_t lds = (size_t)rt.adj_lds[i] * sizeof(double2);
     hipLaunchKernelGGL(k_ring_adjoint, dim3(rt.count[i]), dim3(NT), lds, s, t, rt, rt.first[i],
-                       rt.lds[i], map);
+                       rt.adj_lds[i], map);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -446,6 +446,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
       rt.first[rt.nlaunch] = i;
       rt.count[rt.nlaunch] = 0;
       rt.lds[rt.nlaunch] = 0;
+      rt.adj_lds[rt.nlaunch] = 0;
       ++rt.nlaunch;
     }
     const int g = rt.nlaunch - 1;
@@ -454,6 +455,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rt.fftP[g] = fftP[r];
     rt.lds[g] = kind[r] == kRingGeneral ? std::max(rt.lds[g], need[r])
                                        : ring16_lds_slots(kind[r], fftP[r]);
+    rt.adj_lds[g] = std::max(rt.adj_lds[g], need[r]);
     max_need = std::max(max_need, need[r]);
   }
   WM_TRY(ring16_set_lds_limits(), "register-FFT ring kernel LDS limit");
@@ -512,8 +514,12 @@ static bool build_ring_tables(wavemoth_plan p) {
   return true;
 }
 
-// Clears the phi0 phase of every ring pair (phase shifting off).
+// Clears the phi0 phase of every ring pair (phase shifting off).  The
+// table is rewritten only after all work queued on the device has finished
+// (the plan's stream and any caller stream of wavemoth_execute_device), so
+// no execute in flight reads a half-updated table.
 static bool refresh_phases(wavemoth_plan p) {
+  WM_TRY(hipDeviceSynchronize(), "wait for pending executes");
   std::vector<RingPair> pairs(p->rt.npairs);
   WM_TRY(hipMemcpy(pairs.data(), p->rt.pairs, pairs.size() * sizeof(RingPair),
                    hipMemcpyDeviceToHost),

@@ -112,6 +112,10 @@ struct RingTables {
   int first[kMaxLaunches] = {0};
   int count[kMaxLaunches] = {0};
   int lds[kMaxLaunches] = {0};
+  // complex slots k_ring_adjoint needs for the group: the largest
+  // ring_lds_slots over its pairs (a register-FFT group's lds[i] is sized
+  // for the synthesis kernel's layout, not for the adjoint's LDS FFTs)
+  int adj_lds[kMaxLaunches] = {0};
   int kind[kMaxLaunches] = {0};  // RingKind
   int fftP[kMaxLaunches] = {0};  // FFT length of a register-FFT group
 };

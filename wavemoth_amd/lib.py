@@ -124,6 +124,62 @@ class ShtPlan:
         return {k: int(lib().wavemoth_plan_info(self._plan, i)) for i, k in enumerate(keys)}
 
 
+def _c_array(x, name):
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+        raise ValueError("%s must be a C-contiguous float64 array" % name)
+    return x
+
+
+def legendre_transform_auxdata(m, lmin, nk):
+    """Recurrence coefficients of one parity strip (wavemoth/lib.pyx:282-290):
+    3 (nk - 2) values, empty for nk < 3."""
+    if nk < 3:
+        return np.zeros(0)
+    out = np.empty(3 * (nk - 2))
+    lib().wavemoth_legendre_transform_auxdata(int(m), int(lmin), int(nk), out.ctypes.data)
+    return out
+
+
+def legendre_transform(m, lmin, a, y, x_squared, P, Pp1, repeat=1, use_sse=False,
+                       auxdata=None):
+    """Per-strip transform y[i, j] = sum_k P_k(x_i) a[k, j] for rows
+    l = lmin, lmin + 2, ... (wavemoth/lib.pyx:220-280), run on the GPU.
+    Same argument checks and ValueErrors as the reference.  ``use_sse``
+    selects between the reference's two CPU kernels, which compute the same
+    sums; both settings run the one GPU kernel here."""
+    a = _c_array(a, "a")
+    y = _c_array(y, "y")
+    for nm, v in (("x_squared", x_squared), ("P", P), ("Pp1", Pp1)):
+        _c_array(v, nm)
+    if a.ndim != 2 or y.ndim != 2 or x_squared.ndim != 1 or P.ndim != 1 or Pp1.ndim != 1:
+        raise ValueError("nonconforming arrays")
+    nx = x_squared.shape[0]
+    if not nx == P.shape[0] == Pp1.shape[0]:
+        raise ValueError("nonconforming arrays")
+    nk, nvecs = a.shape
+    if not nvecs == y.shape[1] or y.shape[0] != nx:
+        raise ValueError("nonconforming arrays")
+    abig = np.zeros((2 * nk, nvecs))
+    abig[0::2, :] = a
+    if auxdata is None:
+        auxdata = legendre_transform_auxdata(m, lmin, nk)
+    else:
+        _c_array(auxdata, "auxdata")
+        if auxdata.shape[0] != 3 * (nk - 2):
+            raise ValueError("auxdata.shape[0] != 3 * (nk - 2)")
+    if nvecs % 2 != 0:
+        raise ValueError("nvecs not divisible by 2")
+    aux = auxdata if auxdata.size else np.zeros(1)
+    L = lib()
+    for _ in range(repeat):
+        L.wavemoth_legendre_transform(nx, nk, nvecs, abig.ctypes.data, y.ctypes.data,
+                                      x_squared.ctypes.data, aux.ctypes.data, P.ctypes.data,
+                                      Pp1.ctypes.data)
+        e = last_error()
+        if e:
+            raise RuntimeError(e)
+
+
 def query_resourcefile(filename):
     """(Nside, lmax) from a resource-file header, or None
     (src/wavemoth.c:144-155)."""
