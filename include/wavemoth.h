@@ -155,6 +155,26 @@ int wavemoth_execute_adjoint_device(wavemoth_plan plan, const double *d_map, dou
                                     void *stream);
 void wavemoth_execute_adjoint(wavemoth_plan plan);
 
+/* The adjoint without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments, kernels and result as
+   wavemoth_execute_adjoint_device, but the partial sums over ring groups
+   live in a fixed 256 MiB device array of the library (held by each GPU it
+   runs on until the process ends) instead of a buffer the plan allocates
+   on first use.  The ring groups are the plan's (wavemoth_plan_info key 12)
+   reduced by halving until G * nalm_padded * nmaps * 16 bytes fit (key 13);
+   with the same count the result is bitwise that of the other entry,
+   otherwise it differs only by the summation order.  Returns -1 with
+   wavemoth_last_error set when one ring group does not fit (it fits while
+   nmaps * lmax^2 <= ~3.3e7 for mmax = lmax: lmax 5780 at one map, 2040 at
+   eight), when the plan's device
+   is not current, or for NULL / misaligned pointers.  Every call on a GPU
+   shares the array: calls that overlap on one GPU (other streams, other
+   threads, other plans) race and must be serialised by the caller. */
+int wavemoth_execute_adjoint_device_noalloc(wavemoth_plan plan, const double *d_map,
+                                            double *d_alm, void *stream);
+
 /* Stage timing: with profiling on, every execute records HIP events on its
    stream around (0) a_lm prescale (plus the seed kernels with
    WAVEMOTH_SEEDS_ON_THE_FLY), (1) the Legendre sweep, (2) the ring stage.
@@ -169,8 +189,10 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
    the plan, 7 flags, 8 legendre-kernel launches per execute, 9 device bytes
    held by the plan, 10 (l, ring-pair) slots the Legendre sweep evaluates
    including its wave/chunk padding, 11 contraction scheme (-1 FP64,
-   0 MIXED_F32, 1 MIXED_BF16X3, 2 MIXED_BF16X6, 3 MIXED_F16X3).  Returns -1 for an
-   unknown key (key 11 of an FP64 plan is also -1). */
+   0 MIXED_F32, 1 MIXED_BF16X3, 2 MIXED_BF16X6, 3 MIXED_F16X3), 12 ring groups
+   of the adjoint's partial sums, 13 ring groups of the allocation-free
+   adjoint (0: it cannot run this plan).  Returns -1 for an unknown key
+   (key 11 of an FP64 plan is also -1). */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 
 /* ---- batched synthesis (extension) -------------------------------------

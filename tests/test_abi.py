@@ -142,3 +142,12 @@ def test_two_mixed_flags_rejected():
     flags = WAVEMOTH_MIXED_F32 | WAVEMOTH_MIXED_BF16X3
     assert not L.wavemoth_plan_to_healpix(4, 8, 8, 1, 1, None, None, 0, flags, None)
     assert "MIXED" in _native.last_error()
+
+
+def test_noalloc_adjoint_rejects_null_arguments():
+    """The allocation-free adjoint checks its arguments before any device
+    call."""
+    from wavemoth_amd import _native
+    L = _native.lib()
+    assert L.wavemoth_execute_adjoint_device_noalloc(None, None, None, None) == -1
+    assert "NULL" in _native.last_error()

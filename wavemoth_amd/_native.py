@@ -31,6 +31,7 @@ SIGNATURES = {
     "wavemoth_execute_device": (_c_int, [_vp, _vp, _vp, _vp]),
     "wavemoth_execute_adjoint_device": (_c_int, [_vp, _vp, _vp, _vp]),
     "wavemoth_execute_adjoint": (None, [_vp]),
+    "wavemoth_execute_adjoint_device_noalloc": (_c_int, [_vp, _vp, _vp, _vp]),
     "wavemoth_set_profiling": (_c_int, [_vp, _c_int]),
     "wavemoth_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int)]),
     "wavemoth_plan_info": (_c_i64, [_vp, _c_int]),

@@ -23,6 +23,7 @@
 #include <mutex>
 
 #include "wm_fft.h"
+#include "wm_host.h"
 
 namespace wm {
 
@@ -298,6 +299,51 @@ int adjoint_groups(const LegTables &t) {
   while (2 * G <= nchunks && (int64_t)(t.M + 1) * 2 * G <= 16384 &&
          row_bytes * 2 * G <= 2.0 * 1024 * 1024 * 1024)
     G *= 2;
+  return G;
+}
+
+// Partial sums of the adjoint entry that allocates nothing
+// (wavemoth_execute_adjoint_device_noalloc): a fixed array in device
+// memory, held by each GPU the library runs on until the process ends.
+// Every call on a GPU writes its rows before reading them (each wave of
+// k_legendre_adj zeroes its own rows first), so the contents left by an
+// earlier call never matter; calls that overlap on one GPU race on it.
+constexpr size_t kAdjScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) double2 scratch[kAdjScratchBytes / sizeof(double2)];
+
+size_t adjoint_scratch_elems() { return kAdjScratchBytes / sizeof(double2); }
+
+// Device address of `scratch` on the current device (which must be
+// `device`), looked up once per device; nullptr with the error set on
+// failure.
+double2 *adjoint_scratch(int device) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double2 *addr[kMaxDevices] = {nullptr};
+  if (device < 0 || device >= kMaxDevices) {
+    set_err("adjoint scratch: device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_err(std::string("adjoint scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    addr[device] = (double2 *)ptr;
+  }
+  return addr[device];
+}
+
+// Ring groups for partial sums that must fit `cap` complex entries: the
+// largest power of two G <= adjoint_groups(t) with G nlpad nmaps <= cap, or
+// 0 when one group does not fit.
+int adjoint_groups_within(const LegTables &t, size_t cap) {
+  const size_t per = (size_t)t.nlpad * (size_t)t.K;
+  int G = adjoint_groups(t);
+  while (G > 0 && (size_t)G * per > cap) G /= 2;
   return G;
 }
 

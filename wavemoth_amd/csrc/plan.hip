@@ -857,6 +857,42 @@ int wavemoth_execute_adjoint_device(wavemoth_plan p, const double *d_map, double
   return run_adjoint(p, d_map, (double2 *)d_alm, (hipStream_t)stream) ? 0 : -1;
 }
 
+int wavemoth_execute_adjoint_device_noalloc(wavemoth_plan p, const double *d_map,
+                                            double *d_alm, void *stream) {
+  g_err.clear();
+  if (!p || !d_map || !d_alm) {
+    set_err("NULL argument");
+    return -1;
+  }
+  if (((uintptr_t)d_map | (uintptr_t)d_alm) & 15) {
+    set_err("device buffers must be 16-byte aligned");
+    return -1;
+  }
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device) {
+    set_err("adjoint without allocation: make the plan's device current first");
+    return -1;
+  }
+  double2 *scr = adjoint_scratch(dev);
+  if (!scr) return -1;
+  const size_t cap = adjoint_scratch_elems();
+  const int G = adjoint_groups_within(p->lt, cap);
+  if (G == 0) {
+    set_err("adjoint without allocation: the partial sums of one ring group need " +
+            std::to_string((size_t)p->lt.nlpad * p->K * sizeof(double2)) +
+            " bytes, above the fixed scratch of " + std::to_string(cap * sizeof(double2)) +
+            " bytes (use wavemoth_execute_adjoint_device)");
+    return -1;
+  }
+  const hipError_t e = launch_adjoint(p->lt, p->rt, d_map, (double2 *)d_alm, scr, G,
+                                      (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_err(std::string("adjoint kernels: ") + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
 void wavemoth_execute_adjoint(wavemoth_plan p) {
   g_err.clear();
   if (!p || !p->h_in || !p->h_out) {
@@ -928,6 +964,8 @@ int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
     case 9: return (int64_t)p->arena.bytes;
     case 10: return p->T_exec;
     case 11: return p->mix;
+    case 12: return adjoint_groups(p->lt);
+    case 13: return adjoint_groups_within(p->lt, adjoint_scratch_elems());
     default: return -1;
   }
 }

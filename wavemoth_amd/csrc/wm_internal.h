@@ -219,6 +219,9 @@ hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, in
                                int nblocks, hipStream_t s);
 hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu);
 int adjoint_groups(const LegTables &t);
+int adjoint_groups_within(const LegTables &t, size_t cap);
+size_t adjoint_scratch_elems();
+double2 *adjoint_scratch(int device);
 hipError_t adjoint_set_lds_limit(int slots);
 hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double *map,
                           double2 *alm, double2 *partial, int G, hipStream_t s);

@@ -65,7 +65,8 @@ class DevicePlan:
 
     def info(self):
         keys = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
-                "flags", "legendre_launches", "device_bytes", "triples_executed", "scheme"]
+                "flags", "legendre_launches", "device_bytes", "triples_executed", "scheme",
+                "adjoint_groups", "adjoint_groups_noalloc"]
         return {k: int(lib().wavemoth_plan_info(self._plan, i)) for i, k in enumerate(keys)}
 
 
