@@ -146,3 +146,34 @@ def test_mixed_get_work(scheme):
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     print("%s work rel rms %.3e" % (NAME[scheme], err), flush=True)
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("N,L,M,K,flags", [(96, 200, 150, 40, 0), (48, 100, 100, 100, 0),
+                                           (64, 128, 128, 64, 0x100), (2048, 128, 128, 1, 0),
+                                           (8, 300, 300, 16, 0)])
+def test_fp16_register_kernel_vs_producer_kernel(monkeypatch, N, L, M, K, flags):
+    """The FP16x3 contraction with Lambda generated in registers (four
+    l-segments per ring from plan-time checkpoints; the default) and the
+    producer/consumer kernel (WAVEMOTH_MIX_PRODUCERS=1) both meet the bound
+    against the FP64 path, on shapes with a partial 128-ring block (Nside
+    96), several column groups (100 maps), mmax < lmax, seeds generated per
+    execute (checkpoints recomputed; flag 0x100), one map, and more l than
+    rings (Nside 8, lmax 300)."""
+    import torch
+    from wavemoth_amd.torch_sht import random_alm
+    dev = torch.device("cuda:0")
+    print("case FP16x3 kernels N=%d L=%d M=%d K=%d flags=%#x" % (N, L, M, K, flags), flush=True)
+    alm = random_alm(L, M, K, 500 + K, dev)
+    ref, _ = _maps(N, L, M, K, 0, alm)
+    got, _ = _maps(N, L, M, K, F16 | flags, alm)
+    monkeypatch.setenv("WAVEMOTH_MIX_PRODUCERS", "1")
+    old, _ = _maps(N, L, M, K, F16 | flags, alm)
+    r_new, _ = _err(got, ref)
+    r_old, _ = _err(old, ref)
+    r_pair, _ = _err(got, old)
+    print("rel rms: register kernel %.3e  producer kernel %.3e  between %.3e"
+          % (r_new, r_old, r_pair), flush=True)
+    assert bool(torch.isfinite(got).all())
+    assert r_new <= 1e-6 and r_old <= 1e-6 and r_pair <= 1e-6
+    per_map = torch.linalg.norm(got - ref, dim=0) / torch.linalg.norm(ref, dim=0)
+    assert float(per_map.max()) <= 1e-5

@@ -767,6 +767,390 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
   }
 }
 
+// ------------------------------------------- FP16x3 with register Lambda
+
+// k_mixr: the FP16x3 contraction without producer waves.  The Lambda
+// operand is generated in registers, in the A layout of
+// v_mfma_f32_32x32x16_f16 (lane: row = lane % 32, 8 consecutive K values
+// for lane half h = lane / 32).
+//
+// A workgroup covers 128 ring pairs of one m and one parity and 128
+// columns: 4 ring groups of 32 ring pairs, each with 2 waves over the
+// columns (CT column tiles of 32 per wave).  A block's l-range of
+// 4 ns chunks (16 l each, 8 per parity) splits into 4 segments of ns
+// chunks; K-step j takes chunk j of every segment.  The lane halves of the
+// two waves of a ring group step the FP64 recurrence of the same ring over
+// the 4 segments (wave c, half h: segment 2 c + h), each wave builds the A
+// fragment of its 2 segments and exchanges it with the other through LDS,
+// and each runs the MFMAs of all 4 segments against its own columns, so
+// no recurrence step is computed twice.  Segments 1..3 start from
+// checkpoints of the recurrence state written by k_mix_ckpt with the same
+// steps.  The B rows and coefficients of each K-step are staged through
+// LDS two K-steps ahead; while the MFMAs of K-step j run, the lanes build
+// the A fragment of K-step j + 1 (one workgroup barrier per K-step).  FP32
+// MFMA sums over kRFlush K-steps (128 l of the parity) are added into FP64
+// accumulators.
+constexpr int kRRings = 32;   // ring pairs per wave (one 32-row tile)
+constexpr int kRGroups = 4;   // ring groups per workgroup
+constexpr int kRBlock = kRRings * kRGroups;  // ring pairs per workgroup
+constexpr int kRSeg = 4;      // l-segments per ring (2 per wave)
+constexpr int kRWaves = 2 * kRGroups;
+constexpr int kRFlush = 4;    // K-steps per FP32 accumulation
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int CT>
+struct MixRLds {
+  static constexpr int cols = 2 * CT * 32;  // columns of the workgroup
+  // B: [segment][part][cols][8 halves]
+  static constexpr int b_seg = 2 * cols * 16;
+  static constexpr int b_buf = kRSeg * b_seg;
+  // coefficients: [segment][8 steps][alpha, beta, gamma, pad] doubles
+  static constexpr int c_buf = kRSeg * 8 * 4 * 8;
+  // A fragments: [wave][part][lane] 16 bytes
+  static constexpr int a_wave = 2 * 64 * 16;
+  static constexpr int a_buf = kRWaves * a_wave;
+  static constexpr int b_off = 0;
+  static constexpr int c_off = 2 * b_buf;
+  static constexpr int a_off = c_off + 4 * c_buf;
+  static constexpr int bytes = a_off + 2 * a_buf;
+};
+
+// Start of a workgroup's l-range: the smallest seed l of its ring pairs
+// (from the 64-ring wave table), and the number ns of K-steps (chunks per
+// segment).
+__device__ __forceinline__ int mixr_block_l0(const LegTables &t, int m, int bb) {
+  const int w0 = 2 * bb, w1 = 2 * bb + 1;
+  int l0 = t.wstart[(int64_t)m * t.W + w0];
+  if (w1 < t.W) {
+    const int l1 = t.wstart[(int64_t)m * t.W + w1];
+    l0 = l1 < l0 ? l1 : l0;
+  }
+  return l0;
+}
+__device__ __forceinline__ int mixr_segment_steps(int L, int l0) {
+  return ((L - l0) / kChunk + kRSeg) / kRSeg;  // ceil(chunks / kRSeg)
+}
+
+// FP16 scheme: lam_scale Lambda at (sl + p, sl + p + 2) from the seed
+// (the monic start values of the seed chunk, scaled by their A_l).
+__device__ __forceinline__ void mix16_start(const LegTables &t, int m, int r, double x, int sl,
+                                            int p, double lam_scale, double *us, double *vs) {
+  const int64_t offm = t.off[m];
+  const double *ct_ = t.ctil + (offm - m);
+  const double2 sv = t.seed_v[(int64_t)m * t.R + r];
+  const double t1 = fma(x, sv.y, -(ct_[sl] * sv.x));
+  const double t2 = fma(x, t1, -(ct_[sl + 1] * sv.y));
+  const double t3 = fma(x, t2, -(ct_[sl + 2] * t1));
+  const int64_t j0 = offm - m + sl + p;
+  *us = (p ? t1 : sv.y) * (lam_scale * t.ascale[j0]);
+  *vs = (p ? t3 : t2) * (lam_scale * t.ascale[j0 + 2]);
+}
+
+// Recurrence coefficients of the step that yields l + 4 from (l, l + 2):
+// the table entries of l + 2, zero past m's padded l-range (the values
+// there multiply zero B rows and only have to stay finite).
+__device__ __forceinline__ double mix16_coef(const double *tab, int64_t offm, int m, int lp,
+                                             int l) {
+  const int jj = l + 2 - m;
+  return jj < lp ? tab[offm + jj] : 0.0;
+}
+
+// Coefficient tid < 96 of K-step `step` (segment tid / 24, recurrence step
+// (tid % 24) / 3, alpha / beta / gamma = tid % 3) of a workgroup.
+__device__ __forceinline__ double mixr_stage_coef(const double *atab, const double *btab,
+                                                  const double *gtab, int64_t offm, int m,
+                                                  int lp, int l0, int ns, int p, int step,
+                                                  int tid) {
+  const int g = tid / 24, r2 = tid - g * 24;
+  const int i = r2 / 3, which = r2 - i * 3;
+  const int l = l0 + kChunk * (g * ns + step) + p + 2 * i;
+  const double *tab = which == 0 ? atab : (which == 1 ? btab : gtab);
+  return mix16_coef(tab, offm, m, lp, l);
+}
+
+// Checkpoints: state (lam_scale Lambda_ls, lam_scale Lambda_{ls + 2}) of
+// ring r, parity p at the first l = ls of segments 1 .. kRSeg - 1 of r's
+// workgroup (zero when the seed starts at or after it), at
+// ckpt[((m * 2 + p) * R + r) * (kRSeg - 1) + segment - 1].  One thread
+// per (m, p, r).
+__global__ __launch_bounds__(256) void k_mix_ckpt(LegTables t, const double *__restrict__ atab,
+                                                  const double *__restrict__ btab,
+                                                  const double *__restrict__ gtab,
+                                                  double lam_scale, double2 *__restrict__ ckpt) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int R = t.R, L = t.L;
+  if (id >= (int64_t)(t.M + 1) * 2 * R) return;
+  const int r = (int)(id % R);
+  const int mp = (int)(id / R), p = mp & 1, m = mp >> 1;
+  double2 out[kRSeg - 1];
+#pragma unroll
+  for (int s = 0; s < kRSeg - 1; ++s) out[s] = make_double2(0.0, 0.0);
+  const int l0 = mixr_block_l0(t, m, r / kRBlock);
+  const int sl = t.seed_l[(int64_t)m * R + r];
+  if (l0 <= L && sl <= L) {
+    const int ns = mixr_segment_steps(L, l0);
+    const int llast = l0 + kChunk * ns * (kRSeg - 1);
+    if (sl < llast) {
+      const double x = t.ring_x[r], x2 = x * x;
+      const int64_t offm = t.off[m];
+      const int lp = (L - m + kChunk) / kChunk * kChunk;
+      double u, v;
+      mix16_start(t, m, r, x, sl, p, lam_scale, &u, &v);
+      for (int lc = sl; lc < llast; lc += kChunk) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int l = lc + p + 2 * i;
+          const double a = mix16_coef(atab, offm, m, lp, l);
+          const double b = mix16_coef(btab, offm, m, lp, l);
+          const double g = mix16_coef(gtab, offm, m, lp, l);
+          const double w = fma(fma(a, x2, b), v, -(g * u));
+          u = v;
+          v = w;
+        }
+        const int ln = lc + kChunk;  // state (u, v) now holds l = ln + p, ln + p + 2
+        const int sg = (ln - l0) / (kChunk * ns);
+        if (ln == l0 + kChunk * ns * sg && sg >= 1 && sg < kRSeg) {
+#pragma unroll
+          for (int s = 1; s < kRSeg; ++s)
+            if (s == sg) out[s - 1] = make_double2(u, v);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < kRSeg - 1; ++s) ckpt[id * (kRSeg - 1) + s] = out[s];
+}
+
+template <int CT>
+__global__ __launch_bounds__(64 * kRWaves) void k_mixr(
+    LegTables t, const unsigned short *__restrict__ bq, int NCp, int ncg, int bpm,
+    int64_t part_stride, double lam_scale, const int *__restrict__ cexp,
+    const double *__restrict__ atab, const double *__restrict__ btab,
+    const double *__restrict__ gtab, const double2 *__restrict__ ckpt) {
+  using Ly = MixRLds<CT>;
+  constexpr int NT = 64 * kRWaves;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x;
+  const int s8 = b >> 3;
+  const int m = (b & 7) + 8 * (s8 / bpm);
+  if (m > t.M) return;
+  const int L = t.L, R = t.R, K = t.K, Mp1 = t.M + 1;
+  const int W2 = (R + kRBlock - 1) / kRBlock;
+  const int item = s8 % bpm;  // (column group, parity, ring block)
+  const int cg = item / (2 * W2);
+  const int rem = item - cg * 2 * W2;
+  const int p = rem / W2, bb = rem - p * W2;
+  const int n = lane & 31, h = lane >> 5;
+  const int rg = wv % kRGroups, chh = wv / kRGroups;
+  const int wpart = rg + kRGroups * (1 - chh);  // the other wave of this ring group
+  const int ring0 = bb * kRBlock + rg * kRRings;
+  const int colg0 = cg * Ly::cols;           // first column of the workgroup
+  const int colw0 = colg0 + chh * CT * 32;   // first column of this wave
+  const int l0 = __builtin_amdgcn_readfirstlane(mixr_block_l0(t, m, bb));
+  const int64_t offm = t.off[m];
+
+  double qd[CT][16];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) qd[ct][e] = 0.0;
+
+  if (l0 <= L) {
+    const int ns = mixr_segment_steps(L, l0);
+    const int lp = (L - m + kChunk) / kChunk * kChunk;
+    const int64_t c0 = (offm + l0 - m) / kChunk;
+    const int64_t cend = (offm + lp) / kChunk;
+    // this lane's ring (A row) and segment
+    const int sg = 2 * chh + h;
+    const int rr = ring0 + n;
+    const bool rok = rr < R;
+    const double x = rok ? t.ring_x[rr] : 0.0;
+    const double x2 = x * x;
+    const int sl = rok ? t.seed_l[(int64_t)m * R + rr] : kSeedNone;
+    double us = 0.0, vs = 0.0, u = 0.0, v = 0.0;
+    if (sl <= L) mix16_start(t, m, rr, x, sl, p, lam_scale, &us, &vs);
+    if (sg > 0 && sl < l0 + kChunk * ns * sg && sl <= L) {
+      const double2 c = ckpt[(((int64_t)m * 2 + p) * R + rr) * (kRSeg - 1) + sg - 1];
+      u = c.x;
+      v = c.y;
+    }
+    // FP16 scales of this lane's columns, undone at the FP64 flush
+    double unscale[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int col = colw0 + ct * 32 + n;
+      unscale[ct] = ldexp(1.0 / lam_scale, -cexp[(int64_t)m * NCp + (col < NCp ? col : 0)]);
+    }
+
+    // Stage j = B rows of chunk j of every segment (zero past m's chunks
+    // and past the last column; bbuf j % 2) and the coefficients of K-step
+    // j + 1 (cbuf (j + 1) % 4), loaded two K-steps ahead into one of two
+    // register sets.  A(j) of both waves of a ring group sits in abuf j % 2.
+    // Macros, so that no closure object holds the register arrays.
+    constexpr int NVEC = kRSeg * 2 * Ly::cols;  // 16-byte vectors per stage
+    constexpr int NV = (NVEC + NT - 1) / NT;
+    constexpr int NC = kRSeg * 24;              // coefficients per stage
+    u32x4 breg[2][NV];
+    double creg[2] = {0.0, 0.0};
+#define WM_R_COEF(step, tid) \
+  mixr_stage_coef(atab, btab, gtab, offm, m, lp, l0, ns, p, step, tid)
+#define WM_R_CSLOT(tid) ((((tid) / 24) * 8 + ((tid) % 24) / 3) * 4 + (tid) % 3)
+#define WM_R_GLOAD(S, jg)                                                                    \
+  do {                                                                                       \
+    _Pragma("unroll") for (int x_ = 0; x_ < NV; ++x_) {                                      \
+      const int vi = threadIdx.x + NT * x_;                                                  \
+      u32x4 val = (u32x4){0u, 0u, 0u, 0u};                                                   \
+      if (vi < NVEC) {                                                                       \
+        const int g_ = vi / (2 * Ly::cols), r2 = vi - g_ * (2 * Ly::cols);                   \
+        const int q_ = r2 / Ly::cols, c_ = r2 - q_ * Ly::cols;                               \
+        const int64_t ch = c0 + (int64_t)g_ * ns + (jg);                                     \
+        const int col = colg0 + c_;                                                          \
+        if (ch < cend && col < NCp)                                                          \
+          val = *(const u32x4 *)(bq + q_ * part_stride + ((ch * 2 + p) * NCp + col) * 8);    \
+      }                                                                                      \
+      breg[S][x_] = val;                                                                     \
+    }                                                                                        \
+    if (threadIdx.x < NC) creg[S] = WM_R_COEF((jg) + 1, threadIdx.x);                        \
+  } while (0)
+#define WM_R_LSTORE(S, BB, CB)                                                               \
+  do {                                                                                       \
+    unsigned char *bd = lds + Ly::b_off + (BB) * Ly::b_buf;                                  \
+    _Pragma("unroll") for (int x_ = 0; x_ < NV; ++x_) {                                      \
+      const int vi = threadIdx.x + NT * x_;                                                  \
+      if (vi < NVEC) *(u32x4 *)(bd + vi * 16) = breg[S][x_];                                 \
+    }                                                                                        \
+    if (threadIdx.x < NC)                                                                    \
+      ((double *)(lds + Ly::c_off + (CB) * Ly::c_buf))[WM_R_CSLOT(threadIdx.x)] = creg[S];   \
+  } while (0)
+    // A(jk) from cbuf CB: this lane's 8 recurrence steps, the fp16 parts
+    // (high part by round-toward-zero packing, low part the exact residual
+    // by one mixed-precision FMA reading the fp16 half), written to abuf AB
+#define WM_R_ABUILD(jk, CB, AB)                                                              \
+  do {                                                                                       \
+    const int lc = l0 + kChunk * (sg * ns + (jk));                                           \
+    if (lc == sl) {                                                                          \
+      u = us;                                                                                \
+      v = vs;                                                                                \
+    }                                                                                        \
+    const double *cf = (const double *)(lds + Ly::c_off + (CB) * Ly::c_buf) + sg * 32;       \
+    float f[8];                                                                              \
+    _Pragma("unroll") for (int i_ = 0; i_ < 8; ++i_) {                                       \
+      f[i_] = (float)u;                                                                      \
+      const double2 ab = *(const double2 *)(cf + 4 * i_);                                    \
+      const double gm = cf[4 * i_ + 2];                                                      \
+      const double w = fma(fma(ab.x, x2, ab.y), v, -(gm * u));                               \
+      u = v;                                                                                 \
+      v = w;                                                                                 \
+    }                                                                                        \
+    unsigned wh[4], wl[4];                                                                   \
+    _Pragma("unroll") for (int i_ = 0; i_ < 8; i_ += 2) {                                    \
+      typedef __fp16 h16x2 __attribute__((ext_vector_type(2)));                              \
+      const h16x2 hh = __builtin_amdgcn_cvt_pkrtz(f[i_], f[i_ + 1]);                         \
+      const unsigned hb = __builtin_bit_cast(unsigned, hh);                                  \
+      float r0, r1;                                                                          \
+      asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r0) : "v"(hb),           \
+          "v"(f[i_]));                                                                       \
+      asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"                  \
+          : "=v"(r1) : "v"(hb), "v"(f[i_ + 1]));                                             \
+      wh[i_ >> 1] = hb;                                                                      \
+      wl[i_ >> 1] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(r0, r1));        \
+    }                                                                                        \
+    unsigned char *ad = lds + Ly::a_off + (AB) * Ly::a_buf + wv * Ly::a_wave + lane * 16;    \
+    *(u32x4 *)ad = (u32x4){wh[0], wh[1], wh[2], wh[3]};                                      \
+    *(u32x4 *)(ad + 64 * 16) = (u32x4){wl[0], wl[1], wl[2], wl[3]};                          \
+  } while (0)
+    // MFMAs of K-step j: for both waves' A fragments (abuf AB) the B
+    // fragments of their 2 segments and this wave's CT column tiles (bbuf
+    // BB), 3 MFMAs per (fragment, tile), consecutive MFMAs on different
+    // tiles; FRESH: the first product overwrites acc
+#define WM_R_MMA(BB, AB, FRESH)                                                              \
+  do {                                                                                       \
+    _Pragma("unroll") for (int kh = 0; kh < 2; ++kh) {                                       \
+      const int wsrc = kh == 0 ? wv : wpart;                                                 \
+      const int csrc = kh == 0 ? chh : 1 - chh;                                              \
+      const unsigned char *as_ = lds + Ly::a_off + (AB) * Ly::a_buf + wsrc * Ly::a_wave +    \
+                                 lane * 16;                                                  \
+      const f16x8 ah = *(const f16x8 *)as_;                                                  \
+      const f16x8 al = *(const f16x8 *)(as_ + 64 * 16);                                      \
+      const unsigned char *bs = lds + Ly::b_off + (BB) * Ly::b_buf +                         \
+                                (2 * csrc + h) * Ly::b_seg + (chh * CT * 32 + n) * 16;       \
+      f16x8 bh[CT], bl[CT];                                                                  \
+      _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                    \
+        bh[ct] = *(const f16x8 *)(bs + (ct * 32) * 16);                                      \
+        bl[ct] = *(const f16x8 *)(bs + (Ly::cols + ct * 32) * 16);                           \
+      }                                                                                      \
+      _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) acc[ct] =                            \
+          __builtin_amdgcn_mfma_f32_32x32x16_f16(                                            \
+              ah, bh[ct], ((FRESH) && kh == 0) ? (f32x16){} : acc[ct], 0, 0, 0);             \
+      _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) acc[ct] =                            \
+          __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[ct], acc[ct], 0, 0, 0);              \
+      _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) acc[ct] =                            \
+          __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[ct], acc[ct], 0, 0, 0);              \
+    }                                                                                        \
+  } while (0)
+
+    f32x16 acc[CT];
+    // prologue: stage 0 and the coefficients of K-step 0 into LDS, A(0)
+    WM_R_GLOAD(0, 0);
+    if (threadIdx.x < NC)
+      ((double *)(lds + Ly::c_off))[WM_R_CSLOT(threadIdx.x)] = WM_R_COEF(0, threadIdx.x);
+    WM_R_LSTORE(0, 0, 1);
+    WM_R_GLOAD(1, 1);
+    __syncthreads();
+    WM_R_ABUILD(0, 0, 0);
+    __syncthreads();
+    static_assert(kRFlush == 4, "buffer indices are static within a flush group of 4");
+    for (int j0 = 0; j0 < ns; j0 += kRFlush) {
+#pragma unroll
+      for (int jj = 0; jj < kRFlush; ++jj) {
+        const int j = j0 + jj;
+        if (j < ns) {
+          // register set jj & 1 (stage j) went to LDS before K-step j began
+          WM_R_GLOAD(jj & 1, j + 2);
+          WM_R_MMA(jj & 1, jj & 1, jj == 0);
+          WM_R_ABUILD(j + 1, (jj + 1) & 3, (jj + 1) & 1);
+#pragma unroll
+          for (int q_ = 0; q_ < 6 * CT; ++q_) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // then VALU
+          }
+          WM_R_LSTORE((jj + 1) & 1, (jj + 1) & 1, (jj + 2) & 3);
+          __syncthreads();
+        }
+      }
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) qd[ct][e] += unscale[ct] * (double)acc[ct][e];
+    }
+#undef WM_R_COEF
+#undef WM_R_CSLOT
+#undef WM_R_GLOAD
+#undef WM_R_LSTORE
+#undef WM_R_ABUILD
+#undef WM_R_MMA
+  }
+
+  // store: acc entry e = 4 i + jj holds row 8 i + 4 h + jj, column n
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    const int col = colw0 + ct * 32 + n;
+    if (col < 2 * K) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int ring = ring0 + 8 * (e >> 2) + 4 * h + (e & 3);
+        if (ring < R) {
+          const int64_t at = 2 * q_index(ring, p, col >> 1, m, K, Mp1) + (col & 1);
+          ((float *)t.qbuf)[at] = (float)qd[ct][e];
+        }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------- launchers
 
 int64_t mix_bmat_elems(const LegTables &t, int NCp) {
@@ -827,6 +1211,46 @@ static hipError_t launch_mix_s(const LegTables &t, const void *bmat, int NCp,
   }
 }
 
+template <int CT>
+static hipError_t launch_mixr_t(const LegTables &t, const void *bmat, int NCp,
+                                int64_t part_stride, double lam_scale, const int *cexp,
+                                const double *atab, const double *btab, const double *gtab,
+                                const double2 *ckpt, hipStream_t s) {
+  using Ly = MixRLds<CT>;
+  const int ncg = (NCp + Ly::cols - 1) / Ly::cols;
+  const int W2 = (t.R + kRBlock - 1) / kRBlock;
+  const int bpm = ncg * 2 * W2;
+  const int mgroups = (t.M + 1 + 7) / 8;
+  const unsigned grid = (unsigned)(8 * mgroups * bpm);
+  hipLaunchKernelGGL((k_mixr<CT>), dim3(grid), dim3(64 * kRWaves), Ly::bytes, s, t,
+                     (const unsigned short *)bmat, NCp, ncg, bpm, part_stride, lam_scale, cexp,
+                     atab, btab, gtab, ckpt);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixr(const LegTables &t, const void *bmat, int NCp, int64_t part_stride,
+                       double lam_scale, const int *cexp, const double *atab,
+                       const double *btab, const double *gtab, const double2 *ckpt,
+                       hipStream_t s) {
+  // two waves of CT column tiles per 32 ring pairs: 64 columns per
+  // workgroup up to 64 columns, else groups of 128
+  if (NCp <= 64)
+    return launch_mixr_t<1>(t, bmat, NCp, part_stride, lam_scale, cexp, atab, btab, gtab, ckpt, s);
+  return launch_mixr_t<2>(t, bmat, NCp, part_stride, lam_scale, cexp, atab, btab, gtab, ckpt, s);
+}
+
+int64_t mix_ckpt_elems(const LegTables &t) {
+  return (int64_t)(t.M + 1) * 2 * t.R * (kRSeg - 1);
+}
+
+hipError_t launch_mix_ckpt(const LegTables &t, const double *atab, const double *btab,
+                           const double *gtab, double lam_scale, double2 *ckpt, hipStream_t s) {
+  const int64_t n = (int64_t)(t.M + 1) * 2 * t.R;  // one thread per (m, parity, ring)
+  hipLaunchKernelGGL(k_mix_ckpt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, atab,
+                     btab, gtab, lam_scale, ckpt);
+  return hipGetLastError();
+}
+
 hipError_t mix_set_lds_limits() {
   // the FP16 prescale's column maxima: NCp <= 8192 (nmaps <= 4096)
   hipError_t e = hipFuncSetAttribute((const void *)k_mix_prep_f16,
@@ -840,6 +1264,12 @@ hipError_t mix_set_lds_limits() {
   WM_SET_S(0) WM_SET_S(1) WM_SET_S(2) WM_SET_S(3)
 #undef WM_SET_S
 #undef WM_SET
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_mixr<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)MixRLds<1>::bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_mixr<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)MixRLds<2>::bytes);
   return e;
 }
 

@@ -14,6 +14,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -58,6 +59,10 @@ struct _wavemoth_plan {
   // schemes) or alpha, beta, gamma of the normalised one (FP16)
   double *mix_dtil = nullptr, *mix_etil = nullptr, *mix_gtil = nullptr;
   double mix_lam_scale = 1.0;
+  // FP16x3: recurrence checkpoints of the register-Lambda kernel, and the
+  // choice between it (default) and the producer/consumer kernel
+  double2 *mix_ckpt = nullptr;
+  bool mix_regs = true;
   LegCounts cnt;
   int64_t T_exec = 0;
   DevArena arena;
@@ -287,6 +292,18 @@ static bool build_legendre_tables(wavemoth_plan p) {
     }
     p->mix_lam_scale = mix_lambda_scale(p->L);
     WM_TRY(mix_set_lds_limits(), "mixed kernel LDS limit");
+    if (p->mix == 3) {
+      const char *ev = std::getenv("WAVEMOTH_MIX_PRODUCERS");
+      p->mix_regs = !(ev && ev[0] == '1');
+    }
+    if (p->mix == 3 && p->mix_regs) {
+      if (!dalloc(p->arena, &p->mix_ckpt, (size_t)mix_ckpt_elems(t), "recurrence checkpoints"))
+        return false;
+      WM_TRY(launch_mix_ckpt(t, p->mix_dtil, p->mix_etil, p->mix_gtil, p->mix_lam_scale,
+                             p->mix_ckpt, p->stream),
+             "checkpoint kernel");
+      WM_TRY(hipStreamSynchronize(p->stream), "checkpoint kernel");
+    }
     if (!dalloc(p->arena, &t.atil, 1, "atil")) return false;
   } else if (!dalloc(p->arena, &t.atil, (size_t)t.nlpad * legendre_padded_maps(p->K), "atil")) {
     return false;
@@ -561,15 +578,24 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
   if (p->fly) {
     WM_TRY(launch_seeds(p->lt, s), "seed kernel");
     WM_TRY(launch_wave_bounds(p->lt, s), "wave-bounds kernel");
+    if (p->mix_ckpt)
+      WM_TRY(launch_mix_ckpt(p->lt, p->mix_dtil, p->mix_etil, p->mix_gtil, p->mix_lam_scale,
+                             p->mix_ckpt, s),
+             "checkpoint kernel");
   }
   if (p->mix >= 0) {
     WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride,
                            p->mix_cexp, s),
            "mixed prescale kernel");
     if (!mark(p, 1, s)) return false;
-    WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, p->mix_lam_scale,
-                      p->mix_cexp, p->mix_dtil, p->mix_etil, p->mix_gtil, s),
-           "mixed contraction kernel");
+    if (p->mix_ckpt)
+      WM_TRY(launch_mixr(p->lt, p->mix_b, p->mix_ncp, p->mix_pstride, p->mix_lam_scale,
+                         p->mix_cexp, p->mix_dtil, p->mix_etil, p->mix_gtil, p->mix_ckpt, s),
+             "mixed contraction kernel");
+    else
+      WM_TRY(launch_mix(p->lt, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride, p->mix_lam_scale,
+                        p->mix_cexp, p->mix_dtil, p->mix_etil, p->mix_gtil, s),
+             "mixed contraction kernel");
   } else {
     WM_TRY(launch_prep_alm(p->lt, d_alm, s), "a_lm prescale kernel");
     if (!mark(p, 1, s)) return false;

@@ -190,6 +190,18 @@ hipError_t launch_mix(const LegTables &t, const void *bmat, int NCp, int scheme,
                       const double *dtil, const double *etil, const double *gtil,
                       hipStream_t s);
 double mix_lambda_scale(int lmax);
+// FP16x3 contraction with the Lambda operand generated in registers
+// (k_mixr): its recurrence runs as four l-segments per ring, segments 1..3
+// starting from checkpoints written by launch_mix_ckpt (mix_ckpt_elems(t)
+// entries; after the seeds and wave bounds, again whenever they change).  atab / btab / gtab: alpha, beta,
+// gamma of the normalised step (dtil / etil / gtil of the FP16 scheme).
+int64_t mix_ckpt_elems(const LegTables &t);
+hipError_t launch_mix_ckpt(const LegTables &t, const double *atab, const double *btab,
+                           const double *gtab, double lam_scale, double2 *ckpt, hipStream_t s);
+hipError_t launch_mixr(const LegTables &t, const void *bmat, int NCp, int64_t part_stride,
+                       double lam_scale, const int *cexp, const double *atab,
+                       const double *btab, const double *gtab, const double2 *ckpt,
+                       hipStream_t s);
 
 // Shape of the Legendre sweep for nmaps = K.
 int legendre_map_block(int K);
