@@ -786,8 +786,8 @@ __global__ __launch_bounds__(64 * (2 + 2 * CB)) void k_mix(
 // no recurrence step is computed twice.  Segments 1..3 start from
 // checkpoints of the recurrence state written by k_mix_ckpt with the same
 // steps.  The B rows and coefficients of each K-step are staged through
-// LDS two K-steps ahead; while the MFMAs of K-step j run, the lanes build
-// the A fragment of K-step j + 1 (one workgroup barrier per K-step).  FP32
+// LDS, loaded two K-steps ahead; while the MFMAs of K-step j run, the lanes
+// build the A fragment of K-step j + 1 (one workgroup barrier per K-step).  FP32
 // MFMA sums over kRFlush K-steps (128 l of the parity) are added into FP64
 // accumulators.
 constexpr int kRRings = 32;   // ring pairs per wave (one 32-row tile)
@@ -820,11 +820,15 @@ struct MixRLds {
 // (from the 64-ring wave table), and the number ns of K-steps (chunks per
 // segment).
 __device__ __forceinline__ int mixr_block_l0(const LegTables &t, int m, int bb) {
-  const int w0 = 2 * bb, w1 = 2 * bb + 1;
-  int l0 = t.wstart[(int64_t)m * t.W + w0];
-  if (w1 < t.W) {
-    const int l1 = t.wstart[(int64_t)m * t.W + w1];
-    l0 = l1 < l0 ? l1 : l0;
+  constexpr int per = kRBlock / 64;  // wave-table entries per workgroup block
+  static_assert(per * 64 == kRBlock, "workgroup blocks are whole 64-ring table entries");
+  int l0 = t.wstart[(int64_t)m * t.W + per * bb];
+#pragma unroll
+  for (int k = 1; k < per; ++k) {
+    if (per * bb + k < t.W) {
+      const int lk = t.wstart[(int64_t)m * t.W + per * bb + k];
+      l0 = lk < l0 ? lk : l0;
+    }
   }
   return l0;
 }
@@ -987,43 +991,69 @@ __global__ __launch_bounds__(64 * kRWaves) void k_mixr(
 
     // Stage j = B rows of chunk j of every segment (zero past m's chunks
     // and past the last column; bbuf j % 2) and the coefficients of K-step
-    // j + 1 (cbuf (j + 1) % 4), loaded two K-steps ahead into one of two
-    // register sets.  A(j) of both waves of a ring group sits in abuf j % 2.
-    // Macros, so that no closure object holds the register arrays.
+    // j + 1 (cbuf (j + 1) % 4).  At the end of K-step j the register set
+    // holding stage j + 1 goes to LDS and is reloaded with stage j + 2, so
+    // each load has a whole K-step to arrive.  A(j) of both waves of a ring
+    // group sits in abuf j % 2.  Macros, so that no closure object holds
+    // the register arrays.
     constexpr int NVEC = kRSeg * 2 * Ly::cols;  // 16-byte vectors per stage
-    constexpr int NV = (NVEC + NT - 1) / NT;
+    static_assert(NVEC % NT == 0, "every thread stages the same number of vectors");
+    constexpr int NV = NVEC / NT;
     constexpr int NC = kRSeg * 24;              // coefficients per stage
-    u32x4 breg[2][NV];
-    double creg[2] = {0.0, 0.0};
-#define WM_R_COEF(step, tid) \
-  mixr_stage_coef(atab, btab, gtab, offm, m, lp, l0, ns, p, step, tid)
+    // per-thread staging state, fixed for the block (a K-step only adds
+    // its index): the chunk of K-step 0 and the element offset of each
+    // vector; chunks past m's range and padded columns read the B
+    // operand's trailing zero chunk instead (no branch)
+    const int64_t zch = t.nlpad / kChunk;
+    const int64_t cstride = (int64_t)2 * NCp * 8;  // elements per chunk
+    int64_t sbase[NV];
+    int64_t schb[NV];
+#pragma unroll
+    for (int x_ = 0; x_ < NV; ++x_) {
+      const int vi = threadIdx.x + NT * x_;
+      const int g_ = vi / (2 * Ly::cols), r2 = vi - g_ * (2 * Ly::cols);
+      const int q_ = r2 / Ly::cols, c_ = r2 - q_ * Ly::cols;
+      const int col = colg0 + c_;
+      sbase[x_] = q_ * part_stride + ((int64_t)p * NCp + (col < NCp ? col : 0)) * 8;
+      // a padded column never reads a real chunk
+      schb[x_] = col < NCp ? c0 + (int64_t)g_ * ns : cend;
+    }
+    const double *ctab = atab;
+    int cjj = 0;  // table offset (past m's start) of the coefficient at K-step 0
+    if (threadIdx.x < NC) {
+      const int g_ = threadIdx.x / 24, r2 = threadIdx.x - g_ * 24;
+      const int i_ = r2 / 3, which = r2 - i_ * 3;
+      ctab = which == 0 ? atab : (which == 1 ? btab : gtab);
+      cjj = l0 + kChunk * g_ * ns + p + 2 * i_ + 2 - m;
+    }
+    u32x4 breg[NV];
+    double creg = 0.0;
+    bool cok = false;
+#define WM_R_COEF(step) \
+  mixr_stage_coef(atab, btab, gtab, offm, m, lp, l0, ns, p, step, threadIdx.x)
 #define WM_R_CSLOT(tid) ((((tid) / 24) * 8 + ((tid) % 24) / 3) * 4 + (tid) % 3)
-#define WM_R_GLOAD(S, jg)                                                                    \
+#define WM_R_GLOAD(jg)                                                                       \
   do {                                                                                       \
     _Pragma("unroll") for (int x_ = 0; x_ < NV; ++x_) {                                      \
-      const int vi = threadIdx.x + NT * x_;                                                  \
-      u32x4 val = (u32x4){0u, 0u, 0u, 0u};                                                   \
-      if (vi < NVEC) {                                                                       \
-        const int g_ = vi / (2 * Ly::cols), r2 = vi - g_ * (2 * Ly::cols);                   \
-        const int q_ = r2 / Ly::cols, c_ = r2 - q_ * Ly::cols;                               \
-        const int64_t ch = c0 + (int64_t)g_ * ns + (jg);                                     \
-        const int col = colg0 + c_;                                                          \
-        if (ch < cend && col < NCp)                                                          \
-          val = *(const u32x4 *)(bq + q_ * part_stride + ((ch * 2 + p) * NCp + col) * 8);    \
-      }                                                                                      \
-      breg[S][x_] = val;                                                                     \
+      const int64_t ch = schb[x_] + (jg);                                                    \
+      breg[x_] = *(const u32x4 *)(bq + sbase[x_] + (ch < cend ? ch : zch) * cstride);        \
     }                                                                                        \
-    if (threadIdx.x < NC) creg[S] = WM_R_COEF((jg) + 1, threadIdx.x);                        \
+    if (threadIdx.x < NC) {                                                                  \
+      /* raw value now, zeroing at the LDS store: a select right after */                    \
+      /* the load would wait for it here */                                                  \
+      const int jj_ = cjj + kChunk * ((jg) + 1);                                             \
+      cok = jj_ < lp;                                                                        \
+      creg = ctab[offm + (cok ? jj_ : 0)];                                                   \
+    }                                                                                        \
   } while (0)
-#define WM_R_LSTORE(S, BB, CB)                                                               \
+#define WM_R_LSTORE(BB, CB)                                                                  \
   do {                                                                                       \
-    unsigned char *bd = lds + Ly::b_off + (BB) * Ly::b_buf;                                  \
-    _Pragma("unroll") for (int x_ = 0; x_ < NV; ++x_) {                                      \
-      const int vi = threadIdx.x + NT * x_;                                                  \
-      if (vi < NVEC) *(u32x4 *)(bd + vi * 16) = breg[S][x_];                                 \
-    }                                                                                        \
+    unsigned char *bd = lds + Ly::b_off + (BB) * Ly::b_buf + threadIdx.x * 16;               \
+    _Pragma("unroll") for (int x_ = 0; x_ < NV; ++x_)* (u32x4 *)(bd + NT * 16 * x_) =         \
+        breg[x_];                                                                            \
     if (threadIdx.x < NC)                                                                    \
-      ((double *)(lds + Ly::c_off + (CB) * Ly::c_buf))[WM_R_CSLOT(threadIdx.x)] = creg[S];   \
+      ((double *)(lds + Ly::c_off + (CB) * Ly::c_buf))[WM_R_CSLOT(threadIdx.x)] =            \
+          cok ? creg : 0.0;                                                                  \
   } while (0)
     // A(jk) from cbuf CB: this lane's 8 recurrence steps, the fp16 parts
     // (high part by round-toward-zero packing, low part the exact residual
@@ -1058,9 +1088,11 @@ __global__ __launch_bounds__(64 * kRWaves) void k_mixr(
       wh[i_ >> 1] = hb;                                                                      \
       wl[i_ >> 1] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(r0, r1));        \
     }                                                                                        \
+    ahr[AB] = (u32x4){wh[0], wh[1], wh[2], wh[3]};                                           \
+    alr[AB] = (u32x4){wl[0], wl[1], wl[2], wl[3]};                                           \
     unsigned char *ad = lds + Ly::a_off + (AB) * Ly::a_buf + wv * Ly::a_wave + lane * 16;    \
-    *(u32x4 *)ad = (u32x4){wh[0], wh[1], wh[2], wh[3]};                                      \
-    *(u32x4 *)(ad + 64 * 16) = (u32x4){wl[0], wl[1], wl[2], wl[3]};                          \
+    *(u32x4 *)ad = ahr[AB];                                                                  \
+    *(u32x4 *)(ad + 64 * 16) = alr[AB];                                                      \
   } while (0)
     // MFMAs of K-step j: for both waves' A fragments (abuf AB) the B
     // fragments of their 2 segments and this wave's CT column tiles (bbuf
@@ -1069,12 +1101,12 @@ __global__ __launch_bounds__(64 * kRWaves) void k_mixr(
 #define WM_R_MMA(BB, AB, FRESH)                                                              \
   do {                                                                                       \
     _Pragma("unroll") for (int kh = 0; kh < 2; ++kh) {                                       \
-      const int wsrc = kh == 0 ? wv : wpart;                                                 \
       const int csrc = kh == 0 ? chh : 1 - chh;                                              \
-      const unsigned char *as_ = lds + Ly::a_off + (AB) * Ly::a_buf + wsrc * Ly::a_wave +    \
+      const unsigned char *as_ = lds + Ly::a_off + (AB) * Ly::a_buf + wpart * Ly::a_wave +   \
                                  lane * 16;                                                  \
-      const f16x8 ah = *(const f16x8 *)as_;                                                  \
-      const f16x8 al = *(const f16x8 *)(as_ + 64 * 16);                                      \
+      const f16x8 ah = __builtin_bit_cast(f16x8, kh == 0 ? ahr[AB] : *(const u32x4 *)as_);   \
+      const f16x8 al =                                                                       \
+          __builtin_bit_cast(f16x8, kh == 0 ? alr[AB] : *(const u32x4 *)(as_ + 64 * 16));    \
       const unsigned char *bs = lds + Ly::b_off + (BB) * Ly::b_buf +                         \
                                 (2 * csrc + h) * Ly::b_seg + (chh * CT * 32 + n) * 16;       \
       f16x8 bh[CT], bl[CT];                                                                  \
@@ -1093,12 +1125,13 @@ __global__ __launch_bounds__(64 * kRWaves) void k_mixr(
   } while (0)
 
     f32x16 acc[CT];
+    u32x4 ahr[2], alr[2];  // this wave's A(j) in registers (set j % 2)
     // prologue: stage 0 and the coefficients of K-step 0 into LDS, A(0)
-    WM_R_GLOAD(0, 0);
+    WM_R_GLOAD(0);
     if (threadIdx.x < NC)
-      ((double *)(lds + Ly::c_off))[WM_R_CSLOT(threadIdx.x)] = WM_R_COEF(0, threadIdx.x);
-    WM_R_LSTORE(0, 0, 1);
-    WM_R_GLOAD(1, 1);
+      ((double *)(lds + Ly::c_off))[WM_R_CSLOT(threadIdx.x)] = WM_R_COEF(0);
+    WM_R_LSTORE(0, 1);
+    WM_R_GLOAD(1);
     __syncthreads();
     WM_R_ABUILD(0, 0, 0);
     __syncthreads();
@@ -1108,8 +1141,6 @@ __global__ __launch_bounds__(64 * kRWaves) void k_mixr(
       for (int jj = 0; jj < kRFlush; ++jj) {
         const int j = j0 + jj;
         if (j < ns) {
-          // register set jj & 1 (stage j) went to LDS before K-step j began
-          WM_R_GLOAD(jj & 1, j + 2);
           WM_R_MMA(jj & 1, jj & 1, jj == 0);
           WM_R_ABUILD(j + 1, (jj + 1) & 3, (jj + 1) & 1);
 #pragma unroll
@@ -1117,7 +1148,12 @@ __global__ __launch_bounds__(64 * kRWaves) void k_mixr(
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
             __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // then VALU
           }
-          WM_R_LSTORE((jj + 1) & 1, (jj + 1) & 1, (jj + 2) & 3);
+          // stage j + 1 (loaded at the end of K-step j - 1) into LDS, then
+          // the loads of stage j + 2 (the scheduling fence keeps the store
+          // from being hoisted above this K-step's work)
+          __builtin_amdgcn_sched_barrier(0);
+          WM_R_LSTORE((jj + 1) & 1, (jj + 2) & 3);
+          WM_R_GLOAD(j + 2);
           __syncthreads();
         }
       }
