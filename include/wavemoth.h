@@ -44,6 +44,14 @@ typedef struct _wavemoth_plan *wavemoth_plan;
 /* Extension flag: generate the per-(m, ring) Lambda seeds inside every
    execute instead of once at plan time. */
 #define WAVEMOTH_SEEDS_ON_THE_FLY 0x100
+/* Extension: run the Legendre stage from the plan's resource file (given,
+   or found under the wavemoth_configure directory) instead of generating
+   Lambda on the device: the strips' stored start rows, recurrence
+   coefficients and dense blocks, summed as src/wavemoth.c:958-1009 and
+   src/legendre_transform.c.in:680-716 do, on the GPU (FP64).  Needs a file
+   with uncompressed matrices (num_levels = 0, as wavemoth_amd.resource
+   writes); excludes WAVEMOTH_MIXED_*. */
+#define WAVEMOTH_RESOURCE_APPLY 0x400
 /* Extension flags (at most one): the Legendre contraction on the matrix
    cores, the Lambda recurrence and its seeds still in FP64.  Sums are
    accumulated in FP32 within l-chunks of 128 and reduced across chunks in
@@ -86,6 +94,23 @@ int64_t wavemoth_get_legendre_flops(wavemoth_plan plan, int m, int odd);
 
 /* replaces src/wavemoth.h:46 (impl src/wavemoth.c:144-155) */
 int wavemoth_query_resourcefile(char *filename, int *out_Nside, int *out_lmax);
+
+/* Reads and validates a rev1 resource file the way plan creation does
+   (the reader of src/wavemoth.c:203-264 and the walk of
+   src/butterfly.c.in:284-432 / src/wavemoth.c:958-1009, every offset and
+   size checked).  Extension: the reference has no validator.  Fills up to
+   ninfo of: 0 lmax, 1 mmax, 2 Nside, 3 matrices present, 4 matrices
+   missing (start 0), 5 compressed trees (heap of more than one node),
+   6 recurrence strips, 7 dense blocks and strips, 8 stored matrix elements
+   (strip rows x columns, interpolation blocks), 9 deepest tree level,
+   10 largest |x^2 - cos^2 theta| of the strips' x^2 against the HEALPix
+   rings, in units of 1e-18 (-1 without strips).  Returns 0, or -1 with
+   wavemoth_last_error set for a missing, truncated or malformed file.
+   wavemoth_plan_to_healpix reads resources the same way: the file given,
+   or <dir>/rev1/<Nside>.dat when wavemoth_configure set a directory and
+   the file exists (a missing one is not an error here: the device path
+   generates Lambda itself), with Nside < 0 taken from the file. */
+int wavemoth_resource_info(const char *filename, int64_t *info, int ninfo);
 
 /* ---- stage-wise hooks (replace src/wavemoth_private.h:132-138, used by
         wavemoth/lib.pyx:55-57,80-81,151-152,187-193) --------------------- */

@@ -64,9 +64,10 @@ def test_query_resourcefile(tmp_path):
 
 def test_resource_file_mismatch_rejected_before_device_use(tmp_path):
     from wavemoth_amd import _native
+    from wavemoth_amd.resource import write_resource
     L = _native.lib()
     fn = tmp_path / "r.dat"
-    _write_header(str(fn), 16, 16, 8)
+    write_resource(str(fn), 8, 16, 16)
     assert not L.wavemoth_plan_to_healpix(4, 8, 8, 1, 1, None, None, 0, 0, str(fn).encode())
     assert "Incompatible Nside" in _native.last_error()
     assert not L.wavemoth_plan_to_healpix(8, 8, 8, 1, 1, None, None, 0, 0, str(fn).encode())
@@ -74,6 +75,11 @@ def test_resource_file_mismatch_rejected_before_device_use(tmp_path):
     assert not L.wavemoth_plan_to_healpix(8, 16, 16, 1, 1, None, None, 0, 0,
                                           str(tmp_path / "nope.dat").encode())
     assert "Error in loading resource" in _native.last_error()
+    # a header without its offset table is malformed
+    _write_header(str(tmp_path / "h.dat"), 16, 16, 8)
+    assert not L.wavemoth_plan_to_healpix(8, 16, 16, 1, 1, None, None, 0, 0,
+                                          str(tmp_path / "h.dat").encode())
+    assert "offset table past the end" in _native.last_error()
 
 
 def test_shtplan_validation_mirrors_reference():

@@ -23,6 +23,7 @@ SIGNATURES = {
     "wavemoth_get_legendre_flops": (_c_i64, [_vp, _c_int, _c_int]),
     "wavemoth_query_resourcefile": (_c_int, [_cp, ctypes.POINTER(_c_int),
                                              ctypes.POINTER(_c_int)]),
+    "wavemoth_resource_info": (_c_int, [_cp, ctypes.POINTER(_c_i64), _c_int]),
     "wavemoth_perform_legendre_transforms": (None, [_vp]),
     "wavemoth_perform_backward_ffts": (None, [_vp]),
     "wavemoth_disable_phase_shifting": (None, [_vp]),
@@ -54,6 +55,7 @@ WAVEMOTH_ESTIMATE = 0x0
 WAVEMOTH_MEASURE = 0x1
 WAVEMOTH_NO_RESOURCE_COPY = 0x10
 WAVEMOTH_SEEDS_ON_THE_FLY = 0x100
+WAVEMOTH_RESOURCE_APPLY = 0x400
 WAVEMOTH_BATCH_NO_GRAPH = 0x200
 WAVEMOTH_MIXED_F32 = 0x1000
 WAVEMOTH_MIXED_BF16X3 = 0x2000

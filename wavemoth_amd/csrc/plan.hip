@@ -18,9 +18,12 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <unistd.h>
+#include <memory>
 
 #include "../../include/wavemoth.h"
 #include "wm_host.h"
+#include "wm_resource.h"
 
 using namespace wm;
 
@@ -69,6 +72,17 @@ struct _wavemoth_plan {
   // adjoint (extension): partial sums over ring groups, allocated on first use
   double2 *adj_partial = nullptr;
   int adj_groups = 0;
+  // rev1 resource file given or found in the configured directory: mapped
+  // and validated at plan time (the GPU path generates Lambda itself)
+  std::unique_ptr<Resource> res;
+  // WAVEMOTH_RESOURCE_APPLY: the Legendre stage runs from the file's strips
+  // (the file image and its strip table on the device)
+  bool res_apply = false;
+  double *res_img = nullptr;
+  ResStrip *res_strips = nullptr;
+  int64_t *res_prefix = nullptr;
+  int res_nstrips = 0;
+  int64_t res_items = 0;
   // stage timing (extension): events recorded around prep / sweep / rings
   bool profiling = false;
   std::vector<hipEvent_t> ev_free;
@@ -583,7 +597,12 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
                              p->mix_ckpt, s),
              "checkpoint kernel");
   }
-  if (p->mix >= 0) {
+  if (p->res_apply) {
+    if (!mark(p, 1, s)) return false;
+    WM_TRY(launch_res_legendre(p->lt, d_alm, p->res_img, p->res_strips, p->res_prefix,
+                               p->res_nstrips, p->res_items, true, s),
+           "resource Legendre kernel");
+  } else if (p->mix >= 0) {
     WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride,
                            p->mix_cexp, s),
            "mixed prescale kernel");
@@ -602,6 +621,43 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
     WM_TRY(launch_legendre(p->lt, s), "legendre kernel");
   }
   if (!mark(p, 2, s)) return false;
+  return true;
+}
+
+// WAVEMOTH_RESOURCE_APPLY: the file image and its strip table on the device.
+static bool build_resource_apply(wavemoth_plan p) {
+  if (!p->res_apply) return true;
+  const Resource &r = *p->res;
+  std::vector<ResStrip> st;
+  std::vector<int64_t> prefix;
+  int64_t items = 0;
+  for (const ResourceStrip &x : r.strips) {
+    ResStrip d;
+    d.m = x.m;
+    d.odd = x.odd;
+    d.nks = (int)(x.nk - x.rstart);
+    d.cstart = (int)x.cstart;
+    d.nx = (int)(x.cstop - x.cstart);
+    d.row = x.row0 + x.rstart;
+    if (x.x2_off >= 0) {
+      d.x2 = x.x2_off / 8;
+      d.p0 = x.p0_off / 8;
+      d.p1 = x.p1_off / 8;
+      d.aux = x.aux_off / 8 + 3 * x.rstart;  // the strip's first recurrence step
+    } else {
+      d.dense = x.dense_off / 8;
+    }
+    prefix.push_back(items);
+    items += d.nx;
+    st.push_back(d);
+  }
+  p->res_nstrips = (int)st.size();
+  p->res_items = items;
+  if (st.empty()) return true;
+  if (!dalloc(p->arena, &p->res_img, r.size / 8 + 1, "resource image")) return false;
+  WM_TRY(hipMemcpy(p->res_img, r.base, r.size, hipMemcpyHostToDevice), "resource upload");
+  if (!upload(p->arena, &p->res_strips, st, "resource strips")) return false;
+  if (!upload(p->arena, &p->res_prefix, prefix, "resource strip prefix")) return false;
   return true;
 }
 
@@ -634,6 +690,22 @@ int wavemoth_query_resourcefile(char *filename, int *out_Nside, int *out_lmax) {
   return 0;
 }
 
+int wavemoth_resource_info(const char *filename, int64_t *info, int ninfo) {
+  g_err.clear();
+  if (!filename || (ninfo > 0 && !info)) {
+    set_err("NULL argument");
+    return -1;
+  }
+  std::unique_ptr<Resource> r(resource_open(filename));
+  if (!r) return -1;
+  const double dev = resource_x2_deviation(*r);
+  const int64_t v[11] = {r->lmax, r->mmax, r->Nside, r->present, r->missing, r->trees,
+                         r->recurrence_strips, r->dense_blocks, r->elements, r->max_depth,
+                         dev < 0 ? -1 : (int64_t)std::ceil(dev * 1e18)};
+  for (int i = 0; i < ninfo && i < 11; ++i) info[i] = v[i];
+  return 0;
+}
+
 wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps, int nthreads,
                                        double *input, double *output, int ordering,
                                        unsigned flags, char *resource_filename) {
@@ -646,6 +718,30 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     set_err("only WAVEMOTH_MMAJOR ordering is supported");
     return nullptr;
   }
+  // Resource: the file given, else <dir>/rev1/<Nside>.dat when a directory
+  // is configured and the file exists (src/wavemoth.c:556-569).  It is
+  // validated completely before any device work; Nside < 0 takes the
+  // file's.  Without one the plan needs none (Lambda is generated on the
+  // device), where the reference would abort.
+  Resource *res = nullptr;
+  if (resource_filename) {
+    res = resource_open(resource_filename);
+    if (!res) return nullptr;
+  } else if (Nside < 0) {
+    set_err("Invalid Nside");
+    return nullptr;
+  } else if (!g_resource_dir.empty()) {
+    const std::string path = resource_default_path(g_resource_dir, Nside);
+    if (access(path.c_str(), F_OK) == 0) {
+      res = resource_open(path.c_str());
+      if (!res) return nullptr;
+    }
+  }
+  if (res && !resource_matches(*res, &Nside, lmax, mmax)) {
+    delete res;
+    return nullptr;
+  }
+  std::unique_ptr<Resource> res_owner(res);
   if (Nside < 1 || Nside > 2048) {
     set_err("Nside must be in [1, 2048]");
     return nullptr;
@@ -662,33 +758,8 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     set_err("lmax too large");
     return nullptr;
   }
-  if (resource_filename) {
-    FILE *fd = std::fopen(resource_filename, "rb");
-    if (!fd) {
-      set_err(std::string("Error in loading resource ") + resource_filename);
-      return nullptr;
-    }
-    int64_t hdr[3];
-    const size_t got = std::fread(hdr, sizeof(hdr), 1, fd);
-    std::fclose(fd);
-    if (got != 1) {
-      set_err(std::string("could not read resource header: ") + resource_filename);
-      return nullptr;
-    }
-    if (hdr[2] != Nside) {
-      set_err("Incompatible Nside");
-      return nullptr;
-    }
-    if (hdr[1] != mmax) {
-      set_err("Incompatible mmax");
-      return nullptr;
-    }
-    if (hdr[0] != lmax) {
-      set_err("Incompatible lmax");
-      return nullptr;
-    }
-  }
   wavemoth_plan p = new _wavemoth_plan();
+  p->res = std::move(res_owner);
   p->N = Nside;
   p->L = lmax;
   p->M = mmax;
@@ -716,6 +787,21 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
       return nullptr;
     }
   }
+  if (flags & WAVEMOTH_RESOURCE_APPLY) {
+    const char *why = !p->res ? "WAVEMOTH_RESOURCE_APPLY needs a resource file"
+                      : p->mix >= 0
+                          ? "WAVEMOTH_RESOURCE_APPLY runs in FP64: no WAVEMOTH_MIXED_* flag"
+                      : (p->res->trees || p->res->max_depth)
+                          ? "WAVEMOTH_RESOURCE_APPLY: the file holds butterfly-compressed "
+                            "matrices, which the device applies only uncompressed"
+                          : nullptr;
+    if (why) {
+      set_err(why);
+      delete p;
+      return nullptr;
+    }
+    p->res_apply = true;
+  }
   p->nalm = 0;
   for (int m = 0; m <= mmax; ++m) p->nalm += lmax - m + 1;
   if (hipGetDevice(&p->device) != hipSuccess) {
@@ -728,7 +814,7 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     delete p;
     return nullptr;
   }
-  if (!build_legendre_tables(p) || !build_ring_tables(p)) {
+  if (!build_legendre_tables(p) || !build_ring_tables(p) || !build_resource_apply(p)) {
     std::string e = g_err;
     free_plan(p);
     g_err = e;

@@ -25,6 +25,7 @@
 
 #include "wavemoth.h"
 #include "wm_host.h"
+#include "wm_internal.h"
 
 namespace wm {
 
@@ -78,6 +79,95 @@ static hipError_t launch_strip(size_t nx, size_t nk, size_t nvecs, const double 
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_strip_transform, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                      (int64_t)nx, (int64_t)nk, (int64_t)nvecs, a, y, x2, aux, P, Pp1);
+  return hipGetLastError();
+}
+
+// The Legendre stage from a resource file's strips: thread (item, map
+// block) with item -> (strip, ring pair) by binary search over the strips'
+// prefix column counts.  It sums P_k(x) a_{l_k} over the strip's rows for
+// up to 8 maps, P_k from the stored start rows by the x^2 recurrence
+// (src/legendre_transform.c.in:680-716) or read from a dense block, which
+// is what src/wavemoth.c:958-1009 (pull_a_through_legendre_block) feeds its
+// kernels; rows before the strip's first are zero there and here.
+__global__ __launch_bounds__(256) void k_res_legendre(LegTables t, const double2 *__restrict__ alm,
+                                                      const double *__restrict__ img,
+                                                      const ResStrip *__restrict__ strips,
+                                                      const int64_t *__restrict__ prefix,
+                                                      int nstrips, int64_t items) {
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= items) return;
+  const int K = t.K, L = t.L, Mp1 = t.M + 1;
+  const int kb0 = blockIdx.y * 8;
+  const int kn = (K - kb0) < 8 ? (K - kb0) : 8;
+  int lo = 0, hi = nstrips - 1;  // last strip with prefix <= it
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  const ResStrip s = strips[lo];
+  const int c = (int)(it - prefix[lo]);
+  const int m = s.m;
+  const int64_t base = (int64_t)m * (2 * L - m + 3) / 2;  // a_lm row of (l = m, m)
+  const int64_t l0 = (int64_t)s.odd + 2 * s.row;          // l - m of the first row
+  double2 acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = make_double2(0.0, 0.0);
+  // (arguments evaluated once, before the loop over maps)
+#define WM_RES_ADD(row, P)                                                    \
+  do {                                                                        \
+    const double p_ = (P);                                                    \
+    const double2 *a_ = alm + (base + l0 + 2 * (int64_t)(row)) * K + kb0;    \
+    _Pragma("unroll") for (int kk_ = 0; kk_ < 8; ++kk_) {                     \
+      if (kk_ < kn) {                                                         \
+        const double2 v_ = a_[kk_];                                           \
+        acc[kk_].x = fma(p_, v_.x, acc[kk_].x);                               \
+        acc[kk_].y = fma(p_, v_.y, acc[kk_].y);                               \
+      }                                                                       \
+    }                                                                         \
+  } while (0)
+  if (s.x2 < 0) {
+    const double *D = img + s.dense + (int64_t)s.nks * c;
+    for (int k = 0; k < s.nks; ++k) WM_RES_ADD(k, D[k]);
+  } else {
+    const double x2 = img[s.x2 + c];
+    double pp = img[s.p0 + c], p = img[s.p1 + c];
+    WM_RES_ADD(0, pp);
+    WM_RES_ADD(1, p);
+    const double *aux = img + s.aux;
+    for (int k = 2; k < s.nks; ++k) {
+      const double pv = (x2 + aux[3 * (k - 2)]) * aux[3 * (k - 2) + 1] * p +
+                        aux[3 * (k - 2) + 2] * pp;
+      pp = p;
+      p = pv;
+      WM_RES_ADD(k, pv);
+    }
+  }
+#undef WM_RES_ADD
+  const int ring = s.cstart + c;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < kn) t.qbuf[q_index(ring, s.odd, kb0 + k, m, K, Mp1)] = acc[k];
+}
+
+__global__ __launch_bounds__(256) void k_zero_q(double2 *q, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = make_double2(0.0, 0.0);
+}
+
+hipError_t launch_res_legendre(const LegTables &t, const double2 *alm, const double *img,
+                               const ResStrip *strips, const int64_t *prefix, int nstrips,
+                               int64_t items, bool zero_q, hipStream_t s) {
+  if (zero_q) {
+    const int64_t n = q_elems(t.R, t.K, t.M + 1);
+    hipLaunchKernelGGL(k_zero_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t.qbuf, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (items == 0) return hipSuccess;
+  const dim3 grid((unsigned)((items + 255) / 256), (unsigned)((t.K + 7) / 8));
+  hipLaunchKernelGGL(k_res_legendre, grid, dim3(256), 0, s, t, alm, img, strips, prefix, nstrips,
+                     items);
   return hipGetLastError();
 }
 

@@ -208,6 +208,24 @@ int legendre_map_block(int K);
 int legendre_default_rl(int K);
 int legendre_padded_maps(int K);
 
+// One strip of a resource matrix on the device (the Legendre stage read
+// from a rev1 file): rows `row`.. of (m, odd) (l = m + odd + 2 row), nks of
+// them, ring pairs [cstart, cstart + nx); offsets in doubles into the file
+// image on the device: a recurrence strip has x2 / p0 / p1 (nx each) and
+// aux (3 per row past the second, of the strip's rows), a dense one a
+// column-major nks x nx block at dense (x2 = -1).
+struct ResStrip {
+  int m = 0, odd = 0, nks = 0, cstart = 0, nx = 0;
+  int64_t row = 0;
+  int64_t x2 = -1, p0 = -1, p1 = -1, aux = -1, dense = -1;
+};
+// q <- the Legendre stage of the strips (FP64 q; items = prefix[nstrips]
+// (ring pair, strip) pairs); (m, odd) without a matrix are zeroed first
+// when zero_q is set.
+hipError_t launch_res_legendre(const LegTables &t, const double2 *alm, const double *img,
+                               const ResStrip *strips, const int64_t *prefix, int nstrips,
+                               int64_t items, bool zero_q, hipStream_t s);
+
 // Launchers (return hipError_t of the launch).
 hipError_t launch_seeds(const LegTables &t, hipStream_t s);
 hipError_t launch_wave_bounds(const LegTables &t, hipStream_t s);
