@@ -485,6 +485,11 @@ def run_mixed(args, dev, res):
     m = schemes[main_key]
     nprod = SCHEMES[main_key][3]
     kidx = ["f32", "bf16x3", "bf16x6", "f16x3"].index(main_key)
+    ncp = (2 * K + 31) // 32 * 32
+    # the contraction kernel the plan launches (FP16x3: Lambda built in
+    # registers, k_mixr<CT>; the others: producer / consumer k_mix<S, CB>)
+    kname = ("k_mixr<%d>" % (1 if ncp <= 64 else 2) if main_key == "f16x3"
+             else "k_mix<%d, %d>" % (kidx, min(ncp // 32, 4)))
     res.update({
         "metric": "maps/s synthesis Nside=%d lmax=%d: FP64 vs batched FP32-MFMA @<=1e-6 err "
                   "(value: %s contraction, the fastest scheme within 1e-6)" % (N, L, m["label"]),
@@ -507,10 +512,10 @@ def run_mixed(args, dev, res):
             "frac": m["frac"],
             "executed_product_tflops": m["executed_product_tflops"],
             "executed_product_frac": m["executed_product_frac"],
-            "traffic": measured_traffic("k_mix<%d> nside=%d lmax=%d nmaps=%d" % (kidx, N, L, K)),
+            "traffic": measured_traffic("%s nside=%d lmax=%d nmaps=%d" % (kname, N, L, K)),
             "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                               "x2.0 / x1.0 from a 1 GiB copy)",
-            "kernel": "k_mix<%d>" % kidx,
+            "kernel": kname,
             "flops_per_launch": 4.0 * K * T_eps,
             "flop_convention": "4*nmaps per (l, m, ring-pair) triple at or after first l (the "
                                "contraction's algorithmic flops, priced against the raw dense "
