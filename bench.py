@@ -33,8 +33,9 @@ mixed the headline scheme's contraction kernel: its algorithmic flops
 executes (x1 FP32, x3 FP16x3 / BF16x3, x6 BF16x6) is a separate field, and
 `ring_stage` prices the ring FFT stage against HBM bandwidth.  Kernel
 durations are HIP events recorded on the execute stream
-around the kernel.  `cpu_baseline` times the CPU oracle (a C restatement of
-the reference algorithm, OpenMP) on a bounded sample of the same workload.
+around the kernel.  `cpu_baseline` times a vectorised C restatement of the
+reference's CPU path (oracle/sht_fast.c, OpenMP; the oracle's plan) on a
+bounded sample of the same workload, min over 10 repetitions.
 --calibrate-copy MIB runs a device copy of that size first (untimed), as
 the known-size kernel against which HBM byte counters are scaled.
 """
@@ -70,77 +71,127 @@ def parse():
     ap.add_argument("--fly", action="store_true",
                     help="generate the Lambda seeds inside every execute")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-m-stride", type=int, default=32,
-                    help="CPU baseline samples every k-th m")
+    ap.add_argument("--cpu-m-stride", type=int, default=None,
+                    help="CPU baseline: Legendre stage on every k-th m (default 16 for the "
+                         "mixed workload, 1 = all m for fp64)")
+    ap.add_argument("--cpu-ring-stride", type=int, default=None,
+                    help="CPU baseline: ring stage on every k-th ring pair (default 16 for "
+                         "the mixed workload, 1 for fp64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--seed", type=int, default=20482048)
     return ap.parse_args()
 
 
-def cpu_baseline(N, L, K, alm, stride):
-    """Time the oracle: Legendre stage on m = 0, stride, 2 stride, ...
-    (extrapolated by (mmax+1)/#sampled), ring stage on the full map."""
-    from oracle import oracle as O
-    M = L
-    op = O.OraclePlan(N, L, M, K, m_start=0, m_stop=M + 1, m_step=stride)
-    nsamp = len(range(0, M + 1, stride))
-    t0 = time.perf_counter()
-    work = op.legendre_stage(alm)
-    t_leg = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    spec = op.ring_spectra(work)
-    op.c2r(spec)
-    t_ring = time.perf_counter() - t0
-    t_leg_full = t_leg * (M + 1) / nsamp
-    total = t_leg_full + t_ring
-    return {
-        "value": K / total,
-        "unit": "maps/s",
-        "cores": O.num_threads(),
-        "kind": "port",
-        "sample": ("oracle (C restatement of the reference's seeds + x^2 recurrence + "
-                   "ring stage, OpenMP over m/rings, numpy c2r): Legendre stage timed on "
-                   "%d of %d m values (every %d-th), %.3f s, scaled by %d/%d; ring stage "
-                   "timed on the full map, %.3f s; seed precompute excluded as the "
-                   "reference's resource precompute is" % (nsamp, M + 1, stride, t_leg,
-                                                           M + 1, nsamp, t_ring)),
-        "legendre_s": t_leg_full,
-        "ring_s": t_ring,
-    }
+def host_info():
+    """The host the CPU baseline ran on: CPU model, logical CPUs, physical
+    cores (distinct (package, core) pairs), NUMA nodes and the CPUs this
+    process may run on."""
+    info = {"cpu": None, "logical_cpus": os.cpu_count(), "physical_cores": None,
+            "numa_nodes": None, "affinity_cpus": None}
+    try:
+        cores, pkg, core = set(), None, None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and info["cpu"] is None:
+                    info["cpu"] = v
+                elif k == "physical id":
+                    pkg = v
+                elif k == "core id":
+                    core = v
+                elif not k and pkg is not None:
+                    cores.add((pkg, core))
+                    pkg = core = None
+        if pkg is not None:
+            cores.add((pkg, core))
+        info["physical_cores"] = len(cores) or None
+    except OSError:
+        pass
+    try:
+        nodes = [d for d in os.listdir("/sys/devices/system/node") if d.startswith("node")]
+        info["numa_nodes"] = len(nodes)
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
 
 
-def cpu_baseline_mixed(N, L, K, alm, stride):
-    """Oracle on a bounded sample of the nmaps = K workload: Legendre stage
-    at nmaps = K on every `stride`-th m (extrapolated), ring stage timed at
-    nmaps = 1 on the full map and multiplied by K (it is independent per map)."""
+def min_time(fn, reps):
+    """The reference's timing rule (bench/benchutils.h:113-137): one untimed
+    warm-up call, then `reps` timed calls; (min, mean) seconds."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return min(ts), sum(ts) / len(ts)
+
+
+def cpu_baseline(N, L, K, alm, m_stride, ring_stride, reps=10):
+    """CPU baseline on a bounded sample of the nmaps = K workload, built from
+    the oracle's plan (its precompute stands for the reference's resource
+    files and is not timed, as the reference's plan is not):
+      - Legendre stage: the vectorised restatement of the reference's SIMD
+        kernel (oracle/sht_fast.c) on m = 0, m_stride, ... at nmaps = K,
+        scaled to all m by the dense (l, m) count;
+      - ring stage: fold + per-ring-pair c2r of all K maps (pocketfft for
+        FFTW) on ring pairs 0, ring_stride, ..., scaled by the pair count.
+    Each leg: one warm-up, then the minimum over `reps` repetitions."""
     import numpy as np
     from oracle import oracle as O
     M = L
-    op = O.OraclePlan(N, L, M, K, m_start=0, m_stop=M + 1, m_step=stride)
-    nsamp = len(range(0, M + 1, stride))
     t0 = time.perf_counter()
-    op.legendre_stage(alm, compact=True)
-    t_leg = time.perf_counter() - t0
-    del op
-    op1 = O.OraclePlan(N, L, M, 1, m_start=0, m_stop=1, m_step=1)
-    work = np.zeros((M + 1, 2, 2 * N, 1), np.complex128)
-    t0 = time.perf_counter()
-    op1.c2r(op1.ring_spectra(work))
-    t_ring1 = time.perf_counter() - t0
-    total = t_leg * (M + 1) / nsamp + t_ring1 * K
+    op = O.OraclePlan(N, L, M, K, m_start=0, m_stop=M + 1, m_step=m_stride)
+    t_plan = time.perf_counter() - t0
+    ms = range(0, M + 1, m_stride)
+    dense_s = sum(L - m + 1 for m in ms)
+    dense_all = sum(L - m + 1 for m in range(M + 1))
+    threads = O.num_threads()
+    work = np.zeros((len(ms), 2, op.R, K), np.complex128)
+    width = [0]
+
+    def leg():
+        width[0] = op.legendre_stage_fast(alm, compact=True, work=work)[1]
+
+    t_leg = min_time(leg, reps)
+    T_s = op.trimmed_triples()
+    del work
+    rs = range(0, op.R, ring_stride)
+    work_s = np.full((M + 1, 2, len(rs), K), 0.5 + 0.25j)
+    t_ring = min_time(lambda: op.ring_stage_sample(work_s, ring_stride, workers=threads), reps)
+    del work_s
+    leg_full = t_leg[0] * dense_all / dense_s
+    ring_full = t_ring[0] * op.R / len(rs)
+    isa = {8: "AVX-512F", 4: "AVX2+FMA"}[width[0]]
     return {
-        "value": K / total,
+        "value": K / (leg_full + ring_full),
         "unit": "maps/s",
-        "cores": O.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": ("oracle (C restatement of the reference's seeds + x^2 recurrence, "
-                   "OpenMP): Legendre stage at nmaps=%d timed on %d of %d m values (every "
-                   "%d-th), %.3f s, scaled by %d/%d; ring stage (fold, phase, numpy c2r) "
-                   "timed at nmaps=1 on the full map (zero work array; its cost does not "
-                   "depend on the data), %.3f s, times %d maps" %
-                   (K, nsamp, M + 1, stride, t_leg, M + 1, nsamp, t_ring1, K)),
-        "legendre_s": t_leg * (M + 1) / nsamp,
-        "ring_s": t_ring1 * K,
+        "sample": ("oracle/sht_fast.c (%s, %d OpenMP threads): the reference's SIMD Legendre "
+                   "kernel restated (strips of rings sharing one recurrence, row blocks in "
+                   "L1, register-blocked product) at nmaps=%d on %d of %d m values (every "
+                   "%d-th), min %.3f s over %d repetitions after a warm-up, scaled by the "
+                   "dense (l, m) count %d/%d; ring stage (fold, phase, pocketfft c2r of all "
+                   "%d maps per ring, %d threads) on %d of %d ring pairs (every %d-th), min "
+                   "%.3f s, scaled by %d/%d; plan precompute (%.1f s) not timed, as the "
+                   "reference's resource precompute is not" %
+                   (isa, threads, K, len(ms), M + 1, m_stride, t_leg[0], reps, dense_all,
+                    dense_s, K, threads, len(rs), op.R, ring_stride, t_ring[0], op.R, len(rs),
+                    t_plan)),
+        "legendre_s": leg_full,
+        "ring_s": ring_full,
+        "legendre_sample_s": {"min": t_leg[0], "mean": t_leg[1]},
+        "ring_sample_s": {"min": t_ring[0], "mean": t_ring[1]},
+        "legendre_gflops": T_s * (5 + 4 * K) / t_leg[0] / 1e9,
+        "flop_convention": "(5 + 4*nmaps) per (l, m, ring-pair) triple at or after first l",
+        "isa": isa,
+        "host": host_info(),
     }
 
 
@@ -247,7 +298,8 @@ def run_fp64(args, dev, res):
         "output_finite": bool(torch.isfinite(out).all().item()),
     })
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(N, L, K, alm_h, args.cpu_m_stride)
+        res["cpu_baseline"] = cpu_baseline(N, L, K, alm_h, args.cpu_m_stride or 1,
+                                           args.cpu_ring_stride or 1)
 
 
 def run_grad(args, dev, res):
@@ -533,7 +585,8 @@ def run_mixed(args, dev, res):
     })
     if not args.no_cpu_baseline:
         a_h = alms["powerlaw"].cpu().numpy()
-        res["cpu_baseline"] = cpu_baseline_mixed(N, L, K, a_h, args.cpu_m_stride * 2)
+        res["cpu_baseline"] = cpu_baseline(N, L, K, a_h, args.cpu_m_stride or 16,
+                                           args.cpu_ring_stride or 16)
 
 
 def main():

@@ -60,6 +60,12 @@ def lib():
         L.or_plan_free.argtypes = [ctypes.c_void_p]
         L.or_legendre_stage.argtypes = [ctypes.c_void_p, _d, _d, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_legendre_stage_fast.argtypes = [ctypes.c_void_p, _d, _d] + [ctypes.c_int] * 5
+        L.or_legendre_stage_fast.restype = ctypes.c_int
+        L.or_fast_width.restype = ctypes.c_int
+        L.or_ring_fold_sample.argtypes = [ctypes.c_void_p, _d, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _i64, _d, ctypes.c_int]
+        L.or_ring_fold_sample.restype = ctypes.c_int
         L.or_ring_spectra.argtypes = [ctypes.c_void_p, _d, ctypes.c_int, _i64, _d]
         L.or_plan_trimmed_triples.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int]
@@ -227,6 +233,23 @@ class OraclePlan:
                                 _ptr(work.view(np.float64)), *self.m_range, int(bool(compact)))
         return work
 
+    def legendre_stage_fast(self, alm, compact=False, width=0, work=None):
+        """legendre_stage through the vectorised kernel of sht_fast.c (the CPU
+        baseline's timed leg) at vector width 4 (AVX2) or 8 (AVX-512), 0 for
+        the widest the CPU has.  Returns (work, width run)."""
+        alm = np.ascontiguousarray(alm, np.complex128)
+        assert alm.shape == (nalm(self.lmax, self.mmax), self.nmaps)
+        nm = len(range(*self.m_range)) if compact else self.mmax + 1
+        if work is None:
+            work = np.zeros((nm, 2, self.R, self.nmaps), np.complex128)
+        assert work.shape == (nm, 2, self.R, self.nmaps) and work.flags.c_contiguous
+        w = lib().or_legendre_stage_fast(self._p, _ptr(alm.view(np.float64)),
+                                         _ptr(work.view(np.float64)), *self.m_range,
+                                         int(bool(compact)), int(width))
+        if w == 0:
+            raise RuntimeError("vector width %d not supported by this CPU" % width)
+        return work, w
+
     def ring_spectra(self, work, phase_shifts=True):
         work = np.ascontiguousarray(work, np.complex128)
         spec = np.zeros((self.nspec, self.nmaps), np.complex128)
@@ -251,9 +274,69 @@ class OraclePlan:
                 out[o:o + n] = f[j]
         return out
 
+    def ring_stage_sample(self, work_s, r_step, workers=1, phase_shifts=True, width=0):
+        """Ring stage of the CPU baseline on the ring pairs r = 0, r_step, ...
+        of the northern half and their mirrors, all maps at once: the fold of
+        sht_fast.c, then per pair one pocketfft c2r over its top and bottom
+        rings and every map (the reference's plan_many c2r with howmany =
+        nmaps, src/wavemoth.c:828-831; the fold writes each ring map-major so
+        that the c2r runs over contiguous bins), pairs spread over `workers`
+        threads.
+        work_s: work[m, odd, s, map] of the sampled rings.  Returns
+        (rings, pixels): the physical ring indices, top then bottom per pair,
+        and their pixels (sum of their lengths, nmaps), in that order."""
+        import scipy.fft as sf
+        from concurrent.futures import ThreadPoolExecutor
+        work_s = np.ascontiguousarray(work_s, np.complex128)
+        K = self.nmaps
+        nsamp = work_s.shape[2]
+        assert work_s.shape == (self.mmax + 1, 2, nsamp, K)
+        assert nsamp == len(range(0, self.R, r_step))
+        mid = 2 * self.Nside - 1
+        soff, poff, rings, tot, npx = [], [], [], 0, 0
+        for s in range(nsamp):
+            r = s * r_step
+            n = int(self.counts[mid + r])
+            cnt = 2 if r > 0 else 1
+            soff.append(tot)
+            poff.append(npx)
+            rings += [mid - r, mid + r] if r > 0 else [mid]
+            tot += cnt * (n // 2 + 1)
+            npx += cnt * n
+        soff = np.array(soff, np.int64)
+        spec = np.empty((tot, K), np.complex128)
+        w = lib().or_ring_fold_sample(self._p, _ptr(work_s.view(np.float64)),
+                                      int(bool(phase_shifts)), int(r_step), nsamp,
+                                      _ptr(soff, _i64), _ptr(spec.view(np.float64)), int(width))
+        if w == 0:
+            raise RuntimeError("vector width %d not supported by this CPU" % width)
+        out = np.empty((npx, K))
+
+        def one(s):
+            r = s * r_step
+            n = int(self.counts[mid + r])
+            nb = n // 2 + 1
+            cnt = 2 if r > 0 else 1
+            X = spec[soff[s]:soff[s] + cnt * nb].reshape(cnt, K, nb)
+            out[poff[s]:poff[s] + cnt * n].reshape(cnt, n, K)[...] = sf.irfft(
+                X, n, axis=2, norm="forward").transpose(0, 2, 1)
+
+        if workers > 1:
+            with ThreadPoolExecutor(workers) as ex:
+                list(ex.map(one, range(nsamp)))
+        else:
+            for s in range(nsamp):
+                one(s)
+        return rings, out
+
     def synthesize(self, alm, phase_shifts=True):
         work = self.legendre_stage(alm)
         return self.c2r(self.ring_spectra(work, phase_shifts))
+
+
+def fast_width():
+    """Widest vector width (doubles) sht_fast.c can run on this CPU, 0 if none."""
+    return lib().or_fast_width()
 
 
 def num_threads():

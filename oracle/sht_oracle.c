@@ -287,24 +287,7 @@ void or_legendre_transform(int64_t nx, int64_t nk, int64_t nvecs, const double *
 
 /* --------------------------------------- plan: precompute + execute */
 
-typedef struct {
-  int N, lmax, mmax, nmaps, R;
-  double *z;      /* [R] */
-  double *x2;     /* [R] */
-  double *phi0s;  /* [4N-1] */
-  int64_t *ring_offsets; /* [4N] */
-  /* per (m, odd, ring): first row k0 used, and up to 4 values:
-     P0, P1 (recurrence) or the dense Lambda rows when <= 4 rows remain. */
-  int32_t *k0;
-  double *seed;   /* [(M+1)*2*R*4] */
-  double *aux;    /* per (m, odd): 3*(nk-2) entries, offsets in aux_off */
-  int64_t *aux_off;
-} or_plan;
-
-static int64_t or_nk(int L, int m, int odd) {
-  int d = L - m - odd;
-  return d < 0 ? 0 : (int64_t)(d / 2 + 1);
-}
+#include "sht_oracle.h"
 
 /* Precompute the per-(m, odd, ring) seeds: the restated resource file.
    Only m in [m_start, m_stop) with stride m_step are filled (bounded
@@ -509,4 +492,45 @@ int or_num_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* Vector width the CPU supports for the vectorised Legendre stage: 8
+   (AVX-512F), 4 (AVX2 + FMA) or 0 (neither). */
+int or_fast_width(void) {
+  __builtin_cpu_init();
+  if (__builtin_cpu_supports("avx512f")) return 8;
+  if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) return 4;
+  return 0;
+}
+
+/* The vectorised Legendre stage of sht_fast.c at the given width (0: the
+   widest the CPU has).  Returns the width run, or 0 if none is available. */
+int or_legendre_stage_fast(const or_plan *p, const double *alm, double *work, int m_start,
+                           int m_stop, int m_step, int compact, int width) {
+  int have = or_fast_width();
+  if (width == 0) width = have;
+  if (width > have) return 0;
+  if (width == 8)
+    or_legendre_stage_fast_w8(p, alm, work, m_start, m_stop, m_step, compact);
+  else if (width == 4)
+    or_legendre_stage_fast_w4(p, alm, work, m_start, m_stop, m_step, compact);
+  else
+    return 0;
+  return width;
+}
+
+/* The sampled ring-stage fold of sht_fast.c at the given width (0: the
+   widest the CPU has).  Returns the width run, or 0 if none is available. */
+int or_ring_fold_sample(const or_plan *p, const double *work, int phase_shifts, int r_step,
+                        int nsamp, const int64_t *soff, double *spec, int width) {
+  int have = or_fast_width();
+  if (width == 0) width = have;
+  if (width > have) return 0;
+  if (width == 8)
+    or_ring_fold_sample_w8(p, work, phase_shifts, r_step, nsamp, soff, spec);
+  else if (width == 4)
+    or_ring_fold_sample_w4(p, work, phase_shifts, r_step, nsamp, soff, spec);
+  else
+    return 0;
+  return width;
 }
