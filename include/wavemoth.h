@@ -48,9 +48,10 @@ typedef struct _wavemoth_plan *wavemoth_plan;
    or found under the wavemoth_configure directory) instead of generating
    Lambda on the device: the strips' stored start rows, recurrence
    coefficients and dense blocks, summed as src/wavemoth.c:958-1009 and
-   src/legendre_transform.c.in:680-716 do, on the GPU (FP64).  Needs a file
-   with uncompressed matrices (num_levels = 0, as wavemoth_amd.resource
-   writes); excludes WAVEMOTH_MIXED_*. */
+   src/legendre_transform.c.in:680-716 do, and for butterfly-compressed
+   matrices the transposed interpolations of their trees
+   (src/butterfly.c.in:284-432), on the GPU (FP64).  Excludes
+   WAVEMOTH_MIXED_*. */
 #define WAVEMOTH_RESOURCE_APPLY 0x400
 /* Extension flags (at most one): the Legendre contraction on the matrix
    cores, the Lambda recurrence and its seeds still in FP64.  Sums are

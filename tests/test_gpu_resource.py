@@ -6,6 +6,10 @@
   (the reference's pull_a_through_legendre_block, src/wavemoth.c:958-1009):
   its work buffer against the oracle's Legendre stage and its map against
   the oracle and the on-the-fly FP64 path, to 1e-12 rel rms;
+- with butterfly-compressed matrices (src/butterfly.c.in:284-432) the
+  device applies the trees: work against the independent numpy walk of the
+  same file to 1e-12 and against the oracle to the compression precision
+  (1e-9 at ID precision 1e-12), map against the on-the-fly path;
 - the flag's preconditions are refused with a message."""
 import os
 
@@ -86,7 +90,6 @@ def test_resource_given_or_configured_leaves_map_unchanged(tmp_path):
 
 def test_resource_apply_preconditions(tmp_path):
     from wavemoth_amd import _native
-    from test_resource import _tree_file
     lib = _native.lib()
     assert not lib.wavemoth_plan_to_healpix(8, 16, 16, 1, 1, None, None, 0, APPLY, None)
     assert "needs a resource file" in _native.last_error()
@@ -94,7 +97,34 @@ def test_resource_apply_preconditions(tmp_path):
     assert not lib.wavemoth_plan_to_healpix(8, 16, 16, 16, 1, None, None, 0, APPLY | 0x1000,
                                             fn.encode())
     assert "no WAVEMOTH_MIXED" in _native.last_error()
-    tree = str(tmp_path / "tree.dat")
-    _tree_file(tree, 1)
-    assert not lib.wavemoth_plan_to_healpix(1, 0, 0, 1, 1, None, None, 0, APPLY, tree.encode())
-    assert "butterfly-compressed" in _native.last_error()
+
+
+@pytest.mark.parametrize("N,L,M,K,cs,levels", [
+    (8, 16, 16, 2, 2, 1),
+    (16, 32, 32, 3, 4, 2),
+    (16, 32, 24, 1, 4, 3),      # full depth: leaves under every root
+    (32, 64, 64, 2, 8, 2),      # forest: 2 roots per matrix
+    (64, 128, 100, 9, 16, 2),   # recurrence strips in the remainders; 9 maps (two map blocks)
+])
+def test_compressed_resource_apply(tmp_path, oracle, N, L, M, K, cs, levels):
+    import wavemoth_amd as wm
+    from wavemoth_amd.resource import write_resource
+    from test_resource_tree import _work_from_tree_file
+    fn = str(tmp_path / "c.dat")
+    write_resource(fn, N, L, M, chunk_size=cs, eps=1e-12, num_levels=levels)
+    alm = wm.power_law_alm(L, M, K, 40 + N)
+    print("case compressed apply N=%d L=%d M=%d K=%d chunk=%d levels=%d" % (N, L, M, K, cs,
+                                                                           levels), flush=True)
+    out = np.zeros((12 * N * N, K))
+    p = wm.ShtPlan(N, L, M, alm, out, 'mmajor', matrix_data_filename=fn, flags=APPLY)
+    got = p.execute().copy()
+    work = p.get_work()
+    tree_work, _ = _work_from_tree_file(fn, alm)
+    ref_work = oracle.OraclePlan(N, L, M, K).legendre_stage(alm)
+    et = np.linalg.norm(work - tree_work) / np.linalg.norm(tree_work)
+    ew = np.linalg.norm(work - ref_work) / np.linalg.norm(ref_work)
+    fly = wm.ShtPlan(N, L, M, alm, np.zeros_like(out), 'mmajor').execute()
+    ef = np.linalg.norm(got - fly) / np.linalg.norm(fly)
+    print("work vs numpy tree walk %.3e  vs oracle %.3e  map vs on-the-fly %.3e" % (et, ew, ef),
+          flush=True)
+    assert et <= 1e-12 and ew <= 1e-9 and ef <= 1e-9

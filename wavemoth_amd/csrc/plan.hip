@@ -83,6 +83,16 @@ struct _wavemoth_plan {
   int64_t *res_prefix = nullptr;
   int res_nstrips = 0;
   int64_t res_items = 0;
+  // compressed matrices: the chunk buffer of the interpolations' blocks,
+  // the interpolations sorted by tree depth (ops res_depth_op[d] ..
+  // res_depth_op[d + 1] - 1, output rows res_depth_item[d] ..), their first
+  // output rows and the filter position codes
+  double2 *res_chunk = nullptr;
+  ResInterp *res_ops = nullptr;
+  int64_t *res_op_prefix = nullptr;
+  int32_t *res_codes = nullptr;
+  std::vector<int> res_depth_op;
+  std::vector<int64_t> res_depth_item;
   // stage timing (extension): events recorded around prep / sweep / rings
   bool profiling = false;
   std::vector<hipEvent_t> ev_free;
@@ -600,8 +610,14 @@ static bool run_legendre(wavemoth_plan p, const double2 *d_alm, hipStream_t s) {
   if (p->res_apply) {
     if (!mark(p, 1, s)) return false;
     WM_TRY(launch_res_legendre(p->lt, d_alm, p->res_img, p->res_strips, p->res_prefix,
-                               p->res_nstrips, p->res_items, true, s),
+                               p->res_nstrips, p->res_items, p->res_chunk, true, s),
            "resource Legendre kernel");
+    for (size_t d = 0; d + 1 < p->res_depth_op.size(); ++d)
+      WM_TRY(launch_res_interp(p->lt, p->res_img, p->res_ops, p->res_op_prefix, p->res_codes,
+                               p->res_depth_op[d], p->res_depth_op[d + 1],
+                               p->res_depth_item[d], p->res_depth_item[d + 1], p->res_chunk,
+                               s),
+             "resource interpolation kernel");
   } else if (p->mix >= 0) {
     WM_TRY(launch_mix_prep(p->lt, d_alm, p->mix_b, p->mix_ncp, p->mix, p->mix_pstride,
                            p->mix_cexp, s),
@@ -639,6 +655,7 @@ static bool build_resource_apply(wavemoth_plan p) {
     d.cstart = (int)x.cstart;
     d.nx = (int)(x.cstop - x.cstart);
     d.row = x.row0 + x.rstart;
+    d.chunk = x.chunk;
     if (x.x2_off >= 0) {
       d.x2 = x.x2_off / 8;
       d.p0 = x.p0_off / 8;
@@ -653,11 +670,92 @@ static bool build_resource_apply(wavemoth_plan p) {
   }
   p->res_nstrips = (int)st.size();
   p->res_items = items;
-  if (st.empty()) return true;
+  // interpolations, stably sorted by depth (a parent's outputs are its
+  // children's inputs), with the filters turned into position codes
+  std::vector<const ResourceInterp *> ord;
+  for (const ResourceInterp &x : r.interps) ord.push_back(&x);
+  std::stable_sort(ord.begin(), ord.end(), [](const ResourceInterp *a, const ResourceInterp *b) {
+    return a->depth < b->depth;
+  });
+  std::vector<ResInterp> ops;
+  std::vector<int64_t> op_prefix;
+  std::vector<int32_t> codes;
+  int64_t oitems = 0;
+  p->res_depth_op.assign(1, 0);
+  p->res_depth_item.assign(1, 0);
+  for (size_t i = 0; i < ord.size(); ++i) {
+    const ResourceInterp &x = *ord[i];
+    if (i > 0 && x.depth != ord[i - 1]->depth) {
+      p->res_depth_op.push_back((int)i);
+      p->res_depth_item.push_back(oitems);
+    }
+    ResInterp o;
+    o.m = x.m;
+    o.odd = x.odd;
+    o.kT = (int)x.kT;
+    o.kB = (int)x.kB;
+    o.nl = (int)x.nl;
+    o.nr = (int)x.nr;
+    o.to_rings = x.to_rings;
+    o.inT = x.inT;
+    o.inB = x.inB;
+    o.outL = x.outL;
+    o.outR = x.outR;
+    o.ipT = x.ipT / 8;
+    o.ipB = x.ipB / 8;
+    const int64_t n = x.nl + x.nr;
+    for (int side = 0; side < 2; ++side) {
+      (side ? o.codeB : o.codeT) = (int64_t)codes.size();
+      const char *mask = r.base + (side ? x.maskB : x.maskT);
+      int32_t zeros = 0, ones = 0;
+      for (int64_t u = 0; u < n; ++u) codes.push_back(mask[u] == 0 ? zeros++ : -1 - ones++);
+    }
+    ops.push_back(o);
+    op_prefix.push_back(oitems);
+    oitems += n;
+  }
+  // every index the kernels form stays inside the chunk buffer, the rings
+  // and the file image
+  const int64_t rows = r.chunk_rows, img = (int64_t)(r.size / 8), R = 2 * (int64_t)p->N;
+  for (const ResStrip &d : st)
+    if (d.chunk >= 0 && (d.chunk + d.nx > rows)) {
+      set_err("resource apply: a strip's chunk rows lie outside the chunk buffer");
+      return false;
+    }
+  for (const ResInterp &o : ops) {
+    const int64_t n = (int64_t)o.nl + o.nr;
+    const bool in_ok = o.inT >= 0 && o.inT + o.kT <= rows && o.inB >= 0 && o.inB + o.kB <= rows;
+    const int64_t lim = o.to_rings ? R : rows;
+    const bool out_ok = o.outL >= 0 && o.outL + o.nl <= lim && o.outR >= 0 && o.outR + o.nr <= lim;
+    const bool ip_ok = o.ipT >= 0 && o.ipT + (n - o.kT) * o.kT <= img && o.ipB >= 0 &&
+                       o.ipB + (n - o.kB) * o.kB <= img;
+    if (!in_ok || !out_ok || !ip_ok || o.m < 0 || o.m > p->M) {
+      set_err("resource apply: an interpolation's rows lie outside its buffers");
+      return false;
+    }
+  }
+  if (!ops.empty()) {
+    p->res_depth_op.push_back((int)ops.size());
+    p->res_depth_item.push_back(oitems);
+  } else {
+    p->res_depth_op.clear();
+    p->res_depth_item.clear();
+  }
+  if (st.empty() && ops.empty()) return true;
   if (!dalloc(p->arena, &p->res_img, r.size / 8 + 1, "resource image")) return false;
   WM_TRY(hipMemcpy(p->res_img, r.base, r.size, hipMemcpyHostToDevice), "resource upload");
-  if (!upload(p->arena, &p->res_strips, st, "resource strips")) return false;
-  if (!upload(p->arena, &p->res_prefix, prefix, "resource strip prefix")) return false;
+  if (!st.empty() && (!upload(p->arena, &p->res_strips, st, "resource strips") ||
+                      !upload(p->arena, &p->res_prefix, prefix, "resource strip prefix")))
+    return false;
+  if (!ops.empty()) {
+    if (!upload(p->arena, &p->res_ops, ops, "resource interpolations") ||
+        !upload(p->arena, &p->res_op_prefix, op_prefix, "resource interpolation prefix") ||
+        !upload(p->arena, &p->res_codes, codes, "resource filter codes"))
+      return false;
+  }
+  if (r.chunk_rows > 0 &&
+      !dalloc(p->arena, &p->res_chunk, (size_t)r.chunk_rows * p->K, "resource chunk buffer"))
+    return false;
   return true;
 }
 
@@ -791,9 +889,6 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     const char *why = !p->res ? "WAVEMOTH_RESOURCE_APPLY needs a resource file"
                       : p->mix >= 0
                           ? "WAVEMOTH_RESOURCE_APPLY runs in FP64: no WAVEMOTH_MIXED_* flag"
-                      : (p->res->trees || p->res->max_depth)
-                          ? "WAVEMOTH_RESOURCE_APPLY: the file holds butterfly-compressed "
-                            "matrices, which the device applies only uncompressed"
                           : nullptr;
     if (why) {
       set_err(why);

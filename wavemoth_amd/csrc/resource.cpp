@@ -75,9 +75,9 @@ struct MatrixWalk {
   }
 
   // A block payload (the Legendre rows of one remainder block) spanning
-  // [pos, end), over `cols` columns; columns [col0, col0 + cols) are ring
-  // pairs when `rings` is set (an identity root).
-  bool block_payload(int64_t pos, int64_t end, int64_t cols, bool rings, int64_t col0) {
+  // [pos, end), over `cols` columns: rings col0.. of q when chunk < 0 (an
+  // identity root), else rows chunk.. of the chunk buffer.
+  bool block_payload(int64_t pos, int64_t end, int64_t cols, int64_t col0, int64_t chunk) {
     if (pos < lo || end > hi || pos > end) return fail("block payload outside the matrix");
     int64_t r0, r1;
     pos = (pos + 15) & ~(int64_t)15;
@@ -85,19 +85,26 @@ struct MatrixWalk {
     if (!rd64(pos, &r0) || !rd64(pos, &r1)) return false;
     if (r0 < 0 || r1 < r0 || r1 > nrows) return fail("block rows outside the matrix");
     const int64_t nk = r1 - r0;
+    auto strip = [&](int64_t c0, int64_t c1) {
+      ResourceStrip s;
+      s.m = m;
+      s.odd = odd;
+      s.row0 = r0;
+      s.nk = nk;
+      s.cstart = chunk < 0 ? col0 + c0 : c0;
+      s.cstop = chunk < 0 ? col0 + c1 : c1;
+      s.chunk = chunk < 0 ? -1 : chunk + c0;
+      return s;
+    };
     int64_t off;
     if (nk <= 4 || cols == 0) {
       if (!arr(pos, cols * nk, end, &off)) return false;
       res->dense_blocks += 1;
       res->elements += cols * nk;
-      if (rings && cols * nk > 0) {
-        ResourceStrip s;
-        s.m = m;
-        s.odd = odd;
-        s.row0 = r0;
-        s.nk = nk;
-        s.cstart = col0;
-        s.cstop = col0 + cols;
+      if (cols > 0) {
+        // (a block of no rows writes zeros, as the reference's dgemm over
+        // an empty inner dimension does)
+        ResourceStrip s = strip(0, cols);
         s.dense_off = off;
         res->strips.push_back(s);
       }
@@ -116,14 +123,8 @@ struct MatrixWalk {
       if (rstart < 0 || rstart > nk) return fail("strip start row outside the block");
       if (cstop < cstart || cstop > cols) return fail("strip columns out of order");
       const int64_t nx = cstop - cstart, nks = nk - rstart;
-      ResourceStrip s;
-      s.m = m;
-      s.odd = odd;
-      s.row0 = r0;
-      s.nk = nk;
+      ResourceStrip s = strip(cstart, cstop);
       s.rstart = rstart;
-      s.cstart = col0 + cstart;
-      s.cstop = col0 + cstop;
       s.aux_off = aux;
       if (nks <= 4) {
         if (!arr(pos, nks * nx, end, &s.dense_off)) return false;
@@ -134,19 +135,28 @@ struct MatrixWalk {
         res->recurrence_strips += 1;
       }
       res->elements += nks * nx;
-      if (rings && nx > 0) res->strips.push_back(s);
+      if (nx > 0) res->strips.push_back(s);
       cstart = cstop;
     }
     if (cstart != cols) return fail("strips do not cover every column");
     return true;
   }
 
+  int64_t alloc_chunk(int64_t rows) {
+    const int64_t r = res->chunk_rows;
+    res->chunk_rows += rows;
+    return r;
+  }
+
   // The traversal of src/butterfly.c.in:284-391 (transpose_apply_node),
-  // checking instead of applying.  Roots pull their input blocks from the
-  // residual payloads; inner nodes read one interpolation block per output
-  // block; leaves (identity nodes) consume n columns of the target.
+  // checking every offset and recording the work instead of applying it.
+  // Roots pull their input blocks from the residual payloads (into chunks,
+  // or straight into q for an identity root); inner nodes read one
+  // interpolation block per input block and write their children's input
+  // blocks (q rings for leaf children); leaves consume n columns of the
+  // target.  in_base: the chunk rows of a non-root node's input blocks.
   bool walk(int64_t inode, bool is_root, int root_idx, int64_t target_start,
-            int64_t *target_stop, int depth) {
+            int64_t *target_stop, int depth, const std::vector<int64_t> &in_base) {
     if (depth > 40) return fail("tree too deep");
     int64_t pos, nblocks;
     if (!node_at(inode, &pos) || !rd32(pos, &nblocks)) return false;
@@ -167,16 +177,15 @@ struct MatrixWalk {
         if (payloads[i] < lo || payloads[i] > hi || (i > 0 && payloads[i] < payloads[i - 1]))
           return fail("residual payload offsets out of order");
       }
+    } else if ((int64_t)in_base.size() != (nblocks == 0 ? 1 : nblocks)) {
+      return fail("node inputs do not match its blocks");
     }
     if (nblocks == 0) {
       int64_t n;
       if (!rd32(pos, &n)) return false;
       if (n < 0 || target_start + n > ncols) return fail("leaf wider than the matrix");
       *target_stop = target_start + n;
-      if (is_root) {
-        const bool rings = depth == 0;
-        if (!block_payload(payloads[0], payloads[1], n, rings, target_start)) return false;
-      }
+      if (is_root && !block_payload(payloads[0], payloads[1], n, target_start, -1)) return false;
       res->max_depth = std::max<int64_t>(res->max_depth, depth);
       return true;
     }
@@ -189,34 +198,77 @@ struct MatrixWalk {
     if (!((nl == 0 && nr == 0) || (nl == nblocks / 2 && nr == nblocks / 2)))
       return fail("children do not match the node's blocks");
     if (nl == 0 && nblocks != 2) return fail("leaf children under more than two blocks");
+    const bool leaves = nl == 0;
+    std::vector<int64_t> base(nblocks);
     int64_t input_pos = 0;
+    for (int64_t b = 0; b < nblocks; ++b) {
+      if (heights[b] < 0) return fail("negative block height");
+      if (is_root) {
+        base[b] = alloc_chunk(heights[b]);
+        if (!block_payload(payloads[b], payloads[b + 1], heights[b], input_pos, base[b]))
+          return false;
+        input_pos += heights[b];
+      } else {
+        base[b] = in_base[b];
+      }
+    }
+    std::vector<int64_t> lbase, rbase;
+    int64_t ring = target_start;
     for (int64_t i = 0; i < nblocks / 2; ++i) {
       int64_t hl, hr;
       int64_t lp = lpos + 4 * i, rp = rpos + 4 * i;
       if (!rd32(lp, &hl) || !rd32(rp, &hr)) return false;
       const int64_t n = hl + hr;
       if (hl < 0 || hr < 0) return fail("negative child block height");
+      ResourceInterp ip;
+      ip.m = m;
+      ip.odd = odd;
+      ip.depth = depth;
+      ip.nl = hl;
+      ip.nr = hr;
+      ip.kT = heights[2 * i];
+      ip.kB = heights[2 * i + 1];
+      ip.inT = base[2 * i];
+      ip.inB = base[2 * i + 1];
+      ip.to_rings = leaves;
+      if (leaves) {
+        if (ring + n > ncols) return fail("leaf wider than the matrix");
+        ip.outL = ring;
+        ip.outR = ring + hl;
+        ring += n;
+      } else {
+        ip.outL = alloc_chunk(hl);
+        ip.outR = alloc_chunk(hr);
+        lbase.push_back(ip.outL);
+        rbase.push_back(ip.outR);
+      }
       for (int j = 0; j < 2; ++j) {
         const int64_t k = heights[2 * i + j];
-        if (k < 0 || k > n) return fail("interpolation block rank out of range");
-        if (is_root) {
-          if (!block_payload(payloads[2 * i + j], payloads[2 * i + j + 1], k, false, input_pos))
-            return false;
-          input_pos += k;
-        }
+        if (k > n) return fail("interpolation block rank out of range");
         if (pos < lo || pos + n > hi) return fail("interpolation filter past the matrix end");
         int64_t zeros = 0;
         for (int64_t u = 0; u < n; ++u) zeros += res->base[pos + u] == 0;
         if (zeros != k) return fail("interpolation filter does not select k columns");
+        (j ? ip.maskB : ip.maskT) = pos;
         pos += n;
         int64_t off;
         if (!arr(pos, (n - k) * k, hi, &off)) return false;
+        (j ? ip.ipB : ip.ipT) = off;
         res->elements += (n - k) * k;
       }
+      res->interps.push_back(ip);
+      res->interp_depths = std::max(res->interp_depths, depth + 1);
     }
     int64_t mid;
-    if (!walk(2 * inode, false, 0, target_start, &mid, depth + 1)) return false;
-    if (!walk(2 * inode + 1, false, 0, mid, target_stop, depth + 1)) return false;
+    if (leaves) {
+      std::vector<int64_t> one(1);
+      if (!walk(2 * inode, false, 0, target_start, &mid, depth + 1, one)) return false;
+      if (!walk(2 * inode + 1, false, 0, mid, target_stop, depth + 1, one)) return false;
+      if (*target_stop != ring) return fail("leaves do not match their parent's blocks");
+      return true;
+    }
+    if (!walk(2 * inode, false, 0, target_start, &mid, depth + 1, lbase)) return false;
+    if (!walk(2 * inode + 1, false, 0, mid, target_stop, depth + 1, rbase)) return false;
     return true;
   }
 
@@ -249,9 +301,10 @@ struct MatrixWalk {
     }
     heap_first = first_index;
     int64_t start = 0;
+    const std::vector<int64_t> none;
     for (int i = 0; i < first_size; ++i) {
       int64_t stop;
-      if (!walk(first_index + i, true, i, start, &stop, 0)) return false;
+      if (!walk(first_index + i, true, i, start, &stop, 0, none)) return false;
       start = stop;
     }
     if (start != ncols) return fail("roots do not cover every column");
@@ -363,7 +416,9 @@ bool resource_matches(const Resource &r, int *Nside, int lmax, int mmax) {
 }
 
 double resource_x2_deviation(const Resource &r) {
-  if (r.strips.empty()) return -1.0;
+  bool rings = false;
+  for (const ResourceStrip &s : r.strips) rings |= s.x2_off >= 0 && s.chunk < 0;
+  if (!rings) return -1.0;
   const int N = (int)r.Nside;
   std::vector<double> x2(2 * N);
   for (int i = 0; i < 2 * N; ++i) {
@@ -372,7 +427,7 @@ double resource_x2_deviation(const Resource &r) {
   }
   double worst = 0.0;
   for (const ResourceStrip &s : r.strips) {
-    if (s.x2_off < 0) continue;
+    if (s.x2_off < 0 || s.chunk >= 0) continue;
     for (int64_t c = s.cstart; c < s.cstop; ++c) {
       double v;
       std::memcpy(&v, r.base + s.x2_off + 8 * (c - s.cstart), 8);

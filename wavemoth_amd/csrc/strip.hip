@@ -93,7 +93,8 @@ __global__ __launch_bounds__(256) void k_res_legendre(LegTables t, const double2
                                                       const double *__restrict__ img,
                                                       const ResStrip *__restrict__ strips,
                                                       const int64_t *__restrict__ prefix,
-                                                      int nstrips, int64_t items) {
+                                                      int nstrips, int64_t items,
+                                                      double2 *__restrict__ chunk) {
   const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (it >= items) return;
   const int K = t.K, L = t.L, Mp1 = t.M + 1;
@@ -144,10 +145,85 @@ __global__ __launch_bounds__(256) void k_res_legendre(LegTables t, const double2
     }
   }
 #undef WM_RES_ADD
+  if (s.chunk >= 0) {
+    double2 *dst = chunk + (s.chunk + c) * K + kb0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < kn) dst[k] = acc[k];
+    return;
+  }
   const int ring = s.cstart + c;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
     if (k < kn) t.qbuf[q_index(ring, s.odd, kb0 + k, m, K, Mp1)] = acc[k];
+}
+
+// The transposed interpolations of one tree depth of the compressed
+// matrices: thread (output row, map block of 8), row -> op by binary
+// search over the ops' first rows.  The T and B blocks of a pair feed the
+// same rows, so one thread forms both terms and writes the sum once (the
+// reference's second scatter adds into the first's output).
+__global__ __launch_bounds__(256) void k_res_interp(LegTables t, const double *__restrict__ img,
+                                                    const ResInterp *__restrict__ ops,
+                                                    const int64_t *__restrict__ prefix,
+                                                    const int32_t *__restrict__ codes, int op_lo,
+                                                    int op_hi, int64_t item_lo, int64_t item_hi,
+                                                    double2 *__restrict__ chunk) {
+  const int64_t it = item_lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= item_hi) return;
+  const int K = t.K, Mp1 = t.M + 1;
+  const int kb0 = blockIdx.y * 8;
+  const int kn = (K - kb0) < 8 ? (K - kb0) : 8;
+  int lo = op_lo, hi = op_hi - 1;  // last op with prefix <= it
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  const ResInterp o = ops[lo];
+  const int p = (int)(it - prefix[lo]);
+  double2 acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int code = codes[(side ? o.codeB : o.codeT) + p];
+    const int kx = side ? o.kB : o.kT;
+    const double2 *x = chunk + (side ? o.inB : o.inT) * K + kb0;
+    if (code >= 0) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < kn) {
+          const double2 v = x[(int64_t)code * K + k];
+          acc[k].x += v.x;
+          acc[k].y += v.y;
+        }
+    } else {
+      const double *A = img + (side ? o.ipB : o.ipT) + (int64_t)(-1 - code) * kx;
+      for (int r = 0; r < kx; ++r) {
+        const double a = A[r];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < kn) {
+            const double2 v = x[(int64_t)r * K + k];
+            acc[k].x = fma(a, v.x, acc[k].x);
+            acc[k].y = fma(a, v.y, acc[k].y);
+          }
+      }
+    }
+  }
+  const bool left = p < o.nl;
+  const int64_t row = left ? o.outL + p : o.outR + (p - o.nl);
+  if (o.to_rings) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < kn) t.qbuf[q_index((int)row, o.odd, kb0 + k, o.m, K, Mp1)] = acc[k];
+  } else {
+    double2 *dst = chunk + row * K + kb0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < kn) dst[k] = acc[k];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_zero_q(double2 *q, int64_t n) {
@@ -157,7 +233,7 @@ __global__ __launch_bounds__(256) void k_zero_q(double2 *q, int64_t n) {
 
 hipError_t launch_res_legendre(const LegTables &t, const double2 *alm, const double *img,
                                const ResStrip *strips, const int64_t *prefix, int nstrips,
-                               int64_t items, bool zero_q, hipStream_t s) {
+                               int64_t items, double2 *chunk, bool zero_q, hipStream_t s) {
   if (zero_q) {
     const int64_t n = q_elems(t.R, t.K, t.M + 1);
     hipLaunchKernelGGL(k_zero_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t.qbuf, n);
@@ -167,7 +243,17 @@ hipError_t launch_res_legendre(const LegTables &t, const double2 *alm, const dou
   if (items == 0) return hipSuccess;
   const dim3 grid((unsigned)((items + 255) / 256), (unsigned)((t.K + 7) / 8));
   hipLaunchKernelGGL(k_res_legendre, grid, dim3(256), 0, s, t, alm, img, strips, prefix, nstrips,
-                     items);
+                     items, chunk);
+  return hipGetLastError();
+}
+
+hipError_t launch_res_interp(const LegTables &t, const double *img, const ResInterp *ops,
+                             const int64_t *prefix, const int32_t *codes, int op_lo, int op_hi,
+                             int64_t item_lo, int64_t item_hi, double2 *chunk, hipStream_t s) {
+  if (item_hi <= item_lo || op_hi <= op_lo) return hipSuccess;
+  const dim3 grid((unsigned)((item_hi - item_lo + 255) / 256), (unsigned)((t.K + 7) / 8));
+  hipLaunchKernelGGL(k_res_interp, grid, dim3(256), 0, s, t, img, ops, prefix, codes, op_lo,
+                     op_hi, item_lo, item_hi, chunk);
   return hipGetLastError();
 }
 

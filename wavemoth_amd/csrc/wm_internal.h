@@ -210,21 +210,44 @@ int legendre_padded_maps(int K);
 
 // One strip of a resource matrix on the device (the Legendre stage read
 // from a rev1 file): rows `row`.. of (m, odd) (l = m + odd + 2 row), nks of
-// them, ring pairs [cstart, cstart + nx); offsets in doubles into the file
-// image on the device: a recurrence strip has x2 / p0 / p1 (nx each) and
-// aux (3 per row past the second, of the strip's rows), a dense one a
-// column-major nks x nx block at dense (x2 = -1).
+// them, over nx columns; offsets in doubles into the file image on the
+// device: a recurrence strip has x2 / p0 / p1 (nx each) and aux (3 per row
+// past the second, of the strip's rows), a dense one a column-major
+// nks x nx block at dense (x2 = -1).  Column c goes to ring pair cstart + c
+// of q (chunk < 0) or to row chunk + c of the chunk buffer (an input block
+// of a compressed matrix's root).
 struct ResStrip {
   int m = 0, odd = 0, nks = 0, cstart = 0, nx = 0;
   int64_t row = 0;
   int64_t x2 = -1, p0 = -1, p1 = -1, aux = -1, dense = -1;
+  int64_t chunk = -1;
 };
-// q <- the Legendre stage of the strips (FP64 q; items = prefix[nstrips]
-// (ring pair, strip) pairs); (m, odd) without a matrix are zeroed first
-// when zero_q is set.
+// q (or chunk rows) <- the Legendre stage of the strips (FP64 q; items =
+// prefix[nstrips] (column, strip) pairs); (m, odd) without a matrix are
+// zeroed first when zero_q is set.
 hipError_t launch_res_legendre(const LegTables &t, const double2 *alm, const double *img,
                                const ResStrip *strips, const int64_t *prefix, int nstrips,
-                               int64_t items, bool zero_q, hipStream_t s);
+                               int64_t items, double2 *chunk, bool zero_q, hipStream_t s);
+
+// One block pair (T, B) of an inner node of a butterfly-compressed matrix
+// (src/butterfly.c.in:222-235, 352-379): its nl + nr output rows are
+//   out[p] = f_T(p) + f_B(p),  f_X(p) = x_X[c]                  (code c >= 0)
+//                                      = sum_r A_X[r, c] x_X[r]  (code -1 - c),
+// x_X the kX chunk rows at inX, A_X the column-major kX x (n - kX)
+// interpolant at ipX (doubles into the file image), codes[codeX + p] the
+// position of p among the filter's identity (0) or interpolated (1)
+// columns.  Rows p < nl go to outL + p, the others to outR + p - nl: rings
+// of q when to_rings (leaf children), else chunk rows.
+struct ResInterp {
+  int m = 0, odd = 0, kT = 0, kB = 0, nl = 0, nr = 0, to_rings = 0;
+  int64_t inT = 0, inB = 0, outL = 0, outR = 0;
+  int64_t codeT = 0, codeB = 0, ipT = 0, ipB = 0;
+};
+// The interpolations ops[op_lo, op_hi) (one tree depth); prefix[i] is the
+// first output row of op i in a numbering over all ops.
+hipError_t launch_res_interp(const LegTables &t, const double *img, const ResInterp *ops,
+                             const int64_t *prefix, const int32_t *codes, int op_lo, int op_hi,
+                             int64_t item_lo, int64_t item_hi, double2 *chunk, hipStream_t s);
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_seeds(const LegTables &t, hipStream_t s);

@@ -30,19 +30,38 @@
 
 namespace wm {
 
-// One Legendre strip of a num_levels = 0 matrix (an identity root whose
-// single remainder block covers every row and column): rows rstart.. of the
-// matrix (l = m + odd + 2 (row_start + rstart + k)), columns (rings)
-// [cstart, cstop).  Offsets are byte offsets into the file.
+// One Legendre strip of a remainder block: rows rstart.. of the block
+// (l = m + odd + 2 (row_start + rstart + k)), its columns [cstart, cstop).
+// The strips of an identity root's block (a num_levels = 0 matrix, or a
+// root that is a leaf) produce rings cstart.. of q directly (chunk < 0);
+// those of a compressed root's blocks produce rows chunk.. of the chunk
+// buffer, the inputs of the root's interpolations.  Offsets are byte
+// offsets into the file.
 struct ResourceStrip {
   int m = 0, odd = 0;
   int64_t row0 = 0;        // first row of the block payload (row_start)
   int64_t nk = 0;          // rows of the block payload (row_stop - row_start)
   int64_t rstart = 0;      // first row of the strip within the block
   int64_t cstart = 0, cstop = 0;
+  int64_t chunk = -1;      // chunk row of column cstart, or -1: columns are rings
   int64_t aux_off = -1;    // 3 (nk - 2) doubles (recurrence strips)
   int64_t x2_off = -1, p0_off = -1, p1_off = -1;  // recurrence strip data
   int64_t dense_off = -1;  // dense strip or dense block: (nk - rstart) x nx, column-major
+};
+
+// One block pair (T, B) of an inner node of a compressed matrix at tree
+// depth `depth` (roots 0): the transposed interpolations of the T and B
+// input blocks (kT and kB chunk rows), summed, give the n = nl + nr rows of
+// the pair's left and right child blocks (src/butterfly.c.in:222-235 and
+// 352-379).  A child that is a leaf takes its rows straight into q (rings
+// outL.. / outR..); an inner child's block is a chunk of its own.
+struct ResourceInterp {
+  int m = 0, odd = 0, depth = 0;
+  int64_t kT = 0, kB = 0, nl = 0, nr = 0;
+  int64_t inT = 0, inB = 0;      // chunk rows of the inputs
+  int64_t outL = 0, outR = 0;    // chunk rows, or rings when to_rings
+  bool to_rings = false;
+  int64_t maskT = 0, ipT = 0, maskB = 0, ipB = 0;  // filter and A_ip offsets
 };
 
 struct Resource {
@@ -53,9 +72,12 @@ struct Resource {
   // statistics
   int64_t present = 0, missing = 0, trees = 0, recurrence_strips = 0, dense_blocks = 0,
           elements = 0, max_depth = 0;
-  // strips of the matrices with an identity root (all of them for files
-  // written without compression)
+  // every remainder block's strips, and the interpolations of the
+  // compressed matrices in tree order (parents before children)
   std::vector<ResourceStrip> strips;
+  std::vector<ResourceInterp> interps;
+  int64_t chunk_rows = 0;  // rows of the chunk buffer the interpolations use
+  int interp_depths = 0;   // 1 + the deepest interpolation
   ~Resource();
 };
 

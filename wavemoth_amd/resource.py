@@ -1,11 +1,13 @@
 """rev1 resource files: the reference's precomputed Legendre matrices.
 
 write_resource() writes the file the reference's precompute step writes
-(wavemoth/lib.pyx:501-592, ResourceComputer.compute), with every (m, odd)
-matrix stored uncompressed, as the butterfly serializer does for
-num_levels = 0 (wavemoth/butterfly.pyx:743-825): an identity root whose one
-remainder block holds the Legendre strips of every row and ring pair
-(wavemoth/lib.pyx:397-487, LegendreMatrixProvider.serialize_block_payload).
+(wavemoth/lib.pyx:501-592, ResourceComputer.compute).  Each (m, odd) matrix
+is stored either uncompressed, as the butterfly serializer does for
+num_levels = 0 (wavemoth/butterfly.pyx:743-825: an identity root whose one
+remainder block holds the Legendre strips of every row and ring pair,
+wavemoth/lib.pyx:397-487), or butterfly-compressed
+(wavemoth/butterfly.pyx:591-720: interpolative decompositions of column
+blocks, merged pairwise up a binary tree) and cut to a number of levels.
 The GPU path does not need the file (it generates Lambda on the device);
 the file is for callers of the reference's API, which accepts one in
 wavemoth_plan_to_healpix(..., resource_filename) or finds it under the
@@ -196,11 +198,10 @@ def _strip_last_row(aux, x2, P0, P1, nk):
     return p if nk > 1 else pp
 
 
-def _block_payload(s, m, odd, thetas, nrows):
-    """One remainder block over all rows and ring pairs of (m, odd):
-    wavemoth/lib.pyx:397-487 (serialize_block_payload with row_start = 0,
-    row_stop = nrows and every column)."""
-    row_start, row_stop = 0, nrows
+def _block_payload(s, m, odd, thetas, row_start, row_stop):
+    """One remainder block: rows [row_start, row_stop) of the (m, odd)
+    matrix over the columns (rings) whose colatitudes are `thetas`:
+    wavemoth/lib.pyx:397-487 (LegendreMatrixProvider.serialize_block_payload)."""
     ncols = thetas.shape[0]
     if ncols == 0 or row_start == row_stop:
         _pad128(s)
@@ -252,58 +253,317 @@ def _block_payload(s, m, odd, thetas, nrows):
     assert prev_stop == Lam.shape[1]
 
 
-def _matrix(s, m, odd, Nside, lmax):
-    """Butterfly matrix of (m, odd) with num_levels = 0: an identity root
-    and one remainder block (wavemoth/butterfly.pyx:743-825; the root's
-    element_count is 0, as IdentityNode.size() returns)."""
+# ------------------------------------------------------ butterfly compression
+
+class Interpolation:
+    """X ~= X[:, filter == 0] [I | A_ip] with the interpolated columns
+    X[:, filter == 1] = X[:, filter == 0] A_ip (wavemoth/butterfly.pyx:
+    InterpolationBlock).  transpose_apply maps k values onto the block's n
+    columns: identity columns take them, the others A_ip^T of them."""
+
+    def __init__(self, filter, A_ip):
+        self.filter = np.ascontiguousarray(filter, dtype=np.int8)
+        self.A_ip = np.asfortranarray(A_ip, dtype=np.float64)
+        self.n = self.filter.shape[0]
+        self.k = self.n - int(self.filter.sum())
+        assert self.A_ip.shape == (self.k, self.n - self.k)
+
+    def transpose_apply(self, x):
+        y = np.empty((self.n,) + x.shape[1:], x.dtype)
+        y[self.filter == 0] = x
+        y[self.filter == 1] = self.A_ip.T @ x if self.k else 0
+        return y
+
+    def size(self):
+        return self.A_ip.size + self.n
+
+
+def interpolative_decomposition(X, eps):
+    """(sorted skeleton columns, Interpolation) of X to precision eps: the
+    reference's sparse_interpolative_decomposition
+    (wavemoth/interpolative_decomposition.pyx:41-98) over libidlight's
+    iddp_id; here scipy.linalg.interpolative's deterministic ID (rand=False,
+    the same pivoted-QR interpolative decomposition) stands in for it.  An
+    all-zero block has rank 0."""
+    import scipy.linalg.interpolative as sli
+    n = X.shape[1]
+    if n == 0:
+        return np.zeros(0, np.int64), Interpolation(np.zeros(0), np.zeros((0, 0)))
+    if X.shape[0] == 0 or not np.any(X):
+        return np.zeros(0, np.int64), Interpolation(np.ones(n), np.zeros((0, n)))
+    k, idx, proj = sli.interp_decomp(np.asfortranarray(X, dtype=np.float64), eps, rand=False)
+    iden, ipol = np.asarray(idx[:k]), np.asarray(idx[k:])
+    proj = np.asarray(proj).reshape(k, n - k)
+    A_ip = proj[:, np.argsort(ipol)] if n > k else np.zeros((k, 0))
+    A_ip = A_ip[np.argsort(iden), :]
+    filt = np.zeros(n, np.int8)
+    filt[ipol] = 1
+    return np.sort(iden), Interpolation(filt, A_ip)
+
+
+class _Leaf:
+    """Identity node over n columns (wavemoth/butterfly.pyx: IdentityNode)."""
+
+    def __init__(self, n, remainders):
+        self.ncols = n
+        self.heights = [n]
+        self.remainders = remainders   # [(row_start, row_stop, columns)]
+        self.children = []
+
+    def depth(self):
+        return 0
+
+    def size(self):
+        return 0
+
+
+class _Inner:
+    """Butterfly node: per block pair (T, B) of its output blocks, one
+    interpolation of the pair's left and right child blocks
+    (wavemoth/butterfly.pyx: InnerNode)."""
+
+    def __init__(self, pairs, children, remainders):
+        self.pairs = pairs
+        self.children = children
+        self.heights = [x for T, B in pairs for x in (T.k, B.k)]
+        self.ncols = sum(c.ncols for c in children)
+        self.remainders = remainders
+
+    def depth(self):
+        return 1 + max(c.depth() for c in self.children)
+
+    def size(self):
+        return (sum(c.size() for c in self.children) +
+                sum(T.size() + B.size() for T, B in self.pairs))
+
+    def transpose_apply(self, x):
+        """Outputs of the interpolation tree below this node for its input
+        blocks x (concatenated in block order): the reference's
+        transpose_apply_interpolations."""
+        L, R = self.children
+        zl, zr = [], []
+        i = 0
+        for (T, B), hl, hr in zip(self.pairs, L.heights, R.heights):
+            t = T.transpose_apply(x[i:i + T.k])
+            i += T.k
+            t = t + B.transpose_apply(x[i:i + B.k])
+            i += B.k
+            zl.append(t[:hl])
+            zr.append(t[hl:])
+        out = []
+        for c, z in ((L, zl), (R, zr)):
+            z = np.concatenate(z) if z else np.zeros((0,) + x.shape[1:])
+            out.append(z if isinstance(c, _Leaf) else c.transpose_apply(z))
+        return np.concatenate(out)
+
+
+def _butterfly(col, nrows, partition, A, eps):
+    """Compress columns [col, partition[-1]) of A (rows 0..nrows): the
+    reference's butterfly_core (wavemoth/butterfly.pyx:591-645).  Returns
+    the skeleton column blocks and the node."""
+    if len(partition) == 1:
+        cols = np.arange(col, partition[0])
+        return [A[:, cols]], _Leaf(len(cols), [(0, nrows, cols)])
+    mid = len(partition) // 2
+    lk, ln = _butterfly(col, nrows, partition[:mid], A, eps)
+    rk, rn = _butterfly(col + ln.ncols, nrows, partition[mid:], A, eps)
+    remainders, pairs, blocks = [], [], []
+    for (r0, r1, lc), (_, _, rc), LX, RX in zip(ln.remainders, rn.remainders, lk, rk):
+        cols = np.concatenate([lc, rc]).astype(np.int64)
+        X = np.hstack([LX, RX])
+        vmid = X.shape[0] // 2
+        pair = []
+        i = r0
+        for part in (X[:vmid], X[vmid:]):
+            rel, ip = interpolative_decomposition(part, eps)
+            pair.append(ip)
+            remainders.append((i, i + part.shape[0], cols[rel]))
+            blocks.append(part[:, rel].copy())
+            i += part.shape[0]
+        pairs.append(tuple(pair))
+    return blocks, _Inner(pairs, (ln, rn), remainders)
+
+
+def _partition(ncols, chunk_size):
+    """End columns of chunk_size-column blocks, padded to a power-of-two
+    count by repeating ends (empty blocks): make_partition and
+    pad_with_empty_columns of wavemoth/butterfly.pyx:678-706."""
+    ends = list(range(chunk_size, ncols, chunk_size))
+    if not ends or ends[-1] != ncols:
+        ends.append(ncols)
+    target = 1
+    while target < len(ends):
+        target *= 2
+    extra = target - len(ends)
+    out = []
+    for e in ends:
+        out.append(e)
+        if extra > 0:
+            out.append(e)
+            extra -= 1
+    return out
+
+
+def compress_matrix(A, chunk_size, eps):
+    """Butterfly compression of a matrix whose columns are rings
+    (butterfly_compress, wavemoth/butterfly.pyx:708-720)."""
+    return _butterfly(0, A.shape[0], _partition(A.shape[1], chunk_size), A, eps)[1]
+
+
+def _nodes_at(node, level):
+    if level == 0:
+        return [node]
+    return [x for c in node.children for x in _nodes_at(c, level - 1)]
+
+
+def choose_levels(tree, memop_cost=20.0):
+    """Compression levels to keep: the argmin over levels of interpolation
+    elements plus residual cost rows x columns x (5/2 + 1) / memop_cost
+    (ResourceComputer.compute_matrix, wavemoth/lib.pyx:519-534, with
+    get_stats of wavemoth/butterfly.pyx:455-480)."""
+    depth = tree.depth()
+    nrows = sum(r1 - r0 for r0, r1, _ in tree.remainders)
+
+    def rcost(rows, cols):
+        return rows * cols * (5.0 / 2.0 + 1.0) / memop_cost
+
+    costs = [rcost(nrows, tree.ncols)]
+    for level in range(1, depth + 1):
+        nodes = _nodes_at(tree, depth - level)
+        res = sum(rcost(r1 - r0, len(c)) for nd in nodes for r0, r1, c in nd.remainders)
+        ip = sum(nd.size() for nd in nodes)
+        costs.append(res + ip)
+    return int(np.argmin(costs))
+
+
+def _interp_bytes(s, ip):
+    s.write(ip.filter.tobytes())
+    _pad128(s)
+    s.write(ip.A_ip.tobytes(order="F"))
+
+
+def _matrix(s, m, odd, Nside, lmax, chunk_size=None, eps=1e-10, num_levels=None,
+            memop_cost=20.0):
+    """Butterfly matrix of (m, odd) (wavemoth/butterfly.pyx:743-825).
+    Without chunk_size, or with num_levels 0: an identity root and one
+    remainder block.  Otherwise the compressed tree cut to num_levels
+    levels (default: choose_levels): 2^(depth - num_levels) roots, each
+    with its remainder blocks, then the heap of nodes.  The residual
+    payload headers list their block offsets once (the reference writes
+    its payloads several times over and points at the last copy; readers
+    only follow the offsets)."""
     start = s.tell()
     assert start % 16 == 0
     thetas = get_ring_thetas(Nside, positive_only=True)
     nrows = max((lmax - m - odd) // 2 + 1, 0)
     ncols = thetas.shape[0]
-    for v in (nrows, ncols, 0, 1):       # nrows, ncols, k_max, nblocks_max
+    tree = None
+    if chunk_size and nrows > 0:
+        lam = lambda_matrix(m, thetas, m + odd + 2 * (nrows - 1), eps=1e-30).T[odd::2, :]
+        tree = compress_matrix(lam, chunk_size, eps)
+        levels = choose_levels(tree, memop_cost) if num_levels is None else num_levels
+        levels = min(levels, tree.depth())
+        if levels == 0:
+            tree = None
+    if tree is None:
+        roots = [_Leaf(ncols, [(0, nrows, np.arange(ncols))])]
+        heap = list(roots)
+        k_max, nblocks_max, size = 0, 1, 0
+        count = 1
+    else:
+        skip = tree.depth() - levels
+        count = 2 ** skip
+        roots = _nodes_at(tree, skip)
+        heap = [None] * (2 ** (tree.depth() + 1) - count)
+
+        def fill(node, idx):
+            heap[idx - count] = node
+            for j, c in enumerate(node.children):
+                fill(c, 2 * idx + j)
+
+        for i, r in enumerate(roots):
+            fill(r, count + i)
+
+        def kmax(node):
+            if isinstance(node, _Leaf):
+                return node.ncols
+            return max([max(node.heights)] + [kmax(c) for c in node.children])
+
+        k_max = max(kmax(r) for r in roots)
+        nblocks_max = max(len(r.heights) for r in roots)
+        size = tree.size()
+    for v in (nrows, ncols, k_max, nblocks_max):
         _i32(s, v)
-    _i64(s, 0)                           # element_count
-    for v in (1, 1, 1, 0):               # first_level_size, heap_size, heap_first_index
+    _i64(s, size)                        # element_count
+    for v in (count, len(heap), count, 0):   # first_level_size, heap_size, heap_first_index
         _i32(s, v)
     resid_slot = s.tell()
-    _i64(s, 0)
+    for _ in roots:
+        _i64(s, 0)
     heap_slot = s.tell()
-    _i64(s, 0)
-    resid_pos = s.tell()
-    _i64(s, 1)                           # one remainder block
-    offs_pos = s.tell()
-    _i64(s, 0)
-    _i64(s, 0)
-    p0 = s.tell()
-    _block_payload(s, m, odd, thetas, nrows)
-    p1 = s.tell()
-    _pad128(s)
-    node_pos = s.tell()
-    _i32(s, 0)                           # identity node
-    _i32(s, ncols)
+    for _ in heap:
+        _i64(s, 0)
+    resid_pos, node_pos = [], []
+    for r in roots:
+        resid_pos.append(s.tell() - start)
+        _i64(s, len(r.remainders))
+        offs = s.tell()
+        for _ in range(len(r.remainders) + 1):
+            _i64(s, 0)
+        pos = []
+        for r0, r1, cols in r.remainders:
+            pos.append(s.tell() - start)
+            _block_payload(s, m, odd, thetas[cols], r0, r1)
+        pos.append(s.tell() - start)
+        end = s.tell()
+        s.seek(offs)
+        for v in pos:
+            _i64(s, v)
+        s.seek(end)
+    for node in heap:
+        _pad128(s)
+        node_pos.append(s.tell() - start)
+        if node is None or isinstance(node, _Leaf):
+            _i32(s, 0)
+            _i32(s, 0 if node is None else node.ncols)
+        else:
+            _i32(s, len(node.heights))
+            s.write(np.asarray(node.heights, np.int32).tobytes())
+            for T, B in node.pairs:
+                _interp_bytes(s, T)
+                _interp_bytes(s, B)
     end = s.tell()
-    for slot, v in ((resid_slot, resid_pos - start), (heap_slot, node_pos - start),
-                    (offs_pos, p0 - start), (offs_pos + 8, p1 - start)):
-        s.seek(slot)
+    s.seek(resid_slot)
+    for v in resid_pos:
+        _i64(s, v)
+    s.seek(heap_slot)
+    for v in node_pos:
         _i64(s, v)
     s.seek(end)
 
 
-def write_resource(target, Nside, lmax, mmax=None):
+def write_resource(target, Nside, lmax, mmax=None, chunk_size=None, eps=1e-10,
+                   num_levels=None, memop_cost=20.0):
     """Write the rev1 resource of (Nside, lmax, mmax) to a path or a binary
-    stream (wavemoth/lib.pyx:553-592).  Every (m, odd) matrix is present
-    and uncompressed."""
+    stream (wavemoth/lib.pyx:553-592).  Every (m, odd) matrix is present.
+    Without chunk_size the matrices are uncompressed (num_levels = 0);
+    with it each is butterfly-compressed over column blocks of chunk_size
+    rings to precision eps and cut to num_levels levels (default: the
+    reference's cost rule with memop_cost, as scripts/precompute.py
+    defaults: chunk size 64, tolerance 1e-10, memop cost 20)."""
     mmax = lmax if mmax is None else mmax
     if not (0 <= mmax <= lmax) or Nside < 1:
         raise ValueError("require Nside >= 1 and 0 <= mmax <= lmax")
+    if chunk_size is not None and chunk_size < 1:
+        raise ValueError("chunk_size must be >= 1")
+    opts = dict(chunk_size=chunk_size, eps=eps, num_levels=num_levels, memop_cost=memop_cost)
     if isinstance(target, (str, bytes, os.PathLike)):
         with open(target, "wb") as f:
-            return write_resource(f, Nside, lmax, mmax)
+            return write_resource(f, Nside, lmax, mmax, **opts)
     s = target if target.seekable() else None
     if s is None:
         buf = io.BytesIO()
-        write_resource(buf, Nside, lmax, mmax)
+        write_resource(buf, Nside, lmax, mmax, **opts)
         target.write(buf.getvalue())
         return
     base = s.tell()
@@ -319,7 +579,7 @@ def write_resource(target, Nside, lmax, mmax=None):
         for odd in (0, 1):
             _pad128(s)
             start = s.tell()
-            _matrix(s, m, odd, Nside, lmax)
+            _matrix(s, m, odd, Nside, lmax, **opts)
             end = s.tell()
             s.seek(table + 8 * (4 * m + 2 * odd))
             _i64(s, start - base)
