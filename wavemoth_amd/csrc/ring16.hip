@@ -93,6 +93,7 @@ int64_t ring16_blocks(int kind, int P, int K, int count) {
   return 8 * ((ux + UB - 1) / UB);
 }
 
+
 // LDS of one block: the twiddle table (double2) first, then the jobs'
 // buffers of cs-byte points
 // buffers per job: FP64 belt 1, FP64 cap 2 (A, B); FP32 twice as many, so
@@ -107,6 +108,7 @@ __host__ __device__ constexpr size_t ring16_bytes(int kind, int P, int cs) {
 int ring16_lds_slots(int kind, int P) {
   return (int)(ring16_bytes(kind, P, sizeof(double2)) / sizeof(double2));
 }
+
 
 // Z_a = (X_a + conj X_c) + i e^{2 pi i a/n} (X_a - conj X_c), c = h - a
 template <typename C>
@@ -304,6 +306,27 @@ __device__ __forceinline__ void job_lane(int tid, int *jb, int *j) {
   }
 }
 
+// Job slot and thread-in-job for the FP32 two-map belt jobs (T >= 64):
+// the two jobs of a pair alternate lanes (lane 2 i job 2 pr, lane 2 i + 1
+// job 2 pr + 1, both at j = wi 32 + i), so the maps k .. k + 1 and
+// k + 2 .. k + 3 of a pixel row are written by adjacent lanes of one store
+// instruction (32 contiguous bytes, one request where the 16-byte stores of
+// lanes l and l + 32 went out separately) and their q entries are read by
+// adjacent lanes too.  Jobs of fewer than 64 threads keep job_lane's
+// layout (side by side in a wave).
+template <int T>
+__device__ __forceinline__ void job_lane_il(int tid, int *jb, int *j) {
+  if constexpr (T >= 64) {
+    constexpr int WPP = T / 32;  // waves per job pair
+    const int l = tid & 63, w = tid >> 6;
+    const int pr = w / WPP, wi = w - pr * WPP;
+    *jb = 2 * pr + (l & 1);
+    *j = wi * 32 + (l >> 1);
+  } else {
+    job_lane<T>(tid, jb, j);
+  }
+}
+
 // pixels 2p, 2p + 1 (p = j + T rho) of maps k and (two) k + 1 of one ring;
 // out = map + k + ring offset.  With K even the two maps of a pixel row are
 // one aligned 16-byte store.
@@ -336,8 +359,9 @@ __device__ __forceinline__ void put16x2(double *out, int j, const C v0[16], cons
 // FP32 belt at K >= 2: a job is (ring pair, maps k = 2 kp and k + 1); the two
 // maps of a ring are transformed together (fft2), the northern ring first,
 // while X^S of both maps waits in registers.  Paired jobs (maps k .. k + 3)
-// share each wave, so a store instruction writes 32 bytes of each of its
-// pixel rows.  The second map's q is the next entry of the q group (k is
+// share each wave in alternate lanes (job_lane_il), so a store instruction
+// writes 32 contiguous bytes of each of its pixel rows.  The second map's q
+// is the next entry of the q group (k is
 // even; past the last map it reads the group's padding or the q tail).
 template <int P, typename C>
 __device__ __forceinline__ void belt_block_maps2(const LegTables &t, const RingTables &rt,
@@ -347,7 +371,7 @@ __device__ __forceinline__ void belt_block_maps2(const LegTables &t, const RingT
   constexpr int T = P / 16, J = RB / T, S = job_slots(P), h = P;
   using X = cx<C>;
   int jb, j;
-  job_lane<T>(threadIdx.x, &jb, &j);
+  job_lane_il<T>(threadIdx.x, &jb, &j);
   C *buf0 = lds + jb * S * ring16_bufs(kRingBelt16, sizeof(C)), *buf1 = buf0 + S;
   const int K = t.K, M = t.M;
   int pi, kp;
@@ -1079,6 +1103,7 @@ hipError_t ring16_set_lds_limits() {
 #define WM_SET(P) WM_SET_K(kRingBelt16, P) WM_SET_K(kRingCap16, P) WM_SET_K(kRingWrap16, P)
     WM_RING16_SIZES(WM_SET)
 #undef WM_SET
+
 #undef WM_SET_K
   });
   return err;
