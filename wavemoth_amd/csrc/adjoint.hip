@@ -149,13 +149,11 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
 // recurrence, 16-l chunk rescale) into the wave's 16 x 65 LDS tile; then
 // lane (li, rs) = (lane mod 16, lane / 16) sums rings 16 rs .. 16 rs + 15 of
 // l = window + li against the chunk's q (staged once per chunk and map
-// block in the wave's LDS), and two butterfly shuffles add the four ring
-// quarters.  Rows partial[g][off_m + l - m][k] belong to this wave only.
+// block in the wave's LDS), and two lane swaps add the four ring quarters.  Rows partial[g][off_m + l - m][k] belong to this wave only.
 constexpr int kAdjWaves = 4;  // waves per workgroup of k_legendre_adj
 
-__host__ __device__ inline int adj_wave_lds_doubles(int K) {
-  const int KB = K < 8 ? K : 8;
-  return 16 * 65 + 2 * 64 * KB * 2;  // tile + q (double2) of one wave
+__host__ __device__ constexpr int adj_wave_lds_doubles(int KN) {
+  return 16 * 65 + 2 * 64 * KN * 2;  // tile + q (double2) of one wave
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -164,107 +162,135 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, int G,
+// v summed over lanes i, i ^ 16, i ^ 32, i ^ 48 (the wave's four rows of 16
+// lanes), every lane getting the sum: with both operands v, permlane16_swap
+// returns (rows 0 0 2 2, rows 1 1 3 3) and permlane32_swap (halves 0 0,
+// halves 1 1), so each sum of a pair adds row 0 to row 1 and 2 to 3, then
+// the two halves.  Register moves: no LDS round trip as with __shfl_xor.
+__device__ __forceinline__ double sum_rows4(double v) {
+  unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false,
+                                             false);
+  const double s = __builtin_bit_cast(double, (unsigned long long)lo[0] |
+                                                  (unsigned long long)hi[0] << 32) +
+                   __builtin_bit_cast(double, (unsigned long long)lo[1] |
+                                                  (unsigned long long)hi[1] << 32);
+  b = __builtin_bit_cast(unsigned long long, s);
+  lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  return __builtin_bit_cast(double, (unsigned long long)lo[0] | (unsigned long long)hi[0] << 32) +
+         __builtin_bit_cast(double, (unsigned long long)lo[1] | (unsigned long long)hi[1] << 32);
+}
+
+// One pass over maps kb .. kb + KN - 1 (KN a compile-time count, so that
+// the map loops unroll and a window's LDS reads go out together instead
+// of one wait per read).
+template <int KN>
+__global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, int G, int kb,
                                                                  double2 *partial) {
   extern __shared__ double adj_lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int L = t.L, R = t.R, M = t.M, K = t.K;
-  const int KB = K < 8 ? K : 8;
   const int64_t item = (int64_t)blockIdx.x * kAdjWaves + wv;
   const int m = (int)(item / G), g = (int)(item - (int64_t)(item / G) * G);
   if (m > M) return;  // the whole wave
-  double *tile = adj_lds + (size_t)wv * adj_wave_lds_doubles(K);  // [16][65]
-  double2 *qs = (double2 *)(tile + 16 * 65);                       // [2][64][KB]
+  double *tile = adj_lds + (size_t)wv * adj_wave_lds_doubles(KN);  // [16][65]
+  double2 *qs = (double2 *)(tile + 16 * 65);                        // [2][64][KN]
   const int64_t offm = t.off[m];
   const int len = L - m + 1;
   const int lp = (len + kChunk - 1) / kChunk * kChunk;
   const double *ct = t.ctil + (offm - m);
   const double *cs = t.cscale + offm / kChunk;
-  double2 *prow = partial + ((int64_t)g * t.nlpad + offm) * K;
-  for (int j = lane; j < lp * K; j += 64) prow[j] = make_double2(0.0, 0.0);
+  double2 *prow = partial + ((int64_t)g * t.nlpad + offm) * K + kb;
+  for (int j = lane; j < lp * KN; j += 64) {
+    const int l = j / KN, kk = j - l * KN;
+    prow[(int64_t)l * K + kk] = make_double2(0.0, 0.0);
+  }
   const int li = lane & 15, rs = lane >> 4;
   const int nchunks = (R + 63) / 64;
-  for (int kb = 0; kb < K; kb += KB) {
-    const int kn = (K - kb) < KB ? (K - kb) : KB;
-    for (int c = g; c < nchunks; c += G) {
-      const int r = c * 64 + lane;
-      const bool ok = r < R;
-      const double x = ok ? t.ring_x[r] : 0.0;
-      const int sl = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
-      int lmin = sl;
-      for (int d = 32; d >= 1; d >>= 1) {
-        const int o = __shfl_xor(lmin, d);
-        lmin = o < lmin ? o : lmin;
-      }
-      lmin = __builtin_amdgcn_readfirstlane(lmin);
-      if (lmin > L) continue;
-      // the chunk's q for maps kb .. kb + kn - 1: qs[par][ring][kk]
-      wave_sync_lds();  // earlier reads of qs done
-      for (int jq = lane; jq < 2 * 64 * kn; jq += 64) {
-        const int par = jq / (64 * kn), rem = jq - par * 64 * kn;
-        const int rr = rem / kn, kk = rem - rr * kn;
-        const int rg = c * 64 + rr;
-        qs[(par * 64 + rr) * KB + kk] =
-            (rg < R) ? t.qbuf[q_index(rg, par, kb + kk, m, K, M + 1)] : make_double2(0.0, 0.0);
-      }
-      double prev = 0.0, cur = 0.0;
-      for (int l0 = lmin; l0 <= L; l0 += kChunk) {
-        // lanes = rings: L~ at l0 .. l0 + 15 into the tile (seeds sit on
-        // the 16-l grid of m)
-        if (sl == l0) {
-          const double2 v = t.seed_v[(int64_t)m * R + r];
-          prev = v.x;
-          cur = v.y;
-        }
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          tile[j * 65 + lane] = cur;
-          const double nx = fma(x, cur, -(ct[l0 + j] * prev));
-          prev = cur;
-          cur = nx;
-        }
-        const double S = cs[(l0 - m) / kChunk];
-        prev *= S;
-        cur *= S;
-        wave_sync_lds();
-        // lane (li, rs): l = l0 + li against rings 16 rs .. 16 rs + 15
-        const int l = l0 + li;
-        const int par = (l - m) & 1;
-        double2 acc[8];
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) acc[kk] = make_double2(0.0, 0.0);
-#pragma unroll 4
-        for (int u = 0; u < 16; ++u) {
-          const int rr = 16 * rs + u;
-          const double v = tile[li * 65 + rr];
-          const double2 *qr = qs + (par * 64 + rr) * KB;
-#pragma unroll
-          for (int kk = 0; kk < 8; ++kk) {
-            if (kk < kn) {
-              const double2 q = qr[kk];
-              acc[kk].x = fma(v, q.x, acc[kk].x);
-              acc[kk].y = fma(v, q.y, acc[kk].y);
-            }
-          }
-        }
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-          if (kk < kn) {
-            acc[kk].x += __shfl_xor(acc[kk].x, 16);
-            acc[kk].y += __shfl_xor(acc[kk].y, 16);
-            acc[kk].x += __shfl_xor(acc[kk].x, 32);
-            acc[kk].y += __shfl_xor(acc[kk].y, 32);
-          }
-        }
-        if (rs == 0 && l <= L) {
-          double2 *dst = prow + (int64_t)(l - m) * K + kb;
-#pragma unroll
-          for (int kk = 0; kk < 8; ++kk)
-            if (kk < kn) dst[kk] = cadd(dst[kk], acc[kk]);
-        }
-        wave_sync_lds();  // tile reads done before the next window
-      }
+  // the first chunk this wave sums into its (zeroed) rows stores them
+  // outright; later chunks add to them
+  bool first = true;
+  for (int c = g; c < nchunks; c += G) {
+    const int r = c * 64 + lane;
+    const bool ok = r < R;
+    const double x = ok ? t.ring_x[r] : 0.0;
+    const int sl = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
+    int lmin = sl;
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int o = __shfl_xor(lmin, d);
+      lmin = o < lmin ? o : lmin;
     }
+    lmin = __builtin_amdgcn_readfirstlane(lmin);
+    if (lmin > L) continue;
+    // the chunk's q for the pass's maps: qs[par][ring][kk]
+    wave_sync_lds();  // earlier reads of qs done
+#pragma unroll
+    for (int i = 0; i < 2 * KN; ++i) {
+      const int jq = lane + 64 * i;
+      const int par = jq / (64 * KN), rem = jq - par * 64 * KN;
+      const int rr = rem / KN, kk = rem - rr * KN;
+      const int rg = c * 64 + rr;
+      qs[(par * 64 + rr) * KN + kk] =
+          (rg < R) ? t.qbuf[q_index(rg, par, kb + kk, m, K, M + 1)] : make_double2(0.0, 0.0);
+    }
+    double prev = 0.0, cur = 0.0;
+    for (int l0 = lmin; l0 <= L; l0 += kChunk) {
+      // lanes = rings: L~ at l0 .. l0 + 15 into the tile (seeds sit on
+      // the 16-l grid of m)
+      if (sl == l0) {
+        const double2 v = t.seed_v[(int64_t)m * R + r];
+        prev = v.x;
+        cur = v.y;
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        tile[j * 65 + lane] = cur;
+        const double nx = fma(x, cur, -(ct[l0 + j] * prev));
+        prev = cur;
+        cur = nx;
+      }
+      const double S = cs[(l0 - m) / kChunk];
+      prev *= S;
+      cur *= S;
+      wave_sync_lds();
+      // lane (li, rs): l = l0 + li against rings 16 rs .. 16 rs + 15
+      const int l = l0 + li;
+      const int par = (l - m) & 1;
+      double2 acc[KN];
+#pragma unroll
+      for (int kk = 0; kk < KN; ++kk) acc[kk] = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int rr = 16 * rs + u;
+        const double v = tile[li * 65 + rr];
+        const double2 *qr = qs + (par * 64 + rr) * KN;
+#pragma unroll
+        for (int kk = 0; kk < KN; ++kk) {
+          const double2 q = qr[kk];
+          acc[kk].x = fma(v, q.x, acc[kk].x);
+          acc[kk].y = fma(v, q.y, acc[kk].y);
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < KN; ++kk) {
+        acc[kk].x = sum_rows4(acc[kk].x);
+        acc[kk].y = sum_rows4(acc[kk].y);
+      }
+      if (rs == 0 && l <= L) {
+        double2 *dst = prow + (int64_t)(l - m) * K;
+        if (first) {
+#pragma unroll
+          for (int kk = 0; kk < KN; ++kk) dst[kk] = acc[kk];
+        } else {
+#pragma unroll
+          for (int kk = 0; kk < KN; ++kk) dst[kk] = cadd(dst[kk], acc[kk]);
+        }
+      }
+      wave_sync_lds();  // tile reads done before the next window
+    }
+    first = false;
   }
 }
 
@@ -359,23 +385,39 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
     if (e != hipSuccess) return e;
   }
   const int64_t waves = (int64_t)(t.M + 1) * G;
-  const size_t alds = (size_t)kAdjWaves * adj_wave_lds_doubles(t.K) * sizeof(double);
+  const unsigned nblk = (unsigned)((waves + kAdjWaves - 1) / kAdjWaves);
   {
-    // largest LDS of any map count (map blocks of 8)
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-      err = hipFuncSetAttribute((const void *)k_legendre_adj,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)((size_t)kAdjWaves * adj_wave_lds_doubles(8) *
-                                      sizeof(double)));
+#define WM_ADJ_LDS(KN)                                                                    \
+  if (err == hipSuccess)                                                                  \
+    err = hipFuncSetAttribute((const void *)k_legendre_adj<KN>,                           \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,                 \
+                              (int)((size_t)kAdjWaves * adj_wave_lds_doubles(KN) * sizeof(double)));
+      WM_ADJ_LDS(1) WM_ADJ_LDS(2) WM_ADJ_LDS(3) WM_ADJ_LDS(4)
+      WM_ADJ_LDS(5) WM_ADJ_LDS(6) WM_ADJ_LDS(7) WM_ADJ_LDS(8)
+#undef WM_ADJ_LDS
     });
     if (err != hipSuccess) return err;
   }
-  hipLaunchKernelGGL(k_legendre_adj, dim3((unsigned)((waves + kAdjWaves - 1) / kAdjWaves)),
-                     dim3(64 * kAdjWaves), alds, s, t, G, partial);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  // passes of up to 8 maps
+  for (int kb = 0; kb < t.K; kb += 8) {
+    const int kn = (t.K - kb) < 8 ? (t.K - kb) : 8;
+    const size_t alds = (size_t)kAdjWaves * adj_wave_lds_doubles(kn) * sizeof(double);
+    switch (kn) {
+#define WM_ADJ_CASE(KN)                                                                   \
+  case KN:                                                                                \
+    hipLaunchKernelGGL(k_legendre_adj<KN>, dim3(nblk), dim3(64 * kAdjWaves), alds, s, t, G, kb, \
+                       partial);                                                          \
+    break;
+      WM_ADJ_CASE(1) WM_ADJ_CASE(2) WM_ADJ_CASE(3) WM_ADJ_CASE(4)
+      WM_ADJ_CASE(5) WM_ADJ_CASE(6) WM_ADJ_CASE(7) WM_ADJ_CASE(8)
+#undef WM_ADJ_CASE
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_adj_finish, dim3((unsigned)((t.nlpad + 255) / 256)), dim3(256), 0, s, t,
                      G, partial, alm);
   return hipGetLastError();
