@@ -146,14 +146,13 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
 // alone (no workgroup barrier; 4 waves per workgroup only share the launch):
 // the group's 64-ring chunks c = g, g + G, ... one after the other, for each
 // the l-range in 16-l windows.  Lanes evaluate their ring (monic
-// recurrence, 16-l chunk rescale) into the wave's 16 x 65 LDS tile; then
-// lane (li, rs) = (lane mod 16, lane / 16) sums rings 16 rs .. 16 rs + 15 of
-// l = window + li against the chunk's q (staged once per chunk and map
-// block in the wave's LDS), and two lane swaps add the four ring quarters.  Rows partial[g][off_m + l - m][k] belong to this wave only.
+// recurrence, 16-l chunk rescale) into the wave's 16 x 65 LDS tile, which
+// the matrix cores then multiply with the chunk's q (k_legendre_adj).  Rows
+// partial[g][off_m + l - m][k] belong to this wave only.
 constexpr int kAdjWaves = 4;  // waves per workgroup of k_legendre_adj
 
-__host__ __device__ constexpr int adj_wave_lds_doubles(int KN) {
-  return 16 * 65 + 2 * 64 * KN * 2;  // tile + q (double2) of one wave
+__host__ __device__ constexpr int adj_wave_lds_doubles() {
+  return 16 * 65;  // the wave's Lambda tile
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -162,52 +161,46 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// v summed over lanes i, i ^ 16, i ^ 32, i ^ 48 (the wave's four rows of 16
-// lanes), every lane getting the sum: with both operands v, permlane16_swap
-// returns (rows 0 0 2 2, rows 1 1 3 3) and permlane32_swap (halves 0 0,
-// halves 1 1), so each sum of a pair adds row 0 to row 1 and 2 to 3, then
-// the two halves.  Register moves: no LDS round trip as with __shfl_xor.
-__device__ __forceinline__ double sum_rows4(double v) {
-  unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
-  auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false,
-                                             false);
-  const double s = __builtin_bit_cast(double, (unsigned long long)lo[0] |
-                                                  (unsigned long long)hi[0] << 32) +
-                   __builtin_bit_cast(double, (unsigned long long)lo[1] |
-                                                  (unsigned long long)hi[1] << 32);
-  b = __builtin_bit_cast(unsigned long long, s);
-  lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
-  hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
-  return __builtin_bit_cast(double, (unsigned long long)lo[0] | (unsigned long long)hi[0] << 32) +
-         __builtin_bit_cast(double, (unsigned long long)lo[1] | (unsigned long long)hi[1] << 32);
-}
+typedef double adj_d4 __attribute__((ext_vector_type(4)));
 
-// One pass over maps kb .. kb + KN - 1 (KN a compile-time count, so that
-// the map loops unroll and a window's LDS reads go out together instead
-// of one wait per read).
+// One pass over maps kb .. kb + KN - 1 (KN <= 4).  A 16-l window is the
+// FP64 matrix-core product D (16 l x 16 columns) = Lambda (16 l x 64 rings)
+// . Q (64 rings x 16 columns), Q's columns the real and imaginary parts of
+// the pass's maps for the even parity (columns 0 .. 2 KN - 1) and the odd
+// parity (2 KN .. 4 KN - 1); row l keeps the columns of its parity.  Sixteen
+// v_mfma_f64_16x16x4f64 per window, four rings each: lane i holds
+// A[i % 16][i / 16] = Lambda_{l0 + i % 16}(ring 4 t + i / 16) from the tile,
+// B[i / 16][i % 16] = Q[4 t + i / 16][i % 16] (the chunk's 16 B values per
+// lane stay in registers for all its windows), and D[i / 16 + 4 j][i % 16]
+// in result register j.  The sum over the chunk's 64 rings happens inside
+// the product: no cross-lane reduction.
 template <int KN>
 __global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, int G, int kb,
                                                                  double2 *partial) {
+  static_assert(KN >= 1 && KN <= 4, "both parities of the pass's maps in 16 columns");
   extern __shared__ double adj_lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int L = t.L, R = t.R, M = t.M, K = t.K;
   const int64_t item = (int64_t)blockIdx.x * kAdjWaves + wv;
   const int m = (int)(item / G), g = (int)(item - (int64_t)(item / G) * G);
   if (m > M) return;  // the whole wave
-  double *tile = adj_lds + (size_t)wv * adj_wave_lds_doubles(KN);  // [16][65]
-  double2 *qs = (double2 *)(tile + 16 * 65);                        // [2][64][KN]
+  double *tile = adj_lds + (size_t)wv * adj_wave_lds_doubles();  // [16][65]
   const int64_t offm = t.off[m];
   const int len = L - m + 1;
   const int lp = (len + kChunk - 1) / kChunk * kChunk;
   const double *ct = t.ctil + (offm - m);
   const double *cs = t.cscale + offm / kChunk;
-  double2 *prow = partial + ((int64_t)g * t.nlpad + offm) * K + kb;
-  for (int j = lane; j < lp * KN; j += 64) {
-    const int l = j / KN, kk = j - l * KN;
-    prow[(int64_t)l * K + kk] = make_double2(0.0, 0.0);
+  double *prow = (double *)(partial + ((int64_t)g * t.nlpad + offm) * K + kb);
+  const int64_t rstride = 2 * (int64_t)K;  // doubles per l row of the partial sums
+  for (int j = lane; j < lp * 2 * KN; j += 64) {
+    const int l = j / (2 * KN), u = j - l * 2 * KN;
+    prow[l * rstride + u] = 0.0;
   }
-  const int li = lane & 15, rs = lane >> 4;
+  // this lane's B column and what it stands for
+  const int col = lane & 15, kq = lane >> 4;
+  const int cpar = col < 2 * KN ? 0 : 1;          // parity of the column
+  const int ck = (col - 2 * KN * cpar) >> 1;      // map of the column (valid if col < 4 KN)
+  const bool cval = col < 4 * KN;
   const int nchunks = (R + 63) / 64;
   // the first chunk this wave sums into its (zeroed) rows stores them
   // outright; later chunks add to them
@@ -224,16 +217,16 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, in
     }
     lmin = __builtin_amdgcn_readfirstlane(lmin);
     if (lmin > L) continue;
-    // the chunk's q for the pass's maps: qs[par][ring][kk]
-    wave_sync_lds();  // earlier reads of qs done
+    double bq[16];
 #pragma unroll
-    for (int i = 0; i < 2 * KN; ++i) {
-      const int jq = lane + 64 * i;
-      const int par = jq / (64 * KN), rem = jq - par * 64 * KN;
-      const int rr = rem / KN, kk = rem - rr * KN;
-      const int rg = c * 64 + rr;
-      qs[(par * 64 + rr) * KN + kk] =
-          (rg < R) ? t.qbuf[q_index(rg, par, kb + kk, m, K, M + 1)] : make_double2(0.0, 0.0);
+    for (int tq = 0; tq < 16; ++tq) {
+      const int rg = c * 64 + 4 * tq + kq;
+      double v = 0.0;
+      if (cval && rg < R) {
+        const double2 q = t.qbuf[q_index(rg, cpar, kb + ck, m, K, M + 1)];
+        v = (col & 1) ? q.y : q.x;
+      }
+      bq[tq] = v;
     }
     double prev = 0.0, cur = 0.0;
     for (int l0 = lmin; l0 <= L; l0 += kChunk) {
@@ -255,40 +248,22 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, in
       prev *= S;
       cur *= S;
       wave_sync_lds();
-      // lane (li, rs): l = l0 + li against rings 16 rs .. 16 rs + 15
-      const int l = l0 + li;
-      const int par = (l - m) & 1;
-      double2 acc[KN];
+      adj_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int kk = 0; kk < KN; ++kk) acc[kk] = make_double2(0.0, 0.0);
+      for (int tq = 0; tq < 16; ++tq)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(tile[col * 65 + 4 * tq + kq], bq[tq], acc, 0,
+                                                   0, 0);
+      wave_sync_lds();  // tile reads done before the next window writes it
+      if (cval) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int rr = 16 * rs + u;
-        const double v = tile[li * 65 + rr];
-        const double2 *qr = qs + (par * 64 + rr) * KN;
-#pragma unroll
-        for (int kk = 0; kk < KN; ++kk) {
-          const double2 q = qr[kk];
-          acc[kk].x = fma(v, q.x, acc[kk].x);
-          acc[kk].y = fma(v, q.y, acc[kk].y);
+        for (int j = 0; j < 4; ++j) {
+          const int l = l0 + kq + 4 * j;
+          if (l <= L && ((l - m) & 1) == cpar) {
+            double *dst = prow + (int64_t)(l - m) * rstride + 2 * ck + (col & 1);
+            *dst = first ? acc[j] : *dst + acc[j];
+          }
         }
       }
-#pragma unroll
-      for (int kk = 0; kk < KN; ++kk) {
-        acc[kk].x = sum_rows4(acc[kk].x);
-        acc[kk].y = sum_rows4(acc[kk].y);
-      }
-      if (rs == 0 && l <= L) {
-        double2 *dst = prow + (int64_t)(l - m) * K;
-        if (first) {
-#pragma unroll
-          for (int kk = 0; kk < KN; ++kk) dst[kk] = acc[kk];
-        } else {
-#pragma unroll
-          for (int kk = 0; kk < KN; ++kk) dst[kk] = cadd(dst[kk], acc[kk]);
-        }
-      }
-      wave_sync_lds();  // tile reads done before the next window
     }
     first = false;
   }
@@ -386,25 +361,10 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
   }
   const int64_t waves = (int64_t)(t.M + 1) * G;
   const unsigned nblk = (unsigned)((waves + kAdjWaves - 1) / kAdjWaves);
-  {
-    static std::once_flag once;
-    static hipError_t err = hipSuccess;
-    std::call_once(once, [] {
-#define WM_ADJ_LDS(KN)                                                                    \
-  if (err == hipSuccess)                                                                  \
-    err = hipFuncSetAttribute((const void *)k_legendre_adj<KN>,                           \
-                              hipFuncAttributeMaxDynamicSharedMemorySize,                 \
-                              (int)((size_t)kAdjWaves * adj_wave_lds_doubles(KN) * sizeof(double)));
-      WM_ADJ_LDS(1) WM_ADJ_LDS(2) WM_ADJ_LDS(3) WM_ADJ_LDS(4)
-      WM_ADJ_LDS(5) WM_ADJ_LDS(6) WM_ADJ_LDS(7) WM_ADJ_LDS(8)
-#undef WM_ADJ_LDS
-    });
-    if (err != hipSuccess) return err;
-  }
-  // passes of up to 8 maps
-  for (int kb = 0; kb < t.K; kb += 8) {
-    const int kn = (t.K - kb) < 8 ? (t.K - kb) : 8;
-    const size_t alds = (size_t)kAdjWaves * adj_wave_lds_doubles(kn) * sizeof(double);
+  const size_t alds = (size_t)kAdjWaves * adj_wave_lds_doubles() * sizeof(double);
+  // passes of up to 4 maps (both parities of a pass fill the 16 columns)
+  for (int kb = 0; kb < t.K; kb += 4) {
+    const int kn = (t.K - kb) < 4 ? (t.K - kb) : 4;
     switch (kn) {
 #define WM_ADJ_CASE(KN)                                                                   \
   case KN:                                                                                \
@@ -412,7 +372,6 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
                        partial);                                                          \
     break;
       WM_ADJ_CASE(1) WM_ADJ_CASE(2) WM_ADJ_CASE(3) WM_ADJ_CASE(4)
-      WM_ADJ_CASE(5) WM_ADJ_CASE(6) WM_ADJ_CASE(7) WM_ADJ_CASE(8)
 #undef WM_ADJ_CASE
     }
     const hipError_t e = hipGetLastError();
