@@ -53,13 +53,13 @@ struct Shape {
 std::mutex g_wisdom_mu;
 std::map<std::tuple<int, int, int, int>, Shape> g_wisdom;  // (N, L, M, K) -> shape
 
-int heuristic_kb(int K) { return K <= 4 ? K : 8; }
-int heuristic_rl(int K) { return K <= 2 ? 2 : 1; }
+int heuristic_kb(int K) { return legendre_map_block(K); }
+int heuristic_rl(int K) { return legendre_default_rl(K); }
 
 std::vector<Shape> candidate_shapes(int K) {
   std::vector<Shape> out;
   std::vector<int> kbs = {heuristic_kb(K)};
-  if (K > 4) kbs.push_back(4);
+  if (K > 4) kbs.push_back(8);
   for (int KB : kbs)
     for (int RL : {1, 2, 4})
       if (batch_shape_supported(RL, KB)) out.push_back({RL, KB});
@@ -99,6 +99,12 @@ struct _wavemoth_batch {
   std::vector<const void *> cur_ptrs;
   hipEvent_t ptr_ev = nullptr;
   bool ptr_ev_live = false;
+  // sweep classes (one per shape): litems[first .. first + count), costliest
+  // class first; leg_blocks: the most blocks of any class's launch
+  struct SweepClass {
+    int RL, KB, first, count, blocks;
+  };
+  std::vector<SweepClass> sweep;
   int leg_blocks = 0;
   int nlitems = 0, nritems = 0;
   int ring_first[2] = {0, 0}, ring_count[2] = {0, 0};
@@ -216,9 +222,9 @@ bool make_dev(DevArena &a, const std::vector<LegTables> &tabs,
   return true;
 }
 
-int sweep_blocks(int cus, int nitems) {
+int sweep_blocks(int cus, int RL, int KB, int nitems) {
   int per_cu = 0;
-  if (batch_legendre_occupancy(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (batch_legendre_occupancy(RL, KB, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
   const int need = (nitems + 3) / 4;
   return std::max(1, std::min(per_cu * cus, need));
 }
@@ -245,7 +251,7 @@ double time_shape(wavemoth_batch b, Geometry *g, int K, Shape sh) {
     std::vector<void *> ptrs = {(void *)alm, nullptr};
     if (hipMemcpy(dp, ptrs.data(), 2 * sizeof(void *), hipMemcpyHostToDevice) != hipSuccess)
       break;
-    const int nb = sweep_blocks(b->cus, d.nlitems);
+    const int nb = sweep_blocks(b->cus, sh.RL, sh.KB, d.nlitems);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess) break;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -257,7 +263,7 @@ double time_shape(wavemoth_batch b, Geometry *g, int K, Shape sh) {
       ok = launch_batch_reset(d, b->stream) == hipSuccess &&
            launch_batch_prep(d, b->stream) == hipSuccess &&
            hipEventRecord(e0, b->stream) == hipSuccess &&
-           launch_batch_legendre(d, nb, b->stream) == hipSuccess &&
+           launch_batch_legendre(d, sh.RL, sh.KB, 0, d.nlitems, nb, b->stream) == hipSuccess &&
            hipEventRecord(e1, b->stream) == hipSuccess &&
            hipEventSynchronize(e1) == hipSuccess;
       float ms = 0.f;
@@ -344,7 +350,8 @@ bool enqueue(wavemoth_batch b, hipStream_t s) {
   WM_TRY(launch_batch_reset(d, s), "batch queue reset");
   WM_TRY(launch_batch_prep(d, s), "batch prescale");
   if (!mark(b, 1, s)) return false;
-  WM_TRY(launch_batch_legendre(d, b->leg_blocks, s), "batch sweep");
+  for (const auto &c : b->sweep)
+    WM_TRY(launch_batch_legendre(d, c.RL, c.KB, c.first, c.count, c.blocks, s), "batch sweep");
   if (!mark(b, 2, s)) return false;
   for (int c = 0; c < 2; ++c)
     WM_TRY(launch_batch_rings(d, b->ring_first[c], b->ring_count[c], 1 + c, b->ring_slots[c],
@@ -417,11 +424,30 @@ bool build_batch(wavemoth_batch b) {
     return (double)b->exec_slots[r] * (5.0 + 4.0 * b->shape[r].KB);
   };
   std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  // one queue per shape, costliest shape first, requests in cost order
+  // within each
+  std::map<int, double> shape_cost;
+  for (int r : ord) shape_cost[batch_shape_index(b->shape[r].RL, b->shape[r].KB)] += cost(r);
+  std::vector<int> shapes;
+  for (auto &kv : shape_cost) shapes.push_back(kv.first);
+  std::stable_sort(shapes.begin(), shapes.end(),
+                   [&](int x, int y) { return shape_cost[x] > shape_cost[y]; });
   std::vector<BatchLegItem> litems;
-  for (int r : ord) sweep_items(b->tabs[r], r, litems);
-  if (litems.size() > (size_t)0x7fffffff) {
-    set_err("batch too large: more than 2^31 sweep items");
-    return false;
+  b->sweep.clear();
+  for (int si : shapes) {
+    _wavemoth_batch::SweepClass c{0, 0, (int)std::min<size_t>(litems.size(), 0x7fffffff), 0, 0};
+    for (int r : ord) {
+      if (batch_shape_index(b->shape[r].RL, b->shape[r].KB) != si) continue;
+      c.RL = b->shape[r].RL;
+      c.KB = b->shape[r].KB;
+      sweep_items(b->tabs[r], r, litems);
+    }
+    if (litems.size() > (size_t)0x7fffffff) {
+      set_err("batch too large: more than 2^31 sweep items");
+      return false;
+    }
+    c.count = (int)litems.size() - c.first;
+    b->sweep.push_back(c);
   }
   // Ring queues: pairs needing more than 80 KiB of LDS (one workgroup per
   // CU) form class 0, the rest class 1; costliest first within a class.
@@ -459,7 +485,7 @@ bool build_batch(wavemoth_batch b) {
         for (int64_t x = 0; x < nb; ++x) v.push_back({r, g, (int)x});
       }
     }
-    if (3 + q16.size() > (size_t)kBatchCounters) {
+    if (3 + q16.size() > (size_t)kBatchSweepCtr) {
       set_err("batch: too many ring classes");
       return false;
     }
@@ -504,7 +530,11 @@ bool build_batch(wavemoth_batch b) {
   if (!make_dev(b->arena, b->tabs, rtabs, litems, ritems, r16items, rtabs[0].twiddle,
                 &b->dev, &b->d_ptrs))
     return false;
-  b->leg_blocks = sweep_blocks(b->cus, b->nlitems);
+  b->leg_blocks = 0;
+  for (auto &c : b->sweep) {
+    c.blocks = sweep_blocks(b->cus, c.RL, c.KB, c.count);
+    b->leg_blocks = std::max(b->leg_blocks, c.blocks);
+  }
   WM_TRY(hipHostMalloc((void **)&b->h_ptrs, (size_t)2 * nreq * sizeof(void *),
                        hipHostMallocDefault),
          "pinned pointer staging");

@@ -320,9 +320,14 @@ hipError_t launch_wave_bounds(const LegTables &t, hipStream_t s) {
 }
 
 // Maps per pass (KB) from nmaps; rings per lane (RL) is a plan choice
-// (heuristic below, or measured by the autotuner).
-int legendre_map_block(int K) { return K <= 4 ? K : 8; }
-int legendre_default_rl(int K) { return K <= 2 ? 2 : 1; }
+// (heuristic below, or measured by the autotuner).  Two rings per lane and
+// at most four maps a pass: each wave then reads a coefficient row for
+// twice the rings, and the 8-map pass's registers (3 waves per SIMD at two
+// rings) are avoided; measured at Nside 2048, lmax 4096, the sweep takes
+// 140 ms at 64 maps against 222 ms with one ring and 8 maps per pass
+// (18.3 against 27.9 at 8 maps, 7.8 against 11.9 at 3).
+int legendre_map_block(int K) { return K <= 4 ? K : 4; }
+int legendre_default_rl(int K) { return 2; }
 int legendre_padded_maps(int K) {
   const int KB = legendre_map_block(K);
   return ((K + KB - 1) / KB) * KB;
@@ -378,26 +383,35 @@ __global__ __launch_bounds__(256) void k_batch_prep(BatchDev b) {
   prep_item(t, b.alm[lo], g - b.prep_off[lo], t.KB, (t.K + t.KB - 1) / t.KB);
 }
 
-#define WM_BATCH_SHAPES(X) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 3) X(4, 1)
+#define WM_BATCH_SHAPES(X) \
+  X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 8) X(4, 1)
 
-bool batch_shape_supported(int RL, int KB) {
-#define WM_SUP(R_, K_) if (RL == R_ && KB == K_) return true;
-  WM_BATCH_SHAPES(WM_SUP)
-#undef WM_SUP
-  return false;
+bool batch_shape_supported(int RL, int KB) { return batch_shape_index(RL, KB) >= 0; }
+
+int batch_shape_index(int RL, int KB) {
+  int i = 0;
+#define WM_IDX(R_, K_)                     \
+  if (RL == R_ && KB == K_) return i;      \
+  ++i;
+  WM_BATCH_SHAPES(WM_IDX)
+#undef WM_IDX
+  return -1;
 }
 
-// Persistent sweep: each wave takes items (request, m, ring chunk, map
-// block) from one global queue until it runs dry; the host orders the
-// queue costliest request first.
-__global__ __launch_bounds__(256, 6) void k_batch_legendre(BatchDev b) {
+// Persistent sweep of one shape: each wave takes items (request, m, ring
+// chunk, map block) of the shape's queue until it runs dry; the host
+// orders each queue costliest request first.  One kernel per shape, so
+// each has the registers of its own shape (as the single-plan sweep).
+template <int R_, int K_>
+__global__ __launch_bounds__(256) void k_batch_legendre(BatchDev b, int first, int count,
+                                                        int ctr) {
   const int lane = threadIdx.x & 63;
   for (;;) {
     int it = 0;
-    if (lane == 0) it = atomicAdd(&b.counters[0], 1);
+    if (lane == 0) it = atomicAdd(&b.counters[ctr], 1);
     it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-    if (it >= b.nlitems) break;
-    const WM_CONST BatchLegItem *ip = const_view(b.litems + it);
+    if (it >= count) break;
+    const WM_CONST BatchLegItem *ip = const_view(b.litems + first + it);
     const int req = ip->req, m = ip->m, w = ip->w, kb = ip->kb;
     const WM_CONST LegTables *tp = const_view(b.tabs + req);
     LegTables t;
@@ -417,13 +431,7 @@ __global__ __launch_bounds__(256, 6) void k_batch_legendre(BatchDev b) {
     t.wseedmax = tp->wseedmax;
     t.atil = tp->atil;
     t.qbuf = tp->qbuf;
-    switch (tp->RL * 16 + tp->KB) {
-#define WM_CASE(R_, K_) \
-  case 16 * R_ + K_: legendre_item<R_, K_>(t, m, w, kb, lane); break;
-      WM_BATCH_SHAPES(WM_CASE)
-#undef WM_CASE
-      default: break;
-    }
+    legendre_item<R_, K_>(t, m, w, kb, lane);
   }
 }
 
@@ -438,13 +446,28 @@ hipError_t launch_batch_prep(const BatchDev &b, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t batch_legendre_occupancy(int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_legendre, 256, 0);
+hipError_t batch_legendre_occupancy(int RL, int KB, int *blocks_per_cu) {
+#define WM_OCC(R_, K_)                                                                   \
+  if (RL == R_ && KB == K_)                                                              \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                   \
+                                                        k_batch_legendre<R_, K_>, 256, 0);
+  WM_BATCH_SHAPES(WM_OCC)
+#undef WM_OCC
+  return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_legendre(const BatchDev &b, int nb, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_legendre, dim3(nb), dim3(256), 0, s, b);
-  return hipGetLastError();
+hipError_t launch_batch_legendre(const BatchDev &b, int RL, int KB, int first, int count,
+                                 int nb, hipStream_t s) {
+  const int ctr = kBatchSweepCtr + batch_shape_index(RL, KB);
+#define WM_LAUNCH(R_, K_)                                                                \
+  if (RL == R_ && KB == K_) {                                                            \
+    hipLaunchKernelGGL((k_batch_legendre<R_, K_>), dim3(nb), dim3(256), 0, s, b, first,  \
+                       count, ctr);                                                      \
+    return hipGetLastError();                                                            \
+  }
+  WM_BATCH_SHAPES(WM_LAUNCH)
+#undef WM_LAUNCH
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_legendre(const LegTables &t, hipStream_t s) {

@@ -135,7 +135,8 @@ struct BatchRing16Item {
   int req, g, b;
 };
 // queue heads: sweep, two general ring classes, register-FFT ring classes
-constexpr int kBatchCounters = 32;
+constexpr int kBatchCounters = 64;
+constexpr int kBatchSweepCtr = 48;  // queue head of sweep shape i: kBatchSweepCtr + i
 struct BatchDev {
   int nreq = 0;
   const LegTables *tabs = nullptr;      // [nreq] per-request tables (K, KB, RL, buffers)
@@ -152,20 +153,23 @@ struct BatchDev {
   int *counters = nullptr;              // [kBatchCounters] work-queue heads
 };
 
-// (RL, KB) sweep shapes the batched kernel is compiled for.  The set is
-// kept to shapes of at most ~80 VGPRs, since the kernel holds the register
-// budget of its largest case.
+// (RL, KB) sweep shapes of the batched sweep: one persistent kernel each,
+// compiled like the single-plan sweep of that shape.
 bool batch_shape_supported(int RL, int KB);
+int batch_shape_index(int RL, int KB);  // position in the shape list, -1 if none
 
 hipError_t launch_batch_reset(const BatchDev &b, hipStream_t s);
 hipError_t launch_batch_prep(const BatchDev &b, hipStream_t s);
-hipError_t launch_batch_legendre(const BatchDev &b, int nblocks, hipStream_t s);
+// Sweep items [first, first + count) of b.litems, all of shape (RL, KB),
+// through queue head kBatchSweepCtr + batch_shape_index(RL, KB).
+hipError_t launch_batch_legendre(const BatchDev &b, int RL, int KB, int first, int count,
+                                 int nblocks, hipStream_t s);
 // Ring items [first, first + count) of b.ritems through queue head `ctr`,
 // with `slots` complex slots of dynamic LDS per block.
 hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
                               int nblocks, hipStream_t s);
 hipError_t batch_rings_set_lds_limit(int slots);
-hipError_t batch_legendre_occupancy(int *blocks_per_cu);
+hipError_t batch_legendre_occupancy(int RL, int KB, int *blocks_per_cu);
 hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu);
 
 // Mixed-precision contraction (matrix cores) of the Legendre stage.
