@@ -20,6 +20,9 @@
  *     returns/records an error; wavemoth_last_error() says why;
  *   - nthreads must be > 0 (as src/wavemoth.c:411-414) but is otherwise
  *     unused: the work runs on the current HIP device;
+ *   - Nside is limited to 4096 (the reference: 2^15, src/wavemoth.c:92):
+ *     every ring FFT stays in one workgroup's LDS (<= 8192 points); the
+ *     adjoint further to Nside <= 2048;
  *   - no resource file is needed: Lambda_l^m is generated on the GPU in FP64.
  *     A resource file passed to wavemoth_plan_to_healpix is validated against
  *     (Nside, lmax, mmax) through its header and otherwise unused.
@@ -176,7 +179,8 @@ int wavemoth_execute_device(wavemoth_plan plan, const double *d_input, double *d
    _device: d_map (npix, nmaps) float64 and d_alm (nalm, nmaps) complex128
    on the device, asynchronous on `stream`; returns 0 on success.
    wavemoth_execute_adjoint: synchronous, reads the plan's host *output*
-   array and overwrites its host *input* array. */
+   array and overwrites its host *input* array.  Plans of Nside > 2048 have
+   no adjoint (an error). */
 int wavemoth_execute_adjoint_device(wavemoth_plan plan, const double *d_map, double *d_alm,
                                     void *stream);
 void wavemoth_execute_adjoint(wavemoth_plan plan);
@@ -216,9 +220,12 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
    held by the plan, 10 (l, ring-pair) slots the Legendre sweep evaluates
    including its wave/chunk padding, 11 contraction scheme (-1 FP64,
    0 MIXED_F32, 1 MIXED_BF16X3, 2 MIXED_BF16X6, 3 MIXED_F16X3), 12 ring groups
-   of the adjoint's partial sums, 13 ring groups of the allocation-free
-   adjoint (0: it cannot run this plan).  Returns -1 for an unknown key
-   (key 11 of an FP64 plan is also -1). */
+   of the adjoint's partial sums (0: the plan has no adjoint), 13 ring
+   groups of the allocation-free adjoint (0: it cannot run this plan),
+   14..17 ring pairs transformed by the general LDS ring kernel, the
+   register-FFT belt, polar-cap and wrapping-ring kernels, 18 the longest
+   register-FFT length among them.  Returns -1 for an unknown key (key 11
+   of an FP64 plan is also -1). */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 
 /* ---- batched synthesis (extension) -------------------------------------

@@ -38,7 +38,8 @@ def test_nthreads_must_be_positive():
 
 @pytest.mark.parametrize("args,msg", [((4, 8, 8, 1, 1, None, None, 1, 0, None), "ordering"),
                                       ((0, 8, 8, 1, 1, None, None, 0, 0, None), "Nside"),
-                                      ((4096, 8, 8, 1, 1, None, None, 0, 0, None), "Nside"),
+                                      ((4097, 8, 8, 1, 1, None, None, 0, 0, None),
+                                       "Nside must be in [1, 4096]"),
                                       ((4, 8, 9, 1, 1, None, None, 0, 0, None), "mmax"),
                                       ((4, 8, 8, 0, 1, None, None, 0, 0, None), "nmaps")])
 def test_plan_argument_errors_return_null(args, msg):
@@ -114,7 +115,7 @@ def test_alm_generator_layout():
 
 @pytest.mark.parametrize("reqs,msg", [([], "nreq"),
                                       ([(4, 8, 8, 1), (0, 8, 8, 1)], "Nside"),
-                                      ([(4096, 8, 8, 1)], "Nside"),
+                                      ([(4097, 8, 8, 1)], "Nside must be in [1, 4096]"),
                                       ([(4, 8, 9, 1)], "mmax"),
                                       ([(4, 8, 8, 1), (4, 8, 8, 0)], "nmaps")])
 def test_batch_argument_errors_return_null(reqs, msg):

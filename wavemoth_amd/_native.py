@@ -95,3 +95,15 @@ def lib():
 def last_error():
     e = lib().wavemoth_last_error()
     return e.decode() if e else ""
+
+
+# wavemoth_plan_info keys, in key order
+PLAN_INFO_KEYS = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
+                  "flags", "legendre_launches", "device_bytes", "triples_executed", "scheme",
+                  "adjoint_groups", "adjoint_groups_noalloc", "rings_general", "rings_belt",
+                  "rings_cap", "rings_wrap", "max_register_fft"]
+
+
+def plan_info(plan):
+    """Every wavemoth_plan_info fact of a plan handle, by name."""
+    return {k: int(lib().wavemoth_plan_info(plan, i)) for i, k in enumerate(PLAN_INFO_KEYS)}

@@ -614,8 +614,8 @@ wavemoth_batch wavemoth_plan_batch(int nreq, const int *Nside, const int *lmax,
   }
   for (int r = 0; r < nreq; ++r) {
     const std::string at = " (request " + std::to_string(r) + ")";
-    if (Nside[r] < 1 || Nside[r] > 2048) {
-      set_err("Nside must be in [1, 2048]" + at);
+    if (Nside[r] < 1 || Nside[r] > kMaxNside) {
+      set_err("Nside must be in [1, " + std::to_string(kMaxNside) + "]" + at);
       return nullptr;
     }
     if (lmax[r] < 0 || mmax[r] < 0 || mmax[r] > lmax[r]) {

@@ -350,6 +350,11 @@ static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 namespace wm {
 
+// Whether the general LDS ring kernel (and its adjoint) holds a ring of
+// h = n/2 bins: a power-of-two h (P = 0) as h + 1 <= kMaxFFT/2 + 1 bins,
+// otherwise the Bluestein length P <= kMaxFFT.
+static bool general_fits(int h, int P) { return P == 0 ? h <= kMaxFFT / 2 : P <= kMaxFFT; }
+
 bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int M,
                          bool phase_shifts, bool wide, hipStream_t stream) {
   const int nrings = 4 * N - 1, mid = 2 * N - 1, R = 2 * N;
@@ -382,6 +387,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
   std::vector<int64_t> choff, vhoff, ch2off, vh2off;
   std::vector<int> h_index(2 * N + 1, -1), i_index(N + 1, -1);
   std::vector<int> kind(R, kRingGeneral), fftP(R, 0);
+  bool all_general = true;  // every ring within the general kernel's limits
   int64_t chtot = 0, vhtot = 0, ch2tot = 0, vh2tot = 0;
   for (int r = 0; r < R; ++r) {
     // The pair is processed with the southern ring's length and phi0, as
@@ -444,18 +450,23 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
         // the two-pass fold covers bins up to P2 - 1: h < P2 unless h is a
         // power of two (P2 = h), which takes the wrapping fold; that one
         // walks (mmax + 1) / n row blocks one after the other, so short
-        // rings (P2 < 256) stay with the general kernel
+        // rings (P2 < 256) stay with the general kernel, and so do the
+        // others at few maps where that kernel holds the ring's FFT
         if (n > M && h < P2) {
           kind[r] = kRingCap16;
-        } else if (P2 >= 256 && wide) {
+        } else if (P2 >= 256 && (wide || !general_fits(h, rp.P))) {
           kind[r] = kRingWrap16;
         }
       }
     }
     need[r] = ring_lds_slots(n, rp.P, M);
-    if (rp.P > kMaxFFT || (rp.P == 0 && h > kMaxFFT)) {
-      set_err("ring stage: Nside too large for the LDS-resident ring FFT (Nside <= 2048)");
-      return false;
+    if (!general_fits(h, rp.P)) {
+      all_general = false;
+      if (kind[r] == kRingGeneral) {
+        set_err("ring stage: a ring's FFT exceeds the LDS-resident limit of " +
+                std::to_string(kMaxFFT) + " points");
+        return false;
+      }
     }
   }
   // Launch groups: register-FFT caps and wrapping rings by FFT length
@@ -476,7 +487,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     return cost[a] > cost[b];
   });
   rt.nlaunch = 0;
-  int max_need = 0;
+  int max_need = 0, adj_need = 0;
   for (int i = 0; i < R; ++i) {
     const int r = order[i];
     if (rt.nlaunch == 0 || key(r) != key(order[rt.first[rt.nlaunch - 1]])) {
@@ -497,11 +508,15 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rt.lds[g] = kind[r] == kRingGeneral ? std::max(rt.lds[g], need[r])
                                        : ring16_lds_slots(kind[r], fftP[r]);
     rt.adj_lds[g] = std::max(rt.adj_lds[g], need[r]);
-    max_need = std::max(max_need, need[r]);
+    if (kind[r] == kRingGeneral) max_need = std::max(max_need, need[r]);
+    adj_need = std::max(adj_need, need[r]);
   }
   WM_TRY(ring16_set_lds_limits(), "register-FFT ring kernel LDS limit");
   WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");
-  WM_TRY(adjoint_set_lds_limit(max_need), "ring adjoint kernel LDS limit");
+  // the adjoint transforms every ring in the general kernel's LDS layout:
+  // rings longer than it holds (Nside > 2048) leave the plan without one
+  rt.adj_ok = all_general && adj_need <= kLdsSlots ? 1 : 0;
+  if (rt.adj_ok) WM_TRY(adjoint_set_lds_limit(adj_need), "ring adjoint kernel LDS limit");
   std::vector<double2> tw(kMaxFFT);
   for (int j = 0; j < kMaxFFT; ++j) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * j / kMaxFFT;
@@ -840,8 +855,8 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     return nullptr;
   }
   std::unique_ptr<Resource> res_owner(res);
-  if (Nside < 1 || Nside > 2048) {
-    set_err("Nside must be in [1, 2048]");
+  if (Nside < 1 || Nside > kMaxNside) {
+    set_err("Nside must be in [1, " + std::to_string(kMaxNside) + "]");
     return nullptr;
   }
   if (lmax < 0 || mmax < 0 || mmax > lmax) {
@@ -1038,7 +1053,15 @@ int wavemoth_execute_device(wavemoth_plan p, const double *d_input, double *d_ou
   return 0;
 }
 
+static bool adjoint_available(wavemoth_plan p) {
+  if (p->rt.adj_ok) return true;
+  set_err("adjoint: ring FFTs longer than " + std::to_string(kMaxFFT) +
+          " points (Nside > 2048) are not supported");
+  return false;
+}
+
 static bool run_adjoint(wavemoth_plan p, const double *d_map, double2 *d_alm, hipStream_t s) {
+  if (!adjoint_available(p)) return false;
   if (!p->adj_partial) {
     p->adj_groups = adjoint_groups(p->lt);
     if (!dalloc(p->arena, &p->adj_partial, (size_t)p->adj_groups * p->lt.nlpad * p->K,
@@ -1080,6 +1103,7 @@ int wavemoth_execute_adjoint_device_noalloc(wavemoth_plan p, const double *d_map
     set_err("adjoint without allocation: make the plan's device current first");
     return -1;
   }
+  if (!adjoint_available(p)) return -1;
   double2 *scr = adjoint_scratch(dev);
   if (!scr) return -1;
   const size_t cap = adjoint_scratch_elems();
@@ -1171,8 +1195,24 @@ int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
     case 9: return (int64_t)p->arena.bytes;
     case 10: return p->T_exec;
     case 11: return p->mix;
-    case 12: return adjoint_groups(p->lt);
-    case 13: return adjoint_groups_within(p->lt, adjoint_scratch_elems());
+    case 12: return p->rt.adj_ok ? adjoint_groups(p->lt) : 0;
+    case 13: return p->rt.adj_ok ? adjoint_groups_within(p->lt, adjoint_scratch_elems()) : 0;
+    case 14:
+    case 15:
+    case 16:
+    case 17: {
+      // ring pairs of the general, belt, cap and wrapping-ring launches
+      int64_t n = 0;
+      for (int g = 0; g < p->rt.nlaunch; ++g)
+        if (p->rt.kind[g] == key - 14) n += p->rt.count[g];
+      return n;
+    }
+    case 18: {
+      int64_t mx = 0;
+      for (int g = 0; g < p->rt.nlaunch; ++g)
+        if (p->rt.kind[g] != kRingGeneral && p->rt.fftP[g] > mx) mx = p->rt.fftP[g];
+      return mx;
+    }
     default: return -1;
   }
 }

@@ -78,10 +78,17 @@ __device__ __forceinline__ bool ring16_job(int b, int J, int jb, int K, int coun
   return *pi < count && u < G && *k < K;
 }
 
-// Blocks of a launch: belt jobs are T = P/16 threads, cap jobs 2T threads,
-// both in 512-thread blocks.
+// Cap and wrapping-ring jobs whose 2T threads would exceed a block (P =
+// 8192): one T-thread job per block runs the two halves in turn through one
+// buffer (cap_block_seq).
+__host__ __device__ constexpr bool cap_seq(int kind, int P) {
+  return kind != kRingBelt16 && P / 8 > RB;
+}
+
+// Blocks of a launch: belt jobs are T = P/16 threads, cap jobs 2T threads
+// (T for cap_seq), all in 512-thread blocks.
 __host__ __device__ constexpr int ring16_jobs(int kind, int P) {
-  return kind == kRingBelt16 ? RB / (P / 16) : RB / (P / 8);
+  return kind == kRingBelt16 ? RB / (P / 16) : cap_seq(kind, P) ? 1 : RB / (P / 8);
 }
 
 int64_t ring16_blocks(int kind, int P, int K, int count) {
@@ -97,13 +104,14 @@ int64_t ring16_blocks(int kind, int P, int K, int count) {
 // LDS of one block: the twiddle table (double2) first, then the jobs'
 // buffers of cs-byte points
 // buffers per job: FP64 belt 1, FP64 cap 2 (A, B); FP32 twice as many, so
-// that both rings of a pair are transformed together (same bytes)
-__host__ __device__ constexpr int ring16_bufs(int kind, int cs) {
-  return (kind == kRingBelt16 ? 1 : 2) * (cs == 8 ? 2 : 1);
+// that both rings of a pair are transformed together (same bytes); a
+// cap_seq job 1 in either precision
+__host__ __device__ constexpr int ring16_bufs(int kind, int P, int cs) {
+  return cap_seq(kind, P) ? 1 : (kind == kRingBelt16 ? 1 : 2) * (cs == 8 ? 2 : 1);
 }
 __host__ __device__ constexpr size_t ring16_bytes(int kind, int P, int cs) {
   return (size_t)kTwSlots * sizeof(double2) +
-         (size_t)ring16_bufs(kind, cs) * ring16_jobs(kind, P) * job_slots(P) * cs;
+         (size_t)ring16_bufs(kind, P, cs) * ring16_jobs(kind, P) * job_slots(P) * cs;
 }
 int ring16_lds_slots(int kind, int P) {
   return (int)(ring16_bytes(kind, P, sizeof(double2)) / sizeof(double2));
@@ -146,7 +154,7 @@ __device__ __forceinline__ void put16(double *out, int j, const C v[16], int K) 
     double2 *o = reinterpret_cast<double2 *>(out) + j;
 #pragma unroll
     for (int rho = 0; rho < 16; ++rho) o[T * rho] = to_d(v[rho]);
-  } else if (T >= 64 && (K & 1) == 0) {
+  } else if (T >= 64 && 2 * T <= RB && (K & 1) == 0) {
     // paired jobs (job_lane): lane l and lane l + 32 hold the same pixels of
     // maps k (even) and k + 1 (out = map + k resp. map + k + 1); after the
     // exchange lane l holds pixel 2p of both maps and lane l + 32 pixel
@@ -291,10 +299,10 @@ __device__ __forceinline__ void belt_pair(const RingPair &rp, const C *__restric
 // and lane l + 32 then handle the same positions of two adjacent maps,
 // whose q rows share a line and whose pixels share a map row, so each
 // load and store request serves both.  Smaller jobs sit side by side in a
-// wave already.
+// wave already; a job of a whole block (T = RB, P = 8192) has no partner.
 template <int T>
 __device__ __forceinline__ void job_lane(int tid, int *jb, int *j) {
-  if constexpr (T >= 64) {
+  if constexpr (T >= 64 && 2 * T <= RB) {
     constexpr int WPP = T / 32;  // waves per job pair
     const int l = tid & 63, w = tid >> 6;
     const int pr = w / WPP, wi = w - pr * WPP;
@@ -316,7 +324,7 @@ __device__ __forceinline__ void job_lane(int tid, int *jb, int *j) {
 // layout (side by side in a wave).
 template <int T>
 __device__ __forceinline__ void job_lane_il(int tid, int *jb, int *j) {
-  if constexpr (T >= 64) {
+  if constexpr (T >= 64 && 2 * T <= RB) {
     constexpr int WPP = T / 32;  // waves per job pair
     const int l = tid & 63, w = tid >> 6;
     const int pr = w / WPP, wi = w - pr * WPP;
@@ -372,7 +380,7 @@ __device__ __forceinline__ void belt_block_maps2(const LegTables &t, const RingT
   using X = cx<C>;
   int jb, j;
   job_lane_il<T>(threadIdx.x, &jb, &j);
-  C *buf0 = lds + jb * S * ring16_bufs(kRingBelt16, sizeof(C)), *buf1 = buf0 + S;
+  C *buf0 = lds + jb * S * ring16_bufs(kRingBelt16, P, sizeof(C)), *buf1 = buf0 + S;
   const int K = t.K, M = t.M;
   int pi, kp;
   const bool act = ring16_job(b, J, jb, (K + 1) / 2, count, &pi, &kp);
@@ -495,7 +503,7 @@ __device__ __forceinline__ void belt_block(const LegTables &t, const RingTables 
   }
   int jb, j;
   job_lane<T>(threadIdx.x, &jb, &j);
-  C *buf = lds + jb * S * ring16_bufs(kRingBelt16, sizeof(C));
+  C *buf = lds + jb * S * ring16_bufs(kRingBelt16, P, sizeof(C));
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
   if (!act) {
@@ -848,7 +856,7 @@ __device__ __forceinline__ void cap_store_block(const LegTables &t, const RingTa
   const RingPair rp = rt.pairs[rt.order[first + pi0]];
   const int i = rp.n / 4;
   const bool south = rp.pix_s >= 0;
-  const C *AN = lds + ring16_bufs(kRingCap16, sizeof(C)) * jj * S, *BN = AN + S;
+  const C *AN = lds + ring16_bufs(kRingCap16, P, sizeof(C)) * jj * S, *BN = AN + S;
   const C *AS = BN + S, *BS = AS + S;
   double *outn = map + kj + (int64_t)rp.pix_n * K;
   double *outs = map + kj + (int64_t)(south ? rp.pix_s : 0) * K;
@@ -954,6 +962,225 @@ __device__ __forceinline__ void cap_block_maps2(const LegTables &t, const RingTa
   }
 }
 
+// g_m = E_m + O_m (north) or E_m - O_m (south)
+template <typename C>
+__device__ __forceinline__ C gring(C e, C o, bool south) {
+  return south ? csub(e, o) : cadd(e, o);
+}
+
+// One ring's fold without wrap (n > mmax) by the T = P/16 threads of a
+// cap_seq job: F_b = g_b (b <= min(h, mmax)), then F_{n-m} += (-1)^s
+// conj g_m for h <= m <= mmax (distinct bins n - m, one writer each).  h < P
+// and mmax - h < h, so bins b = tt + T rho, rho < 16, cover both passes,
+// which run 8 rows per thread at a time.  Ends with a barrier.
+template <int P, typename C>
+__device__ __forceinline__ void cap_fold_seq(const C *__restrict__ qe, const C *__restrict__ qo,
+                                             int64_t qs, int M, int n, int s, bool south,
+                                             C *buf, int tt, bool on) {
+  using Rt = typename cx<C>::R;
+  using X = cx<C>;
+  constexpr int T = P / 16;
+  const int h = n / 2;
+  const Rt sg = s ? -1 : 1;
+#pragma unroll
+  for (int r8 = 0; r8 < 16; r8 += 8) {
+    C e[8], o[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int b = tt + T * (r8 + q);
+      const int64_t bc = b <= M ? b : M;
+      e[q] = qe[bc * qs];
+      o[q] = qo[bc * qs];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int b = tt + T * (r8 + q);
+      if (b <= h) buf[pad(b)] = (on && b <= M) ? gring(e[q], o[q], south) : X::mk(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r8 = 0; r8 < 16; r8 += 8) {
+    C e[8], o[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int m = h + tt + T * (r8 + q);
+      const int64_t mc = m <= M ? m : M;
+      e[q] = qe[mc * qs];
+      o[q] = qo[mc * qs];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int m = h + tt + T * (r8 + q);
+      if (on && m <= M) {
+        const C g = gring(e[q], o[q], south);
+        C &f = buf[pad(n - m)];
+        f.x = fma(sg, g.x, f.x);
+        f.y = fma(-sg, g.y, f.y);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// NQ bins b = tt + T (q0 + q) of one wrapping ring's fold, summed over the
+// row blocks w in order:
+//   F_b = sum_w (-1)^{s w} (g_{w n + b} + (-1)^s conj g_{w n + n - b})  (0 < b <= h),
+//   F_0 = sum_w (-1)^{s w} (g_{w n} + conj g_{w n}) - conj g_0,
+// which regroups P_b + (-1)^s conj P_{n-b} and P_0 + conj(P_0 - g_0) of
+// wrap_fold2 (P_rho = sum_w (-1)^{s w} g_{w n + rho}) by bin.
+template <int P, int NQ, typename C>
+__device__ __forceinline__ void wrap_bins_seq(const C *__restrict__ qe, const C *__restrict__ qo,
+                                              int64_t qs, int M, int n, int s, bool south,
+                                              C *buf, int tt, int q0, bool on) {
+  using Rt = typename cx<C>::R;
+  using X = cx<C>;
+  constexpr int T = P / 16;
+  const int h = n / 2;
+  const Rt sp = s ? -1 : 1;
+  C acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = X::mk(0.0, 0.0);
+  for (int w0 = 0; w0 <= M; w0 += n) {
+    const Rt se = (s && ((w0 / n) & 1)) ? -1 : 1;
+    C e[NQ], o[NQ], ec[NQ], oc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = tt + T * (q0 + q);
+      const int m = w0 + b, mp = b == 0 ? w0 : w0 + n - b;
+      const int64_t mc = m <= M ? m : M, mpc = mp <= M ? mp : M;
+      e[q] = qe[mc * qs];
+      o[q] = qo[mc * qs];
+      ec[q] = qe[mpc * qs];
+      oc[q] = qo[mpc * qs];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = tt + T * (q0 + q);
+      const int m = w0 + b, mp = b == 0 ? w0 : w0 + n - b;
+      if (b <= h && m <= M) {
+        const C g = gring(e[q], o[q], south);
+        acc[q].x = fma(se, g.x, acc[q].x);
+        acc[q].y = fma(se, g.y, acc[q].y);
+      }
+      if (b <= h && mp <= M) {
+        const C g = gring(ec[q], oc[q], south);
+        const Rt c = b == 0 ? se : se * sp;
+        acc[q].x = fma(c, g.x, acc[q].x);
+        acc[q].y = fma(-c, g.y, acc[q].y);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int b = tt + T * (q0 + q);
+    if (b > h) continue;
+    C f = on ? acc[q] : X::mk(0.0, 0.0);
+    if (b == 0 && on) {
+      const C g0 = gring(qe[0], qo[0], south);
+      f.x -= g0.x;
+      f.y += g0.y;
+    }
+    buf[pad(b)] = f;
+  }
+}
+
+// One wrapping ring's fold (n <= mmax, n <= 2P) by the T = P/16 threads of
+// a cap_seq job into buf[pad(b)], b = 0 .. h (h <= P: the bin b = P = 16 T,
+// present when n = 2P, is the 17th of thread 0).  Ends with a barrier.
+template <int P, typename C>
+__device__ __forceinline__ void wrap_fold_seq(const C *__restrict__ qe, const C *__restrict__ qo,
+                                              int64_t qs, int M, int n, int s, bool south,
+                                              C *buf, int tt, bool on) {
+  wrap_bins_seq<P, 8>(qe, qo, qs, M, n, s, south, buf, tt, 0, on);
+  wrap_bins_seq<P, 8>(qe, qo, qs, M, n, s, south, buf, tt, 8, on);
+  if (tt == 0 && n / 2 == P) wrap_bins_seq<P, 1>(qe, qo, qs, M, n, s, south, buf, tt, 16, on);
+  __syncthreads();
+}
+
+// Cap and wrapping-ring block at P = 8192 (cap_seq): one job of T = P/16 =
+// RB threads per block, one buffer.  Per ring (north, then south, each
+// folding its own g from q, so the southern ring reads q again): the fold,
+// the Z entries of both halves (A: even entries, B: odd), then the A
+// Bluestein convolution, whose results c_p y_p stay in registers, then the
+// B one through the same buffer; thread j holds A_p and B_p for the same
+// p = j + T rho (rho < 8, p < i <= P/2) and writes z_p = A_p + B_p,
+// z_{p+i} = A_p - B_p.
+template <int P, bool WRAP, typename C>
+__device__ __forceinline__ void cap_block_seq(const LegTables &t, const RingTables &rt,
+                                              int first, int count, double *__restrict__ map,
+                                              int b, C *buf, const double2 *twl) {
+  constexpr int T = P / 16;
+  static_assert(T == RB, "one job of a whole block");
+  using X = cx<C>;
+  int tt = threadIdx.x;
+  const int K = t.K, M = t.M;
+  int pi, k;
+  const bool act = ring16_job(b, 1, 0, K, count, &pi, &k);
+  if (!act) {
+    pi = 0;
+    k = 0;
+  }
+  const RingPair rp = rt.pairs[rt.order[first + pi]];
+  const bool south = rp.pix_s >= 0;
+  const int n = rp.n, i = n / 4;
+  const C *qe = (const C *)t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  const C *qo = (const C *)t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+  const double2 *chirp = rt.chirp2 + rp.chirp2_off;  // [i]
+  const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
+#pragma unroll 1
+  for (int ring = 0; ring < 2; ++ring) {
+    if (ring == 1 && !south) break;  // block-uniform
+    __syncthreads();  // the previous ring's last reads of buf done
+    if constexpr (WRAP)
+      wrap_fold_seq<P>(qe, qo, qs, M, n, rp.s, ring == 1, buf, tt, act);
+    else
+      cap_fold_seq<P>(qe, qo, qs, M, n, rp.s, ring == 1, buf, tt, act);
+    C za[8], zb[8];
+    cap_z<P>(rp, chirp, buf, tt, za);
+    cap_z<P>(rp, chirp, buf, T + tt, zb);
+    asm volatile("" : "+v"(tt));  // no table load hoisted above the barriers
+    C v[16], a[8];
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      v[rho] = za[rho];
+      v[8 + rho] = X::mk(0.0, 0.0);
+    }
+    fft16::fft<P, false>(v, buf, twl, tt);  // first barrier: cap_z reads done
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], X::of(vhat[tt + T * rho]));
+    fft16::fft<P, true>(v, buf, twl, tt);
+#pragma unroll
+    for (int rho = 0; rho < 8; ++rho) {
+      const int p = tt + T * rho;
+      a[rho] = cmul(v[rho], X::of(chirp[p < i ? p : 0]));
+      v[rho] = zb[rho];
+      v[8 + rho] = X::mk(0.0, 0.0);
+    }
+    asm volatile("" : "+v"(tt));
+    fft16::fft<P, false>(v, buf, twl, tt);
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], X::of(vhat[tt + T * rho]));
+    fft16::fft<P, true>(v, buf, twl, tt);
+    if (act) {
+      double *out = map + k + (int64_t)(ring ? rp.pix_s : rp.pix_n) * K;
+      const double2 ostep = expipi(4 * T, n);
+      double2 om = expipi(4 * tt, n);
+#pragma unroll
+      for (int rho = 0; rho < 8; ++rho) {
+        const int p = tt + T * rho;
+        if (p < i) {
+          const C bb = cmul(v[rho], X::of(cmul(chirp[p], om)));
+          put2(out, p, cadd(a[rho], bb), K);
+          put2(out, p + i, csub(a[rho], bb), K);
+        }
+        om = cmul(om, ostep);
+      }
+    }
+  }
+}
+
 // Block b of a cap (WRAP: wrapping-ring) launch over pairs
 // order[first .. first + count).
 template <int P, bool WRAP, typename C>
@@ -970,7 +1197,7 @@ __device__ __forceinline__ void cap_block(const LegTables &t, const RingTables &
   }
   const int jb = threadIdx.x / (2 * T);
   int tt = threadIdx.x % (2 * T);
-  C *bufA = lds + ring16_bufs(kRingCap16, sizeof(C)) * jb * S, *bufB = bufA + S;
+  C *bufA = lds + ring16_bufs(kRingCap16, P, sizeof(C)) * jb * S, *bufB = bufA + S;
   C *bufA2 = bufB + S, *bufB2 = bufA2 + S;  // dual only
   int pi, k;
   const bool act = ring16_job(b, J, jb, t.K, count, &pi, &k);
@@ -1032,6 +1259,8 @@ __device__ __forceinline__ void ring16_block(const LegTables &t, const RingTable
                                              C *lds, const double2 *twl) {
   if constexpr (KIND == kRingBelt16)
     belt_block<P>(t, rt, first, count, map, b, lds, twl);
+  else if constexpr (cap_seq(KIND, P))
+    cap_block_seq<P, KIND == kRingWrap16>(t, rt, first, count, map, b, lds, twl);
   else
     cap_block<P, KIND == kRingWrap16>(t, rt, first, count, map, b, lds, twl);
 }
@@ -1082,9 +1311,10 @@ __global__ __launch_bounds__(RB, 1) void k_batch_ring16(BatchDev d, int first, i
   }
 }
 
-#define WM_RING16_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#define WM_RING16_SIZES(X) \
+  X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
-bool ring16_size_ok(int P) { return P >= 16 && P <= 4096 && (P & (P - 1)) == 0; }
+bool ring16_size_ok(int P) { return P >= 16 && P <= 8192 && (P & (P - 1)) == 0; }
 
 hipError_t ring16_set_lds_limits() {
   static std::once_flag once;
@@ -1115,7 +1345,8 @@ hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &
                          int count, double *map, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   // FP32 belt jobs, and FP32 cap jobs of a whole block, hold two maps each at K >= 2
-  const bool maps2 = (kind == kRingBelt16 || (kind == kRingCap16 && ring16_jobs(kind, P) == 1)) &&
+  const bool maps2 = (kind == kRingBelt16 ||
+                      (kind == kRingCap16 && !cap_seq(kind, P) && ring16_jobs(kind, P) == 1)) &&
                      t.q32 && t.K >= 2;
   const int64_t nb = ring16_blocks(kind, P, maps2 ? (t.K + 1) / 2 : t.K, count);
   const size_t lds = ring16_bytes(kind, P, t.q32 ? sizeof(float2) : sizeof(double2));

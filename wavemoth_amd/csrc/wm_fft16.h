@@ -150,14 +150,14 @@ __device__ __forceinline__ void pass(C (&v)[NV][16], C *const (&buf)[NV], const 
   }
 }
 
-// P-point transform (16 <= P <= 4096, power of two) of the canonical
+// P-point transform (16 <= P <= 8192, power of two) of the canonical
 // layout, in place: INV = false e^{-2 pi i jk/P}, INV = true e^{+...};
 // unnormalised.  Starts with a barrier before its first LDS write, so
 // the caller may still read buf up to the call.  Every thread of the
 // block must call it.
 template <int P, bool INV, typename C>
 __device__ __forceinline__ void fft(C v[16], C *buf, const double2 *twl, int j) {
-  static_assert(P >= 16 && P <= 4096 && (P & (P - 1)) == 0, "P");
+  static_assert(P >= 16 && P <= kMaxFFT && (P & (P - 1)) == 0, "P");
   C(&vv)[1][16] = *reinterpret_cast<C(*)[1][16]>(v);
   C *const bb[1] = {buf};
   pass<P, 1, INV, 1>(vv, bb, twl, j);
@@ -168,7 +168,7 @@ __device__ __forceinline__ void fft(C v[16], C *buf, const double2 *twl, int j) 
 template <int P, bool INV, typename C>
 __device__ __forceinline__ void fft2(C (&v)[2][16], C *buf0, C *buf1, const double2 *twl,
                                      int j) {
-  static_assert(P >= 16 && P <= 4096 && (P & (P - 1)) == 0, "P");
+  static_assert(P >= 16 && P <= kMaxFFT && (P & (P - 1)) == 0, "P");
   C *const bb[2] = {buf0, buf1};
   pass<P, 1, INV, 2>(v, bb, twl, j);
 }

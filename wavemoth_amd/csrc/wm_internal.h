@@ -17,9 +17,14 @@ constexpr int kChunk = 16;
 // seed_l value of a (m, ring) column whose |Lambda| never reaches the
 // start threshold for l <= lmax.
 constexpr int kSeedNone = 0x3fffffff;
-// Largest power-of-two FFT length held in LDS by the ring kernel
-// (8192 complex doubles = 128 KiB); bounds Nside at 2048.
+// Largest power-of-two FFT length held in LDS by the ring kernels
+// (8192 complex doubles = 128 KiB).
 constexpr int kMaxFFT = 8192;
+// Largest Nside: belt rings of 4 Nside pixels are folded to 2 Nside <=
+// kMaxFFT points, polar caps' split Bluestein halves to P <= kMaxFFT.
+constexpr int kMaxNside = kMaxFFT / 2;
+// LDS of one workgroup (gfx950: 160 KiB) in complex double slots
+constexpr int kLdsSlots = 160 * 1024 / 16;
 constexpr int kRingThreads = 512;
 
 // Device tables of the Legendre stage, all owned by one plan.
@@ -64,9 +69,9 @@ __host__ __device__ inline int64_t q_elems(int R, int K, int Mp1) {
   return (int64_t)R * 2 * q_groups(K) * q_group(K) * Mp1;
 }
 // Rows of q past the last (ring pair, parity, group) that the ring kernels
-// may read (and discard): a row index up to kMaxFFT/2 past mmax stays
+// may read (and discard): a row index up to kMaxFFT past mmax stays
 // inside the allocation.
-constexpr int64_t kQTail = (int64_t)(kMaxFFT / 2) * 8;
+constexpr int64_t kQTail = (int64_t)kMaxFFT * 8;
 __host__ __device__ inline int64_t q_alloc_elems(int R, int K, int Mp1) {
   return q_elems(R, K, Mp1) + kQTail;
 }
@@ -87,7 +92,7 @@ struct RingPair {
   int r;             // ring pair index, 0 = equator
 };
 
-constexpr int kMaxLaunches = 16;
+constexpr int kMaxLaunches = 24;
 
 // Ring kernel of a launch group: the general LDS kernel (k_ring_pairs) or a
 // register-FFT kernel of ring16.hip.
@@ -118,6 +123,7 @@ struct RingTables {
   int adj_lds[kMaxLaunches] = {0};
   int kind[kMaxLaunches] = {0};  // RingKind
   int fftP[kMaxLaunches] = {0};  // FFT length of a register-FFT group
+  int adj_ok = 0;                // every pair fits k_ring_adjoint's LDS
 };
 
 // Batched execution of many independent transforms: prescale, Legendre

@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 from . import _native
-from ._native import lib, last_error
+from ._native import PLAN_INFO_KEYS, lib, last_error, plan_info  # noqa: F401
 from .alm import nalm
 
 _configured = False
@@ -119,9 +119,7 @@ class ShtPlan:
         return lib().wavemoth_get_legendre_flops(self._plan, m, odd)
 
     def info(self):
-        keys = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
-                "flags", "legendre_launches", "device_bytes", "triples_executed"]
-        return {k: int(lib().wavemoth_plan_info(self._plan, i)) for i, k in enumerate(keys)}
+        return plan_info(self._plan)
 
 
 def _c_array(x, name):

@@ -5,7 +5,7 @@ import ctypes
 import torch
 
 from . import _native
-from ._native import lib, last_error
+from ._native import lib, last_error, plan_info
 from .alm import nalm
 
 _PLANS = {}
@@ -64,10 +64,7 @@ class DevicePlan:
         return list(avg), n.value
 
     def info(self):
-        keys = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
-                "flags", "legendre_launches", "device_bytes", "triples_executed", "scheme",
-                "adjoint_groups", "adjoint_groups_noalloc"]
-        return {k: int(lib().wavemoth_plan_info(self._plan, i)) for i, k in enumerate(keys)}
+        return plan_info(self._plan)
 
 
 def random_alm(lmax, mmax, nmaps, seed, device, flat=False):
