@@ -414,7 +414,8 @@ def run_batch(args, dev, res):
     for r in range(len(reqs)):
         ri = b.request_info(r)
         shapes["%d/%d" % (ri["Nside"], ri["nmaps"])] = [ri["rings_per_lane"],
-                                                       ri["maps_per_pass"]]
+                                                       ri["maps_per_pass"],
+                                                       ri["wrap_register"]]
     res.update({
         "metric": "maps/s batched uneven synthesis (%d requests): FP64" % len(reqs),
         "value": maps * args.steps / (t1 - t0),
@@ -431,7 +432,7 @@ def run_batch(args, dev, res):
             "sweep_items": info["sweep_items"], "sweep_workgroups": info["sweep_workgroups"]}),
         "stages_ms": {"prescale": stage_ms[0], "legendre": stage_ms[1], "rings": stage_ms[2],
                       "events": nrec, "note": "direct launches; value uses graph replays"},
-        "sweep_shapes": {"Nside/nmaps -> [rings_per_lane, maps_per_pass]":
+        "sweep_shapes": {"Nside/nmaps -> [rings_per_lane, maps_per_pass, wrap_register]":
                          dict(sorted(shapes.items()))},
         "plan_s": t_plan,
         "device_bytes": info["device_bytes"],

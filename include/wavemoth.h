@@ -239,10 +239,12 @@ int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
    their seed and ring tables.  The three launches are captured once into a
    HIP graph (unless WAVEMOTH_BATCH_NO_GRAPH) and replayed by every execute.
    With WAVEMOTH_MEASURE the sweep shape of each distinct (Nside, lmax,
-   mmax, nmaps) -- rings per lane and maps per pass -- is chosen by timing
-   the candidates on the device at plan time (results are kept for the
-   process); otherwise a fixed heuristic chooses.  Executes of one batch
-   must not overlap. */
+   mmax, nmaps) -- rings per lane and maps per pass -- and the kernel of its
+   wrapping rings (general LDS kernel or register-FFT kernel) are chosen by
+   timing the candidates on the device at plan time (results are kept for
+   the process); otherwise a fixed heuristic chooses, the single plan's.  A
+   measured ring kernel other than the single plan's changes the map in the
+   last bits only.  Executes of one batch must not overlap. */
 typedef struct _wavemoth_batch *wavemoth_batch;
 #define WAVEMOTH_BATCH_NO_GRAPH 0x200
 
@@ -270,7 +272,8 @@ int wavemoth_batch_stage_times(wavemoth_batch batch, double *avg_ms, int *count)
    use, 9 sum of executed (l, ring-pair, map-pass) slots, 10 persistent
    sweep workgroups.  req >= 0, request facts: 0 Nside, 1 lmax, 2 mmax,
    3 nmaps, 4 rings per lane, 5 maps per pass, 6 T_eps, 7 dense T, 8 npix,
-   9 executed slots, 10 shape chosen by measurement (1) or heuristic (0).
+   9 executed slots, 10 shape chosen by measurement (1) or heuristic (0),
+   11 wrapping rings in the register-FFT kernels (1) or the general one (0).
    Returns -1 for an unknown key or request. */
 int64_t wavemoth_batch_info(wavemoth_batch batch, int req, int key);
 

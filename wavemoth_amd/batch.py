@@ -107,7 +107,8 @@ class BatchPlan:
 
     def request_info(self, r):
         keys = ["Nside", "lmax", "mmax", "nmaps", "rings_per_lane", "maps_per_pass",
-                "triples_eps", "triples_dense", "npix", "slots_executed", "measured"]
+                "triples_eps", "triples_dense", "npix", "slots_executed", "measured",
+                "wrap_register"]
         return {k: int(lib().wavemoth_batch_info(self._b, r, i)) for i, k in enumerate(keys)}
 
 

@@ -48,6 +48,9 @@ struct Geometry {
 
 struct Shape {
   int RL = 1, KB = 1;
+  // ring tables: 1 = wrapping rings in the register-FFT kernels (the
+  // single plan's choice from kRingWideMaps maps up), 0 = in the general one
+  int RW = 0;
 };
 
 std::mutex g_wisdom_mu;
@@ -55,6 +58,7 @@ std::map<std::tuple<int, int, int, int>, Shape> g_wisdom;  // (N, L, M, K) -> sh
 
 int heuristic_kb(int K) { return legendre_map_block(K); }
 int heuristic_rl(int K) { return legendre_default_rl(K); }
+int heuristic_rw(int K) { return K >= kRingWideMaps ? 1 : 0; }
 
 std::vector<Shape> candidate_shapes(int K) {
   std::vector<Shape> out;
@@ -78,6 +82,22 @@ void sweep_items(const LegTables &t, int req, std::vector<BatchLegItem> &out) {
     for (int kb = 0; kb < nkb; ++kb)
       for (int w = 0; w < t.W; ++w) out.push_back({req, m, w, kb});
 }
+
+// Register-FFT ring class (kind, FFT size P): r16items[first .. first + count)
+struct R16Class {
+  int kind, P, first, count, blocks;
+};
+
+// Ring-stage queues of a set of requests: the general kernel's (ring pair,
+// map) items in two classes, ritems[first[c] .. first[c] + count[c]) with
+// slots[c] complex LDS slots and blocks[c] workgroups, and the register-FFT
+// launch blocks in one class per (kind, FFT size).
+struct RingQueues {
+  int first[2] = {0, 0}, count[2] = {0, 0}, slots[2] = {0, 0}, blocks[2] = {0, 0};
+  std::vector<R16Class> r16;
+  std::vector<BatchRingItem> ritems;
+  std::vector<BatchRing16Item> r16items;
+};
 
 }  // namespace
 
@@ -106,15 +126,8 @@ struct _wavemoth_batch {
   };
   std::vector<SweepClass> sweep;
   int leg_blocks = 0;
-  int nlitems = 0, nritems = 0;
-  int ring_first[2] = {0, 0}, ring_count[2] = {0, 0};
-  int ring_slots[2] = {0, 0}, ring_blocks[2] = {0, 0};
-  // register-FFT ring classes (kind, FFT size): r16items[first .. first + count)
-  struct R16Class {
-    int kind, P, first, count, blocks;
-  };
-  std::vector<R16Class> r16;
-  int nr16items = 0;
+  int nlitems = 0;
+  RingQueues rq;  // ring-stage classes (their item lists live on the device)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   // device staging of wavemoth_execute_batch, allocated on first use
@@ -141,9 +154,8 @@ Geometry *get_geometry(wavemoth_batch b, int N, int L, int M) {
   return b->geoms.back().get();
 }
 
-// Ring tables of a geometry for requests of K maps (built on first use).
-bool ensure_rings(wavemoth_batch b, Geometry *g, int K) {
-  const int w = K >= kRingWideMaps ? 1 : 0;
+// Ring tables of a geometry in variant w (Shape::RW), built on first use.
+bool ensure_rings(wavemoth_batch b, Geometry *g, int w) {
   if (g->have_rt[w]) return true;
   if (!build_ring_geometry(b->arena, g->rt[w], g->rhost[w], g->N, g->M, true, w == 1,
                            b->stream))
@@ -151,9 +163,100 @@ bool ensure_rings(wavemoth_batch b, Geometry *g, int K) {
   g->have_rt[w] = true;
   return true;
 }
-const RingTables &rings_of(const Geometry *g, int K) { return g->rt[K >= kRingWideMaps ? 1 : 0]; }
-const RingHost &rhost_of(const Geometry *g, int K) {
-  return g->rhost[K >= kRingWideMaps ? 1 : 0];
+
+// Ring queues of requests r with ring tables rts[r] / hosts[r] and K[r]
+// maps.  General-kernel pairs needing more than 80 KiB of LDS (one
+// workgroup per CU) form class 0, the rest class 1, costliest first within
+// a class; register-FFT blocks go one queue per (kind, FFT size), the
+// largest requests first.
+bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
+                 const std::vector<const RingHost *> &hosts, const std::vector<int> &K,
+                 RingQueues *q) {
+  const int nreq = (int)rts.size();
+  const int big_slots = (80 * 1024) / (int)sizeof(double2);
+  struct RI {
+    BatchRingItem it;
+    double c;
+    int need;
+  };
+  std::vector<RI> cls[2];
+  for (int r = 0; r < nreq; ++r) {
+    const RingHost &h = *hosts[r];
+    for (int p = 0; p < (int)h.pairs.size(); ++p) {
+      if (h.kind[p] != kRingGeneral) continue;
+      for (int k = 0; k < K[r]; ++k)
+        cls[h.need[p] > big_slots ? 0 : 1].push_back({{r, p, k}, h.cost[p], h.need[p]});
+    }
+  }
+  std::map<std::pair<int, int>, std::vector<BatchRing16Item>> q16;
+  std::vector<int> rord(nreq);
+  std::iota(rord.begin(), rord.end(), 0);
+  std::stable_sort(rord.begin(), rord.end(), [&](int x, int y) {
+    return hosts[x]->npix * K[x] > hosts[y]->npix * K[y];
+  });
+  for (int r : rord) {
+    const RingTables &rt = *rts[r];
+    for (int g = 0; g < rt.nlaunch; ++g) {
+      if (rt.kind[g] == kRingGeneral || rt.count[g] == 0) continue;
+      const int64_t nb = ring16_blocks(rt.kind[g], rt.fftP[g], K[r], rt.count[g]);
+      auto &v = q16[{rt.kind[g], rt.fftP[g]}];
+      for (int64_t x = 0; x < nb; ++x) v.push_back({r, g, (int)x});
+    }
+  }
+  if (3 + q16.size() > (size_t)kBatchSweepCtr) {
+    set_err("batch: too many ring classes");
+    return false;
+  }
+  q->r16.clear();
+  q->r16items.clear();
+  for (auto &kv : q16) {
+    R16Class c;
+    c.kind = kv.first.first;
+    c.P = kv.first.second;
+    c.first = (int)q->r16items.size();
+    c.count = (int)kv.second.size();
+    int per_cu = 0;
+    WM_TRY(batch_ring16_occupancy(c.kind, c.P, &per_cu), "batch ring occupancy");
+    c.blocks = std::max(1, std::min(std::max(per_cu, 1) * cus, c.count));
+    q->r16items.insert(q->r16items.end(), kv.second.begin(), kv.second.end());
+    q->r16.push_back(c);
+  }
+  q->ritems.clear();
+  for (int c = 0; c < 2; ++c) {
+    std::stable_sort(cls[c].begin(), cls[c].end(),
+                     [](const RI &x, const RI &y) { return x.c > y.c; });
+    q->first[c] = (int)q->ritems.size();
+    q->count[c] = (int)cls[c].size();
+    q->slots[c] = 0;
+    q->blocks[c] = 0;
+    for (const RI &x : cls[c]) {
+      q->ritems.push_back(x.it);
+      q->slots[c] = std::max(q->slots[c], x.need);
+    }
+    if (q->count[c] > 0) {
+      WM_TRY(batch_rings_set_lds_limit(q->slots[c]), "batch ring LDS limit");
+      int per_cu = 0;
+      WM_TRY(batch_rings_occupancy(q->slots[c], &per_cu), "batch ring occupancy");
+      per_cu = std::max(per_cu, 1);
+      q->blocks[c] = std::max(1, std::min(per_cu * cus, q->count[c]));
+    }
+  }
+  return true;
+}
+
+hipError_t launch_ring_queues(const BatchDev &d, const RingQueues &q, hipStream_t s) {
+  for (int c = 0; c < 2; ++c) {
+    const hipError_t e =
+        launch_batch_rings(d, q.first[c], q.count[c], 1 + c, q.slots[c], q.blocks[c], s);
+    if (e != hipSuccess) return e;
+  }
+  for (size_t c = 0; c < q.r16.size(); ++c) {
+    const R16Class &x = q.r16[c];
+    const hipError_t e =
+        launch_batch_ring16(x.kind, x.P, d, x.first, x.count, 3 + (int)c, x.blocks, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 bool ensure_waves(DevArena &a, Geometry *g, int RL, hipStream_t s) {
@@ -245,9 +348,9 @@ double time_shape(wavemoth_batch b, Geometry *g, int K, Shape sh) {
     sweep_items(t, 0, items);
     BatchDev d;
     void **dp = nullptr;
-    if (!ensure_rings(b, g, K)) break;
-    if (!make_dev(a, {t}, {rings_of(g, K)}, items, {}, {}, rings_of(g, K).twiddle, &d, &dp))
-      break;
+    const int w = heuristic_rw(K);
+    if (!ensure_rings(b, g, w)) break;
+    if (!make_dev(a, {t}, {g->rt[w]}, items, {}, {}, g->rt[w].twiddle, &d, &dp)) break;
     std::vector<void *> ptrs = {(void *)alm, nullptr};
     if (hipMemcpy(dp, ptrs.data(), 2 * sizeof(void *), hipMemcpyHostToDevice) != hipSuccess)
       break;
@@ -279,8 +382,66 @@ double time_shape(wavemoth_batch b, Geometry *g, int K, Shape sh) {
   return best;
 }
 
+// Time the ring stage of one request of (g, K) in ring variant w (q
+// zeroed; best of 3 after a warm-up).  Returns milliseconds, or -1 on error.
+double time_rings(wavemoth_batch b, Geometry *g, int K, int w) {
+  DevArena a;
+  double best = -1.0;
+  do {
+    if (!ensure_rings(b, g, w)) break;
+    LegTables t;
+    if (!request_tables(b->arena, a, g, K, {heuristic_rl(K), heuristic_kb(K), w}, b->stream,
+                        &t))
+      break;
+    if (hipMemsetAsync(t.qbuf, 0, (size_t)q_alloc_elems(t.R, K, t.M + 1) * sizeof(double2),
+                       b->stream) != hipSuccess)
+      break;
+    double *map = nullptr;
+    if (!dalloc(a, &map, (size_t)g->rhost[w].npix * K, "tuning map")) break;
+    RingQueues q;
+    if (!ring_queues(b->cus, {&g->rt[w]}, {&g->rhost[w]}, {K}, &q)) break;
+    BatchDev d;
+    void **dp = nullptr;
+    if (!make_dev(a, {t}, {g->rt[w]}, {}, q.ritems, q.r16items, g->rt[w].twiddle, &d, &dp))
+      break;
+    std::vector<void *> ptrs = {nullptr, (void *)map};
+    if (hipMemcpy(dp, ptrs.data(), 2 * sizeof(void *), hipMemcpyHostToDevice) != hipSuccess)
+      break;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess) break;
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      break;
+    }
+    bool ok = true;
+    for (int rep = 0; rep < 4 && ok; ++rep) {
+      ok = launch_batch_reset(d, b->stream) == hipSuccess &&
+           hipEventRecord(e0, b->stream) == hipSuccess &&
+           launch_ring_queues(d, q, b->stream) == hipSuccess &&
+           hipEventRecord(e1, b->stream) == hipSuccess &&
+           hipEventSynchronize(e1) == hipSuccess;
+      float ms = 0.f;
+      if (ok && rep > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+        best = (best < 0 || ms < best) ? ms : best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) best = -1.0;
+  } while (false);
+  (void)hipStreamSynchronize(b->stream);
+  a.release();
+  return best;
+}
+
+// Whether the two ring variants of a geometry differ (some wrapping ring
+// changes kernel); *ok false, with the error set, when building one failed.
+bool ring_variants_differ(wavemoth_batch b, Geometry *g, bool *ok) {
+  *ok = ensure_rings(b, g, 0) && ensure_rings(b, g, 1);
+  return *ok && g->rhost[0].kind != g->rhost[1].kind;
+}
+
 bool choose_shape(wavemoth_batch b, Geometry *g, int K, Shape *out, int *measured) {
-  const Shape h = {heuristic_rl(K), heuristic_kb(K)};
+  const Shape h = {heuristic_rl(K), heuristic_kb(K), heuristic_rw(K)};
   *out = h;
   *measured = 0;
   if (!(b->flags & WAVEMOTH_MEASURE)) return true;
@@ -306,6 +467,21 @@ bool choose_shape(wavemoth_batch b, Geometry *g, int K, Shape *out, int *measure
       *out = sh;
     }
   }
+  // the ring stage: wrapping rings in the general or the register kernels
+  out->RW = h.RW;
+  bool built = true;
+  if (ring_variants_differ(b, g, &built)) {
+    double t[2];
+    for (int w = 0; w < 2; ++w) {
+      t[w] = time_rings(b, g, K, w);
+      if (t[w] < 0) {
+        set_err(std::string("autotuner: timing a ring stage failed: ") + last_err());
+        return false;
+      }
+    }
+    out->RW = t[1] < t[0] ? 1 : 0;
+  }
+  if (!built) return false;
   *measured = 1;
   std::lock_guard<std::mutex> lock(g_wisdom_mu);
   g_wisdom[key] = *out;
@@ -353,15 +529,7 @@ bool enqueue(wavemoth_batch b, hipStream_t s) {
   for (const auto &c : b->sweep)
     WM_TRY(launch_batch_legendre(d, c.RL, c.KB, c.first, c.count, c.blocks, s), "batch sweep");
   if (!mark(b, 2, s)) return false;
-  for (int c = 0; c < 2; ++c)
-    WM_TRY(launch_batch_rings(d, b->ring_first[c], b->ring_count[c], 1 + c, b->ring_slots[c],
-                              b->ring_blocks[c], s),
-           "batch ring stage");
-  for (size_t c = 0; c < b->r16.size(); ++c) {
-    const auto &x = b->r16[c];
-    WM_TRY(launch_batch_ring16(x.kind, x.P, d, x.first, x.count, 3 + (int)c, x.blocks, s),
-           "batch register-FFT ring stage");
-  }
+  WM_TRY(launch_ring_queues(d, b->rq, s), "batch ring stage");
   if (!mark(b, 3, s)) return false;
   return true;
 }
@@ -395,16 +563,17 @@ bool build_batch(wavemoth_batch b) {
   b->exec_slots.resize(nreq);
   for (int r = 0; r < nreq; ++r) {
     Geometry *g = get_geometry(b, b->N[r], b->L[r], b->M[r]);
-    if (!g || !ensure_rings(b, g, b->K[r])) return false;
+    if (!g) return false;
     b->geom[r] = -1;
     for (size_t i = 0; i < b->geoms.size(); ++i)
       if (b->geoms[i].get() == g) b->geom[r] = (int)i;
     b->nalm[r] = nalm_of(b->L[r], b->M[r]);
-    b->npix[r] = rhost_of(g, b->K[r]).npix;
+    b->npix[r] = 12 * (int64_t)b->N[r] * b->N[r];
   }
   for (int r = 0; r < nreq; ++r) {
     Geometry *g = b->geoms[b->geom[r]].get();
     if (!choose_shape(b, g, b->K[r], &b->shape[r], &b->measured[r])) return false;
+    if (!ensure_rings(b, g, b->shape[r].RW)) return false;
     if (!request_tables(b->arena, b->arena, g, b->K[r], b->shape[r], b->stream, &b->tabs[r]))
       return false;
     const int nkb = (b->K[r] + b->shape[r].KB - 1) / b->shape[r].KB;
@@ -449,86 +618,20 @@ bool build_batch(wavemoth_batch b) {
     c.count = (int)litems.size() - c.first;
     b->sweep.push_back(c);
   }
-  // Ring queues: pairs needing more than 80 KiB of LDS (one workgroup per
-  // CU) form class 0, the rest class 1; costliest first within a class.
-  const int big_slots = (80 * 1024) / (int)sizeof(double2);
-  struct RI {
-    BatchRingItem it;
-    double c;
-    int need;
-  };
-  std::vector<RI> cls[2];
+  // Ring queues over every request's ring tables
+  std::vector<const RingTables *> rts(nreq);
+  std::vector<const RingHost *> hosts(nreq);
   for (int r = 0; r < nreq; ++r) {
-    const RingHost &h = rhost_of(b->geoms[b->geom[r]].get(), b->K[r]);
-    for (int p = 0; p < (int)h.pairs.size(); ++p) {
-      if (h.kind[p] != kRingGeneral) continue;
-      for (int k = 0; k < b->K[r]; ++k)
-        cls[h.need[p] > big_slots ? 0 : 1].push_back({{r, p, k}, h.cost[p], h.need[p]});
-    }
+    const Geometry *g = b->geoms[b->geom[r]].get();
+    rts[r] = &g->rt[b->shape[r].RW];
+    hosts[r] = &g->rhost[b->shape[r].RW];
   }
-  // Register-FFT rings: the blocks of every request's register-FFT launch
-  // groups, one queue per (kind, FFT size), costliest request first.
-  std::vector<BatchRing16Item> r16items;
-  {
-    std::map<std::pair<int, int>, std::vector<BatchRing16Item>> q16;
-    std::vector<int> rord(nreq);
-    std::iota(rord.begin(), rord.end(), 0);
-    std::stable_sort(rord.begin(), rord.end(), [&](int x, int y) {
-      return (int64_t)b->npix[x] * b->K[x] > (int64_t)b->npix[y] * b->K[y];
-    });
-    for (int r : rord) {
-      const RingTables &rt = rings_of(b->geoms[b->geom[r]].get(), b->K[r]);
-      for (int g = 0; g < rt.nlaunch; ++g) {
-        if (rt.kind[g] == kRingGeneral || rt.count[g] == 0) continue;
-        const int64_t nb = ring16_blocks(rt.kind[g], rt.fftP[g], b->K[r], rt.count[g]);
-        auto &v = q16[{rt.kind[g], rt.fftP[g]}];
-        for (int64_t x = 0; x < nb; ++x) v.push_back({r, g, (int)x});
-      }
-    }
-    if (3 + q16.size() > (size_t)kBatchSweepCtr) {
-      set_err("batch: too many ring classes");
-      return false;
-    }
-    b->r16.clear();
-    for (auto &kv : q16) {
-      _wavemoth_batch::R16Class c;
-      c.kind = kv.first.first;
-      c.P = kv.first.second;
-      c.first = (int)r16items.size();
-      c.count = (int)kv.second.size();
-      int per_cu = 0;
-      WM_TRY(batch_ring16_occupancy(c.kind, c.P, &per_cu), "batch ring occupancy");
-      c.blocks = std::max(1, std::min(std::max(per_cu, 1) * b->cus, c.count));
-      r16items.insert(r16items.end(), kv.second.begin(), kv.second.end());
-      b->r16.push_back(c);
-    }
-    b->nr16items = (int)r16items.size();
-  }
-  std::vector<BatchRingItem> ritems;
-  for (int c = 0; c < 2; ++c) {
-    std::stable_sort(cls[c].begin(), cls[c].end(),
-                     [](const RI &x, const RI &y) { return x.c > y.c; });
-    b->ring_first[c] = (int)ritems.size();
-    b->ring_count[c] = (int)cls[c].size();
-    b->ring_slots[c] = 0;
-    for (const RI &x : cls[c]) {
-      ritems.push_back(x.it);
-      b->ring_slots[c] = std::max(b->ring_slots[c], x.need);
-    }
-    if (b->ring_count[c] > 0) {
-      WM_TRY(batch_rings_set_lds_limit(b->ring_slots[c]), "batch ring LDS limit");
-      int per_cu = 0;
-      WM_TRY(batch_rings_occupancy(b->ring_slots[c], &per_cu), "batch ring occupancy");
-      per_cu = std::max(per_cu, 1);
-      b->ring_blocks[c] = std::max(1, std::min(per_cu * b->cus, b->ring_count[c]));
-    }
-  }
+  if (!ring_queues(b->cus, rts, hosts, b->K, &b->rq)) return false;
   b->nlitems = (int)litems.size();
-  b->nritems = (int)ritems.size();
   std::vector<RingTables> rtabs(nreq);
-  for (int r = 0; r < nreq; ++r) rtabs[r] = rings_of(b->geoms[b->geom[r]].get(), b->K[r]);
-  if (!make_dev(b->arena, b->tabs, rtabs, litems, ritems, r16items, rtabs[0].twiddle,
-                &b->dev, &b->d_ptrs))
+  for (int r = 0; r < nreq; ++r) rtabs[r] = *rts[r];
+  if (!make_dev(b->arena, b->tabs, rtabs, litems, b->rq.ritems, b->rq.r16items,
+                rtabs[0].twiddle, &b->dev, &b->d_ptrs))
     return false;
   b->leg_blocks = 0;
   for (auto &c : b->sweep) {
@@ -758,7 +861,7 @@ int64_t wavemoth_batch_info(wavemoth_batch b, int req, int key) {
       case 4: return b->sum_flops;
       case 5: return (int64_t)b->arena.bytes;
       case 6: return b->nlitems;
-      case 7: return (int64_t)b->nritems + b->nr16items;
+      case 7: return (int64_t)b->rq.ritems.size() + (int64_t)b->rq.r16items.size();
       case 8: return b->gexec ? 1 : 0;
       case 9: return b->sum_exec;
       case 10: return b->leg_blocks;
@@ -779,6 +882,7 @@ int64_t wavemoth_batch_info(wavemoth_batch b, int req, int key) {
     case 8: return b->npix[req];
     case 9: return b->exec_slots[req];
     case 10: return b->measured[req];
+    case 11: return b->shape[req].RW;
     default: return -1;
   }
 }
