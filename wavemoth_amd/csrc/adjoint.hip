@@ -202,58 +202,74 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, in
   const int ck = (col - 2 * KN * cpar) >> 1;      // map of the column (valid if col < 4 KN)
   const bool cval = col < 4 * KN;
   const int nchunks = (R + 63) / 64;
-  // the first chunk this wave sums into its (zeroed) rows stores them
-  // outright; later chunks add to them
+  // A wave takes its chunks g, g + G, ... two at a time: both chunks'
+  // recurrences run window by window and their products add into one
+  // accumulator, so each pass stores its rows once.  The first pass stores
+  // them outright, later passes (more than two chunks per wave) add.
   bool first = true;
-  for (int c = g; c < nchunks; c += G) {
-    const int r = c * 64 + lane;
-    const bool ok = r < R;
-    const double x = ok ? t.ring_x[r] : 0.0;
-    const int sl = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
-    int lmin = sl;
-    for (int d = 32; d >= 1; d >>= 1) {
-      const int o = __shfl_xor(lmin, d);
-      lmin = o < lmin ? o : lmin;
-    }
-    lmin = __builtin_amdgcn_readfirstlane(lmin);
-    if (lmin > L) continue;
-    double bq[16];
+  for (int c0 = g; c0 < nchunks; c0 += 2 * G) {
+    double bq[2][16];
+    double x[2], prev[2] = {0.0, 0.0}, cur[2] = {0.0, 0.0};
+    int sl[2], lmin[2];
+    bool have[2];
 #pragma unroll
-    for (int tq = 0; tq < 16; ++tq) {
-      const int rg = c * 64 + 4 * tq + kq;
-      double v = 0.0;
-      if (cval && rg < R) {
-        const double2 q = t.qbuf[q_index(rg, cpar, kb + ck, m, K, M + 1)];
-        v = (col & 1) ? q.y : q.x;
+    for (int h = 0; h < 2; ++h) {
+      const int c = c0 + h * G;
+      have[h] = c < nchunks;  // wave-uniform
+      const int r = c * 64 + lane;
+      const bool ok = have[h] && r < R;
+      x[h] = ok ? t.ring_x[r] : 0.0;
+      sl[h] = ok ? t.seed_l[(int64_t)m * R + r] : kSeedNone;
+      int lm = sl[h];
+      for (int d = 32; d >= 1; d >>= 1) {
+        const int o = __shfl_xor(lm, d);
+        lm = o < lm ? o : lm;
       }
-      bq[tq] = v;
-    }
-    double prev = 0.0, cur = 0.0;
-    for (int l0 = lmin; l0 <= L; l0 += kChunk) {
-      // lanes = rings: L~ at l0 .. l0 + 15 into the tile (seeds sit on
-      // the 16-l grid of m)
-      if (sl == l0) {
-        const double2 v = t.seed_v[(int64_t)m * R + r];
-        prev = v.x;
-        cur = v.y;
-      }
+      lmin[h] = __builtin_amdgcn_readfirstlane(lm);
+      if (lmin[h] > L) have[h] = false;
 #pragma unroll
-      for (int j = 0; j < kChunk; ++j) {
-        tile[j * 65 + lane] = cur;
-        const double nx = fma(x, cur, -(ct[l0 + j] * prev));
-        prev = cur;
-        cur = nx;
+      for (int tq = 0; tq < 16; ++tq) {
+        const int rg = c * 64 + 4 * tq + kq;
+        double v = 0.0;
+        if (have[h] && cval && rg < R) {
+          const double2 q = t.qbuf[q_index(rg, cpar, kb + ck, m, K, M + 1)];
+          v = (col & 1) ? q.y : q.x;
+        }
+        bq[h][tq] = v;
       }
-      const double S = cs[(l0 - m) / kChunk];
-      prev *= S;
-      cur *= S;
-      wave_sync_lds();
+    }
+    if (!have[0] && !have[1]) continue;
+    const int lstart = have[0] ? (have[1] && lmin[1] < lmin[0] ? lmin[1] : lmin[0]) : lmin[1];
+    for (int l0 = lstart; l0 <= L; l0 += kChunk) {
       adj_d4 acc = {0.0, 0.0, 0.0, 0.0};
+      const double S = cs[(l0 - m) / kChunk];
 #pragma unroll
-      for (int tq = 0; tq < 16; ++tq)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(tile[col * 65 + 4 * tq + kq], bq[tq], acc, 0,
-                                                   0, 0);
-      wave_sync_lds();  // tile reads done before the next window writes it
+      for (int h = 0; h < 2; ++h) {
+        if (!have[h] || l0 < lmin[h]) continue;  // wave-uniform
+        // lanes = rings: L~ at l0 .. l0 + 15 into the tile (seeds sit on
+        // the 16-l grid of m)
+        const int r = (c0 + h * G) * 64 + lane;
+        if (sl[h] == l0) {
+          const double2 v = t.seed_v[(int64_t)m * R + r];
+          prev[h] = v.x;
+          cur[h] = v.y;
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          tile[j * 65 + lane] = cur[h];
+          const double nx = fma(x[h], cur[h], -(ct[l0 + j] * prev[h]));
+          prev[h] = cur[h];
+          cur[h] = nx;
+        }
+        prev[h] *= S;
+        cur[h] *= S;
+        wave_sync_lds();
+#pragma unroll
+        for (int tq = 0; tq < 16; ++tq)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(tile[col * 65 + 4 * tq + kq], bq[h][tq],
+                                                     acc, 0, 0, 0);
+        wave_sync_lds();  // tile reads done before the next chunk writes it
+      }
       if (cval) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
