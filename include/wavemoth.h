@@ -20,9 +20,11 @@
  *     returns/records an error; wavemoth_last_error() says why;
  *   - nthreads must be > 0 (as src/wavemoth.c:411-414) but is otherwise
  *     unused: the work runs on the current HIP device;
- *   - Nside is limited to 4096 (the reference: 2^15, src/wavemoth.c:92):
- *     every ring FFT stays in one workgroup's LDS (<= 8192 points); the
- *     adjoint further to Nside <= 2048;
+ *   - Nside is limited to 8192 (the reference: 2^15, src/wavemoth.c:92).
+ *     Ring FFTs of up to 8192 points stay in one workgroup's LDS (Nside <=
+ *     4096); longer ones run as four-step FFTs through a device scratch
+ *     buffer.  Batch plans are limited to Nside <= 4096, the adjoint to
+ *     Nside <= 2048;
  *   - no resource file is needed: Lambda_l^m is generated on the GPU in FP64.
  *     A resource file passed to wavemoth_plan_to_healpix is validated against
  *     (Nside, lmax, mmax) through its header and otherwise unused.
@@ -224,8 +226,9 @@ int wavemoth_stage_times(wavemoth_plan plan, double *avg_ms, int *count);
    groups of the allocation-free adjoint (0: it cannot run this plan),
    14..17 ring pairs transformed by the general LDS ring kernel, the
    register-FFT belt, polar-cap and wrapping-ring kernels, 18 the longest
-   register-FFT length among them.  Returns -1 for an unknown key (key 11
-   of an FP64 plan is also -1). */
+   register-FFT length among them, 19 ring pairs of the four-step path
+   (transforms longer than 8192 points).  Returns -1 for an unknown key
+   (key 11 of an FP64 plan is also -1). */
 int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
 
 /* ---- batched synthesis (extension) -------------------------------------

@@ -38,8 +38,8 @@ def test_nthreads_must_be_positive():
 
 @pytest.mark.parametrize("args,msg", [((4, 8, 8, 1, 1, None, None, 1, 0, None), "ordering"),
                                       ((0, 8, 8, 1, 1, None, None, 0, 0, None), "Nside"),
-                                      ((4097, 8, 8, 1, 1, None, None, 0, 0, None),
-                                       "Nside must be in [1, 4096]"),
+                                      ((8193, 8, 8, 1, 1, None, None, 0, 0, None),
+                                       "Nside must be in [1, 8192]"),
                                       ((4, 8, 9, 1, 1, None, None, 0, 0, None), "mmax"),
                                       ((4, 8, 8, 0, 1, None, None, 0, 0, None), "nmaps")])
 def test_plan_argument_errors_return_null(args, msg):

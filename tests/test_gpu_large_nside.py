@@ -1,4 +1,6 @@
-"""Nside 4096 (201,326,592 pixels, 16383 rings; rings up to 16384 pixels):
+"""Nside 4096 and 8192.
+
+Nside 4096 (201,326,592 pixels, 16383 rings; rings up to 16384 pixels):
 the ring FFTs at the LDS limit of 8192 points.
 
 - Belt rings (2 Nside = 8192-point FFTs) in the register-FFT belt kernel
@@ -15,6 +17,10 @@ the ring FFTs at the LDS limit of 8192 points.
 - A batch holding an Nside 4096 request matches its single plan bitwise.
 - The adjoint is refused at Nside 4096 (its ring kernel holds at most
   8192-point Bluestein FFTs, Nside <= 2048).
+- Nside 8192 (805,306,368 pixels): rings past the LDS limit (belt
+  transforms of 16384 points, polar-cap Bluestein lengths of 32768) in the
+  four-step long path, FP64 and FP32 against the oracle, at lmax 4096 and
+  at lmax = mmax = 20000 (the belt and the long caps wrap).
 """
 import numpy as np
 import pytest
@@ -44,34 +50,34 @@ def _sparse_alm(L, M, K, seed):
 _REF = {}
 
 
-def _oracle_map(L, M, K, seed):
+def _oracle_map(L, M, K, seed, Ns=N):
     """Oracle map of the sparse a_lm: the Legendre stage of each m
     progression into one work array (disjoint m slots), then the full ring
     stage."""
     from oracle import oracle as O
-    key = (L, M, K, seed)
+    key = (Ns, L, M, K, seed)
     if key not in _REF:
         _REF.clear()
         alm, _ = _sparse_alm(L, M, K, seed)
         work = None
         for a, b, s in _progressions(M):
-            op = O.OraclePlan(N, L, M, K, a, b, s)
+            op = O.OraclePlan(Ns, L, M, K, a, b, s)
             if work is None:
-                work = np.zeros((M + 1, 2, 2 * N, K), np.complex128)
+                work = np.zeros((M + 1, 2, 2 * Ns, K), np.complex128)
             op.legendre_stage_fast(alm, work=work)
         _REF[key] = op.c2r(op.ring_spectra(work))
         del work
     return _REF[key]
 
 
-def _run(L, M, K, flags, seed):
+def _run(L, M, K, flags, seed, Ns=N):
     import torch
     from wavemoth_amd.torch_sht import DevicePlan
     alm, ms = _sparse_alm(L, M, K, seed)
-    p = DevicePlan(N, L, M, K, flags)
+    p = DevicePlan(Ns, L, M, K, flags)
     info = p.info()
     d_alm = torch.from_numpy(alm).cuda()
-    d_map = torch.full((12 * N * N, K), float("nan"), dtype=torch.float64, device="cuda")
+    d_map = torch.full((12 * Ns * Ns, K), float("nan"), dtype=torch.float64, device="cuda")
     p.synthesize_into(d_alm, d_map)
     torch.cuda.synchronize()
     out = d_map.cpu().numpy()
@@ -79,17 +85,18 @@ def _run(L, M, K, flags, seed):
     return out, info, ms
 
 
-def _errors(got, ref):
+def _errors(got, ref, Ns=N):
     from wavemoth_amd.healpix import get_ring_offsets
-    offs = get_ring_offsets(N)
+    offs = get_ring_offsets(Ns)
     rms = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     # per ring class: polar caps past ring 2048, the rest of the caps, belt
     d2 = np.add.reduceat((got - ref) ** 2, offs[:-1], axis=0).sum(axis=1)
     r2 = np.add.reduceat(ref ** 2, offs[:-1], axis=0).sum(axis=1)
-    ring = np.arange(4 * N - 1)
-    north = np.minimum(ring, 4 * N - 2 - ring) + 1  # ring number from its pole
-    cls = {"cap>2048": (north > 2048) & (north < N), "cap<=2048": north <= 2048,
-           "belt": north >= N}
+    ring = np.arange(4 * Ns - 1)
+    north = np.minimum(ring, 4 * Ns - 2 - ring) + 1  # ring number from its pole
+    cut = Ns // 2  # caps past it: split Bluestein halves of 2 Nside points or more
+    cls = {"cap>Nside/2": (north > cut) & (north < Ns), "cap<=Nside/2": north <= cut,
+           "belt": north >= Ns}
     per = {k: float(np.sqrt(d2[c].sum() / r2[c].sum())) for k, c in cls.items()}
     return rms, per
 
@@ -166,3 +173,29 @@ def test_adjoint_refused_past_nside2048():
     out = torch.empty((33 * 34 // 2, 1), dtype=torch.complex128, device="cuda")
     with pytest.raises(RuntimeError, match="Nside > 2048"):
         p.adjoint_into(g, out)
+
+
+N8 = 8192
+
+
+@pytest.mark.parametrize("L,M,K,flags", [(4096, 4096, 1, 0), (20000, 20000, 1, 0),
+                                         (4096, 4096, 2, F32)])
+def test_nside8192_long_path(L, M, K, flags):
+    """Nside 8192: the belt (16384-point transforms) and the caps past ring
+    4096 (Bluestein length 32768) in the four-step long path; at mmax 20000
+    both wrap (n <= mmax).  FP64 within 1e-10 of the oracle, FP32 ring
+    stage (MIXED_F32) within 1e-6."""
+    name = "MIXED_F32" if flags else "FP64"
+    print("case %s Nside=%d lmax=%d mmax=%d nmaps=%d" % (name, N8, L, M, K), flush=True)
+    got, info, ms = _run(L, M, K, flags, 8192 + K, Ns=N8)
+    print({k: info[k] for k in ("rings_general", "rings_belt", "rings_cap", "rings_wrap",
+                                "rings_long", "max_register_fft")}, flush=True)
+    # long pairs: the belt (r = 0 .. N) and caps c = 4097 .. 8191
+    assert info["rings_long"] == (N8 + 1) + (N8 - 1 - 4096)
+    assert info["adjoint_groups"] == 0
+    assert np.isfinite(got).all()
+    ref = _oracle_map(L, M, K, 8192 + K, Ns=N8)
+    rms, per = _errors(got, ref, Ns=N8)
+    bound = 1e-6 if flags else 1e-10
+    print("%s rel rms %.3e per class %s" % (name, rms, per), flush=True)
+    assert rms <= bound and all(v <= bound for v in per.values()), (rms, per)

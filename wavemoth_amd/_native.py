@@ -101,7 +101,7 @@ def last_error():
 PLAN_INFO_KEYS = ["Nside", "lmax", "mmax", "nmaps", "npix", "triples_dense", "triples_eps",
                   "flags", "legendre_launches", "device_bytes", "triples_executed", "scheme",
                   "adjoint_groups", "adjoint_groups_noalloc", "rings_general", "rings_belt",
-                  "rings_cap", "rings_wrap", "max_register_fft"]
+                  "rings_cap", "rings_wrap", "max_register_fft", "rings_long"]
 
 
 def plan_info(plan):

@@ -16,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBNAME = "libwavemoth_hip.so"
 SOURCES = ["plan.hip", "batch.hip", "legendre.hip", "mixed.hip", "ring.hip", "ring16.hip",
+           "ring_long.hip",
            "adjoint.hip", "strip.hip", "resource.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -160,6 +160,10 @@ bool ensure_rings(wavemoth_batch b, Geometry *g, int w) {
   if (!build_ring_geometry(b->arena, g->rt[w], g->rhost[w], g->N, g->M, true, w == 1,
                            b->stream))
     return false;
+  if (!g->rhost[w].lr.groups.empty()) {  // none up to Nside kMaxNside
+    set_err("batch: rings longer than the register-FFT kernels hold");
+    return false;
+  }
   g->have_rt[w] = true;
   return true;
 }

@@ -404,26 +404,13 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rp.chirp2_off = rp.vhat2_off = -1;
     rp.P2 = 0;
     const int h = n / 2;
+    rp.chirp_off = rp.vhat_off = 0;  // set below for the rings with tables
     if (is_pow2(h)) {
       rp.P = 0;
-      rp.chirp_off = rp.vhat_off = 0;
       cost[r] = (double)h * std::log2((double)h + 1.0) + (double)(M + 1);
     } else {
-      const int P = next_pow2(2 * h - 1);
-      if (h_index[h] < 0) {
-        h_index[h] = (int)hlist.size();
-        hlist.push_back(h);
-        plist.push_back(P);
-        choff.push_back(chtot);
-        vhoff.push_back(vhtot);
-        chtot += h;
-        vhtot += P;
-      }
-      const int hi = h_index[h];
-      rp.P = P;
-      rp.chirp_off = choff[hi];
-      rp.vhat_off = vhoff[hi];
-      cost[r] = 2.0 * (double)P * std::log2((double)P) + (double)(M + 1);
+      rp.P = next_pow2(2 * h - 1);
+      cost[r] = 2.0 * (double)rp.P * std::log2((double)rp.P) + (double)(M + 1);
     }
     if (is_pow2(h) && M <= h && ring16_size_ok(h)) {
       // non-aliasing power-of-two ring: one FFT of h points
@@ -463,11 +450,41 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     if (!general_fits(h, rp.P)) {
       all_general = false;
       if (kind[r] == kRingGeneral) {
-        set_err("ring stage: a ring's FFT exceeds the LDS-resident limit of " +
-                std::to_string(kMaxFFT) + " points");
-        return false;
+        // longer than the LDS holds: the four-step path
+        const int PL = rp.P == 0 ? h : rp.P;
+        if (!long_size_ok(PL)) {
+          set_err("ring stage: a ring's transform of " + std::to_string(PL) +
+                  " points exceeds the supported lengths");
+          return false;
+        }
+        kind[r] = kRingLong;
+        fftP[r] = PL;
       }
     }
+  }
+  // the adjoint transforms every ring in the general kernel's LDS layout:
+  // rings longer than it holds (Nside > 2048) leave the plan without one
+  int adj_need = 0;
+  for (int r = 0; r < R; ++r) adj_need = std::max(adj_need, need[r]);
+  rt.adj_ok = all_general && adj_need <= kLdsSlots ? 1 : 0;
+  // Bluestein tables (chirp of h entries, kernel spectrum of P) of each
+  // distinct h among the rings that use them: the general kernel's and the
+  // long path's, and every ring's when the plan has an adjoint
+  for (int r = 0; r < R; ++r) {
+    RingPair &rp = pairs[r];
+    if (rp.P == 0 || !(kind[r] == kRingGeneral || kind[r] == kRingLong || rt.adj_ok)) continue;
+    const int h = rp.n / 2;
+    if (h_index[h] < 0) {
+      h_index[h] = (int)hlist.size();
+      hlist.push_back(h);
+      plist.push_back(rp.P);
+      choff.push_back(chtot);
+      vhoff.push_back(vhtot);
+      chtot += h;
+      vhtot += rp.P;
+    }
+    rp.chirp_off = choff[h_index[h]];
+    rp.vhat_off = vhoff[h_index[h]];
   }
   // Launch groups: register-FFT caps and wrapping rings by FFT length
   // (largest first), the belt, then the general kernel's rings whose LDS
@@ -475,6 +492,8 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
   // within each.
   const int big_slots = (80 * 1024) / (int)sizeof(double2);
   auto key = [&](int r) -> int64_t {
+    if (kind[r] == kRingLong)  // one group per (length, Bluestein or not)
+      return 5 * 100000 + 2 * (int64_t)std::log2((double)fftP[r]) + (pairs[r].P ? 1 : 0);
     if (kind[r] == kRingCap16) return 0 * 100000 + (8192 - fftP[r]);
     if (kind[r] == kRingWrap16) return 1 * 100000 + (8192 - fftP[r]);
     if (kind[r] == kRingBelt16) return 2 * 100000 + (8192 - fftP[r]);
@@ -487,7 +506,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     return cost[a] > cost[b];
   });
   rt.nlaunch = 0;
-  int max_need = 0, adj_need = 0;
+  int max_need = 0;
   for (int i = 0; i < R; ++i) {
     const int r = order[i];
     if (rt.nlaunch == 0 || key(r) != key(order[rt.first[rt.nlaunch - 1]])) {
@@ -506,16 +525,13 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rt.kind[g] = kind[r];
     rt.fftP[g] = fftP[r];
     rt.lds[g] = kind[r] == kRingGeneral ? std::max(rt.lds[g], need[r])
-                                       : ring16_lds_slots(kind[r], fftP[r]);
+                : kind[r] == kRingLong   ? 0
+                                         : ring16_lds_slots(kind[r], fftP[r]);
     rt.adj_lds[g] = std::max(rt.adj_lds[g], need[r]);
     if (kind[r] == kRingGeneral) max_need = std::max(max_need, need[r]);
-    adj_need = std::max(adj_need, need[r]);
   }
   WM_TRY(ring16_set_lds_limits(), "register-FFT ring kernel LDS limit");
   WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");
-  // the adjoint transforms every ring in the general kernel's LDS layout:
-  // rings longer than it holds (Nside > 2048) leave the plan without one
-  rt.adj_ok = all_general && adj_need <= kLdsSlots ? 1 : 0;
   if (rt.adj_ok) WM_TRY(adjoint_set_lds_limit(adj_need), "ring adjoint kernel LDS limit");
   std::vector<double2> tw(kMaxFFT);
   for (int j = 0; j < kMaxFFT; ++j) {
@@ -529,16 +545,84 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
   if (!dalloc(a, &rt.vhat, (size_t)vhtot, "vhat")) return false;
   if (!dalloc(a, &rt.chirp2, (size_t)ch2tot, "split chirp")) return false;
   if (!dalloc(a, &rt.vhat2, (size_t)vh2tot, "split vhat")) return false;
+  // the long path: its (ring pair, hemisphere) entries by launch group, and
+  // two scratch buffers (at most 2^26 complex entries each, at least one
+  // vector of the longest transform)
+  LongRings &lr = host.lr;
+  lr = LongRings();
+  std::vector<LongVec> lvecs;
+  int64_t lwant = 0;
+  int maxP = 0;
+  for (int g = 0; g < rt.nlaunch; ++g) {
+    if (rt.kind[g] != kRingLong) continue;
+    LongGroup lg{(int)lvecs.size(), 0, rt.fftP[g], pairs[order[rt.first[g]]].P != 0};
+    for (int i = rt.first[g]; i < rt.first[g] + rt.count[g]; ++i) {
+      const int r = order[i];
+      lvecs.push_back({r, 0});
+      if (pairs[r].pix_s >= 0) lvecs.push_back({r, 1});
+    }
+    lg.count = (int)lvecs.size() - lg.first;
+    lr.groups.push_back(lg);
+    lwant += (int64_t)lg.count * lg.P;
+    maxP = std::max(maxP, lg.P);
+  }
+  for (size_t i = 0; i < hlist.size(); ++i)
+    if (plist[i] > kMaxFFT) {
+      lwant = std::max(lwant, (int64_t)plist[i]);
+      maxP = std::max(maxP, plist[i]);
+    }
+  if (maxP > 0) {
+    WM_TRY(long_set_lds_limits(), "long-ring kernel LDS limit");
+    lr.elems = std::max<int64_t>(maxP, std::min<int64_t>(lwant, (int64_t)1 << 26));
+    if (!upload(a, &lr.vecs, lvecs, "long-ring entries")) return false;
+    if (!dalloc(a, &lr.u, (size_t)lr.elems, "long-ring scratch")) return false;
+    if (!dalloc(a, &lr.v, (size_t)lr.elems, "long-ring scratch")) return false;
+  }
   if (!hlist.empty()) {
-    int *d_h = nullptr, *d_p = nullptr;
-    int64_t *d_co = nullptr, *d_vo = nullptr;
-    if (!upload(a, &d_h, hlist, "hlist")) return false;
-    if (!upload(a, &d_p, plist, "plist")) return false;
-    if (!upload(a, &d_co, choff, "choff")) return false;
-    if (!upload(a, &d_vo, vhoff, "vhoff")) return false;
-    WM_TRY(launch_bluestein_prep(rt.twiddle, rt.chirp, rt.vhat, d_h, d_co, d_vo, d_p,
-                                 (int)hlist.size(), stream),
-           "bluestein prep");
+    // short lengths by the LDS prep kernel, long ones by the four-step FFT
+    std::vector<int> hs, ps;
+    std::vector<int64_t> cs, vs;
+    for (size_t i = 0; i < hlist.size(); ++i) {
+      if (plist[i] <= kMaxFFT) {
+        hs.push_back(hlist[i]);
+        ps.push_back(plist[i]);
+        cs.push_back(choff[i]);
+        vs.push_back(vhoff[i]);
+      }
+    }
+    if (!hs.empty()) {
+      int *d_h = nullptr, *d_p = nullptr;
+      int64_t *d_co = nullptr, *d_vo = nullptr;
+      if (!upload(a, &d_h, hs, "hlist")) return false;
+      if (!upload(a, &d_p, ps, "plist")) return false;
+      if (!upload(a, &d_co, cs, "choff")) return false;
+      if (!upload(a, &d_vo, vs, "vhoff")) return false;
+      WM_TRY(launch_bluestein_prep(rt.twiddle, rt.chirp, rt.vhat, d_h, d_co, d_vo, d_p,
+                                   (int)hs.size(), stream),
+             "bluestein prep");
+    }
+    std::vector<int> lens;
+    for (size_t i = 0; i < hlist.size(); ++i)
+      if (plist[i] > kMaxFFT && std::find(lens.begin(), lens.end(), plist[i]) == lens.end())
+        lens.push_back(plist[i]);
+    for (int P : lens) {
+      std::vector<int> hP;
+      std::vector<int64_t> cP, vP;
+      for (size_t i = 0; i < hlist.size(); ++i)
+        if (plist[i] == P) {
+          hP.push_back(hlist[i]);
+          cP.push_back(choff[i]);
+          vP.push_back(vhoff[i]);
+        }
+      int *d_h = nullptr;
+      int64_t *d_co = nullptr, *d_vo = nullptr;
+      if (!upload(a, &d_h, hP, "long hlist")) return false;
+      if (!upload(a, &d_co, cP, "long choff")) return false;
+      if (!upload(a, &d_vo, vP, "long vhoff")) return false;
+      WM_TRY(launch_long_bluestein_prep(rt.twiddle, rt.chirp, rt.vhat, d_h, d_co, d_vo,
+                                        (int)hP.size(), P, lr.u, lr.v, lr.elems, stream),
+             "long bluestein prep");
+    }
     WM_TRY(hipStreamSynchronize(stream), "bluestein prep");
   }
   if (!h2list.empty()) {
@@ -776,6 +860,8 @@ static bool build_resource_apply(wavemoth_plan p) {
 
 static bool run_rings(wavemoth_plan p, double *d_map, hipStream_t s) {
   WM_TRY(launch_ring_synth(p->lt, p->rt, d_map, s), "ring kernel");
+  if (!p->rhost.lr.groups.empty())
+    WM_TRY(launch_long_rings(p->lt, p->rt, p->rhost.lr, d_map, s), "long-ring kernels");
   if (p->profiling && !p->ev_used.empty() && p->ev_used.back()[2] != nullptr)
     if (!mark(p, 3, s)) return false;
   return true;
@@ -855,8 +941,8 @@ wavemoth_plan wavemoth_plan_to_healpix(int Nside, int lmax, int mmax, int nmaps,
     return nullptr;
   }
   std::unique_ptr<Resource> res_owner(res);
-  if (Nside < 1 || Nside > kMaxNside) {
-    set_err("Nside must be in [1, " + std::to_string(kMaxNside) + "]");
+  if (Nside < 1 || Nside > kMaxNsidePlan) {
+    set_err("Nside must be in [1, " + std::to_string(kMaxNsidePlan) + "]");
     return nullptr;
   }
   if (lmax < 0 || mmax < 0 || mmax > lmax) {
@@ -1210,8 +1296,16 @@ int64_t wavemoth_plan_info(wavemoth_plan p, int key) {
     case 18: {
       int64_t mx = 0;
       for (int g = 0; g < p->rt.nlaunch; ++g)
-        if (p->rt.kind[g] != kRingGeneral && p->rt.fftP[g] > mx) mx = p->rt.fftP[g];
+        if (p->rt.kind[g] != kRingGeneral && p->rt.kind[g] != kRingLong &&
+            p->rt.fftP[g] > mx)
+          mx = p->rt.fftP[g];
       return mx;
+    }
+    case 19: {
+      int64_t n = 0;
+      for (int g = 0; g < p->rt.nlaunch; ++g)
+        if (p->rt.kind[g] == kRingLong) n += p->rt.count[g];
+      return n;
     }
     default: return -1;
   }

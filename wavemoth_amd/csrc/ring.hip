@@ -355,7 +355,8 @@ hipError_t ring_set_lds_limit(int slots) {
 hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *map,
                              hipStream_t s) {
   for (int i = 0; i < rt.nlaunch; ++i) {
-    if (rt.count[i] == 0) continue;
+    // the long path (launch_long_rings) runs the kRingLong groups
+    if (rt.count[i] == 0 || rt.kind[i] == kRingLong) continue;
     hipError_t e;
     if (rt.kind[i] != kRingGeneral) {
       e = launch_ring16(rt.kind[i], rt.fftP[i], t, rt, rt.first[i], rt.count[i], map, s);

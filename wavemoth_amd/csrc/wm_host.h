@@ -79,6 +79,22 @@ bool build_leg_geometry(DevArena &a, LegTables &t, int N, int L, int M, hipStrea
 // the number of (l, ring) slots the sweep executes with them.
 bool build_leg_waves(DevArena &a, LegTables &t, int RL, hipStream_t s, int64_t *T_exec);
 
+// Rings of the long path (kRingLong), in groups of one transform length P
+// (P = h for a power-of-two h, else the Bluestein length): entries
+// vecs[first .. first + count) of (ring pair, hemisphere); with K maps a
+// group is count K vectors, (entry e, map k) = vector e K + k.  u, v: two
+// scratch buffers of elems complex entries each.
+struct LongGroup {
+  int first, count, P;
+  bool blue;
+};
+struct LongRings {
+  std::vector<LongGroup> groups;
+  LongVec *vecs = nullptr;
+  double2 *u = nullptr, *v = nullptr;
+  int64_t elems = 0;
+};
+
 // Host copy of a geometry's ring-pair table with each pair's LDS need and
 // cost estimate (used to order work).
 struct RingHost {
@@ -87,7 +103,21 @@ struct RingHost {
   std::vector<double> cost;  // relative cost of the pair
   std::vector<int> kind;     // RingKind of the pair
   int64_t npix = 0;
+  LongRings lr;              // the long path's rings and scratch
 };
+
+// Long path (ring_long.hip): whether P is a supported four-step length, the
+// kernels' LDS limits, the ring stage of the long rings, and the plan-time
+// Bluestein tables of long lengths (nent entries of length P: h values,
+// chirp and vhat offsets on the device).
+bool long_size_ok(int P);
+hipError_t long_set_lds_limits();
+hipError_t launch_long_rings(const LegTables &t, const RingTables &rt, const LongRings &lr,
+                             double *map, hipStream_t s);
+hipError_t launch_long_bluestein_prep(const double2 *twiddle, double2 *chirp, double2 *vhat,
+                                      const int *d_h, const int64_t *d_choff,
+                                      const int64_t *d_vhoff, int nent, int P, double2 *u,
+                                      double2 *v, int64_t elems, hipStream_t s);
 
 // Ring geometry, Bluestein tables and the single-transform launch groups.
 // wide: plan for kRingWideMaps or more maps (wrapping rings then take the

@@ -20,9 +20,12 @@ constexpr int kSeedNone = 0x3fffffff;
 // Largest power-of-two FFT length held in LDS by the ring kernels
 // (8192 complex doubles = 128 KiB).
 constexpr int kMaxFFT = 8192;
-// Largest Nside: belt rings of 4 Nside pixels are folded to 2 Nside <=
-// kMaxFFT points, polar caps' split Bluestein halves to P <= kMaxFFT.
+// Largest Nside of the register-FFT ring kernels (and of batch plans): belt
+// rings of 4 Nside pixels are folded to 2 Nside <= kMaxFFT points, polar
+// caps' split Bluestein halves to P <= kMaxFFT.
 constexpr int kMaxNside = kMaxFFT / 2;
+// Largest Nside of a single plan: longer rings take the four-step path.
+constexpr int kMaxNsidePlan = 8192;
 // LDS of one workgroup (gfx950: 160 KiB) in complex double slots
 constexpr int kLdsSlots = 160 * 1024 / 16;
 constexpr int kRingThreads = 512;
@@ -96,7 +99,18 @@ constexpr int kMaxLaunches = 24;
 
 // Ring kernel of a launch group: the general LDS kernel (k_ring_pairs) or a
 // register-FFT kernel of ring16.hip.
-enum RingKind { kRingGeneral = 0, kRingBelt16 = 1, kRingCap16 = 2, kRingWrap16 = 3 };
+enum RingKind {
+  kRingGeneral = 0,
+  kRingBelt16 = 1,
+  kRingCap16 = 2,
+  kRingWrap16 = 3,
+  kRingLong = 4  // transform longer than the LDS holds: four-step FFT through HBM
+};
+// A ring of the long path: ring pair index (into RingTables::pairs) and
+// hemisphere (0 north, g = E + O; 1 south, g = E - O).
+struct LongVec {
+  int pair, hemi;
+};
 // From this many maps the wrapping rings (n <= mmax) take the register-FFT
 // kernel; with fewer, its few long-latency jobs leave the device idle and
 // the general kernel is faster.
