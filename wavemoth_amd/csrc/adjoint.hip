@@ -29,9 +29,14 @@ namespace wm {
 
 using namespace fft;
 
+// MAXP: the plan's longest transform (Bluestein length, or twice a power-of-
+// two ring's h); the per-thread loops and FFT passes are sized for it.
+template <int MAXP>
 __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt, int first,
                                                      int lds_slots,
                                                      const double *__restrict__ map) {
+  constexpr int kHalf = (MAXP / 2 + 1 + NT - 1) / NT;  // h-range entries per thread
+  constexpr int kFull = (MAXP + NT - 1) / NT;          // P-range entries per thread
   extern __shared__ double2 lds[];
   double2 *twl = lds + lds_slots - kTwSlots;
   const RingPair rp = rt.pairs[rt.order[first + blockIdx.x]];
@@ -60,7 +65,7 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
       }
       if (P == 0) {
         __syncthreads();
-        lds_fft<false>(lds, h, twl);
+        lds_fft<false, MAXP>(lds, h, twl);
       } else {
 #pragma unroll
         for (int i = 0; i < kFull; ++i) {
@@ -68,14 +73,14 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
           if (a < P) lds[pidx(a)] = make_double2(0.0, 0.0);
         }
         __syncthreads();
-        lds_fft<false>(lds, P, twl);
+        lds_fft<false, MAXP>(lds, P, twl);
 #pragma unroll
         for (int i = 0; i < kFull; ++i) {
           const int a = tid + NT * i;
           if (a < P) lds[pidx(a)] = cmul(lds[pidx(a)], vhat[a]);
         }
         __syncthreads();
-        lds_fft<true>(lds, P, twl);
+        lds_fft<true, MAXP>(lds, P, twl);
 #pragma unroll
         for (int i = 0; i < kHalf; ++i) {
           const int p = tid + NT * i;
@@ -370,8 +375,17 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
     if (rt.count[i] == 0) continue;
     // every pair of the group fits: adj_lds[i] >= ring_lds_slots of each
     const size_t lds = (size_t)rt.adj_lds[i] * sizeof(double2);
-    hipLaunchKernelGGL(k_ring_adjoint, dim3(rt.count[i]), dim3(NT), lds, s, t, rt, rt.first[i],
-                       rt.adj_lds[i], map);
+    switch (rt.adj_maxp) {
+#define WM_RADJ(MP)                                                                         \
+  case MP:                                                                                  \
+    hipLaunchKernelGGL(k_ring_adjoint<MP>, dim3(rt.count[i]), dim3(NT), lds, s, t, rt,      \
+                       rt.first[i], rt.adj_lds[i], map);                                    \
+    break;
+      WM_RADJ(1024) WM_RADJ(2048) WM_RADJ(4096) WM_RADJ(8192)
+#undef WM_RADJ
+      default:
+        return hipErrorInvalidValue;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -404,9 +418,12 @@ hipError_t adjoint_set_lds_limit(int slots) {
   static int cur = 0;
   std::lock_guard<std::mutex> lock(mu);
   if (slots <= cur) return hipSuccess;
-  const hipError_t e = hipFuncSetAttribute((const void *)k_ring_adjoint,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)((size_t)slots * sizeof(double2)));
+  hipError_t e = hipSuccess;
+  for (const void *f : {(const void *)k_ring_adjoint<1024>, (const void *)k_ring_adjoint<2048>,
+                        (const void *)k_ring_adjoint<4096>, (const void *)k_ring_adjoint<8192>})
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((size_t)slots * sizeof(double2)));
   if (e == hipSuccess) cur = slots;
   return e;
 }

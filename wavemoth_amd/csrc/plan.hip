@@ -464,9 +464,13 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
   }
   // the adjoint transforms every ring in the general kernel's LDS layout:
   // rings longer than it holds (Nside > 2048) leave the plan without one
-  int adj_need = 0;
-  for (int r = 0; r < R; ++r) adj_need = std::max(adj_need, need[r]);
+  int adj_need = 0, adj_len = 0;
+  for (int r = 0; r < R; ++r) {
+    adj_need = std::max(adj_need, need[r]);
+    adj_len = std::max(adj_len, pairs[r].P ? pairs[r].P : pairs[r].n);
+  }
   rt.adj_ok = all_general && adj_need <= kLdsSlots ? 1 : 0;
+  rt.adj_maxp = std::max(1024, next_pow2(adj_len));
   // Bluestein tables (chirp of h entries, kernel spectrum of P) of each
   // distinct h among the rings that use them: the general kernel's and the
   // long path's, and every ring's when the plan has an adjoint

@@ -169,22 +169,22 @@ static __device__ __forceinline__ void stockham_pass(double2 *buf, int P, int Ns
   __syncthreads();
 }
 
-// In-place power-of-two FFT (P <= kMaxFFT) in padded LDS by the whole block.
-// INV = false: e^{-2 pi i jk/P};  INV = true: e^{+2 pi i jk/P}.  Expects the
-// data visible to all threads on entry; leaves it so on exit.
-template <bool INV>
+// In-place power-of-two FFT (P <= MAXP <= kMaxFFT) in padded LDS by the
+// whole block.  INV = false: e^{-2 pi i jk/P};  INV = true: e^{+2 pi i jk/P}.
+// Expects the data visible to all threads on entry; leaves it so on exit.
+template <bool INV, int MAXP = kMaxFFT>
 static __device__ void lds_fft(double2 *buf, int P, const double2 *twl) {
   int Ns = 1;
   while (Ns < P) {
     const int rem = P / Ns;
     if (rem >= 8) {
-      stockham_pass<8, INV>(buf, P, Ns, twl);
+      stockham_pass<8, INV, MAXP>(buf, P, Ns, twl);
       Ns *= 8;
     } else if (rem == 4) {
-      stockham_pass<4, INV>(buf, P, Ns, twl);
+      stockham_pass<4, INV, MAXP>(buf, P, Ns, twl);
       Ns *= 4;
     } else {
-      stockham_pass<2, INV>(buf, P, Ns, twl);
+      stockham_pass<2, INV, MAXP>(buf, P, Ns, twl);
       Ns *= 2;
     }
   }

@@ -138,6 +138,7 @@ struct RingTables {
   int kind[kMaxLaunches] = {0};  // RingKind
   int fftP[kMaxLaunches] = {0};  // FFT length of a register-FFT group
   int adj_ok = 0;                // every pair fits k_ring_adjoint's LDS
+  int adj_maxp = 0;              // its size class: >= every Bluestein P and 2 h
 };
 
 // Batched execution of many independent transforms: prescale, Legendre
