@@ -490,10 +490,17 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rp.chirp_off = choff[h_index[h]];
     rp.vhat_off = vhoff[h_index[h]];
   }
+  // The general kernel's rings: size class and LDS need at that class
+  std::vector<int> gneed(R, 0);
+  for (int r = 0; r < R; ++r) {
+    if (kind[r] != kRingGeneral) continue;
+    fftP[r] = ring_class(pairs[r].n, pairs[r].P);
+    gneed[r] = ring_lds_slots(pairs[r].n, pairs[r].P, M, fftP[r]);
+  }
   // Launch groups: register-FFT caps and wrapping rings by FFT length
   // (largest first), the belt, then the general kernel's rings whose LDS
-  // need exceeds 80 KiB (one block per CU) and the rest; costliest first
-  // within each.
+  // need exceeds 80 KiB (one block per CU) and the rest, each by size class;
+  // costliest first within each.
   const int big_slots = (80 * 1024) / (int)sizeof(double2);
   auto key = [&](int r) -> int64_t {
     if (kind[r] == kRingLong)  // one group per (length, Bluestein or not)
@@ -501,7 +508,7 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     if (kind[r] == kRingCap16) return 0 * 100000 + (8192 - fftP[r]);
     if (kind[r] == kRingWrap16) return 1 * 100000 + (8192 - fftP[r]);
     if (kind[r] == kRingBelt16) return 2 * 100000 + (8192 - fftP[r]);
-    return (need[r] > big_slots ? 3 : 4) * 100000;
+    return (gneed[r] > big_slots ? 3 : 4) * 100000 + (8192 - fftP[r]);
   };
   std::vector<int> order(R);
   for (int i = 0; i < R; ++i) order[i] = i;
@@ -528,11 +535,11 @@ bool build_ring_geometry(DevArena &a, RingTables &rt, RingHost &host, int N, int
     rt.count[g]++;
     rt.kind[g] = kind[r];
     rt.fftP[g] = fftP[r];
-    rt.lds[g] = kind[r] == kRingGeneral ? std::max(rt.lds[g], need[r])
+    rt.lds[g] = kind[r] == kRingGeneral ? std::max(rt.lds[g], gneed[r])
                 : kind[r] == kRingLong   ? 0
                                          : ring16_lds_slots(kind[r], fftP[r]);
     rt.adj_lds[g] = std::max(rt.adj_lds[g], need[r]);
-    if (kind[r] == kRingGeneral) max_need = std::max(max_need, need[r]);
+    if (kind[r] == kRingGeneral) max_need = std::max(max_need, gneed[r]);
   }
   WM_TRY(ring16_set_lds_limits(), "register-FFT ring kernel LDS limit");
   WM_TRY(ring_set_lds_limit(max_need), "ring kernel LDS limit");

@@ -48,10 +48,19 @@ __device__ __forceinline__ void store_pix2(double *out, int p, double2 z, int os
 // (outk = map + k, ostride = nmaps: the (npix, nmaps) layout).
 // Used by k_ring_pairs and the batched kernel.  Q: the element type of q
 // (float2 for the mixed-precision plans, widened to double on load).
-template <typename Q>
+// MAXP: the size class (>= the ring's Bluestein P and n = 2h, see
+// ring_class); the per-thread bin, pair, output and staging counts follow
+// from it, the twiddle table stays the kMaxFFT one.
+template <typename Q, int MAXP = kMaxFFT>
 __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair, int k,
                                double2 *lds, int lds_slots, double *__restrict__ outk,
                                int ostride) {
+  constexpr int kBins = (MAXP / 2 + 1 + NT - 1) / NT;   // bins 0 .. h per thread
+  constexpr int kPairs = (MAXP / 4 + 1 + NT - 1) / NT;  // Z pairs per thread
+  constexpr int kOut = (MAXP / 2 + NT - 1) / NT;        // output pairs per thread
+  constexpr int kFull = (MAXP + NT - 1) / NT;           // FFT points per thread
+  constexpr int kStage = kBins;                         // staged m per thread
+  constexpr int kStageM = kStage * NT;                  // m staged per round
   double2 *twl = lds + lds_slots - kTwSlots;
   const RingPair rp = rt.pairs[pair];
   const int n = rp.n, h = n / 2, B = h + 1, K = t.K, M = t.M, P = rp.P;
@@ -179,7 +188,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
       __syncthreads();
       // 5. transforms and output
       if (P == 0) {
-        lds_fft<true>(lds, h, twl);
+        lds_fft<true, MAXP>(lds, h, twl);
 #pragma unroll
         for (int i = 0; i < kOut; ++i) {
           const int p = tid + NT * i;
@@ -194,14 +203,14 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
           if (a < P) lds[pidx(a)] = make_double2(0.0, 0.0);
         }
         __syncthreads();
-        lds_fft<false>(lds, P, twl);
+        lds_fft<false, MAXP>(lds, P, twl);
 #pragma unroll
         for (int i = 0; i < kFull; ++i) {
           const int a = tid + NT * i;
           if (a < P) lds[pidx(a)] = cmul(lds[pidx(a)], vhat[a]);
         }
         __syncthreads();
-        lds_fft<true>(lds, P, twl);
+        lds_fft<true, MAXP>(lds, P, twl);
 #pragma unroll
         for (int i = 0; i < kOut; ++i) {
           const int p = tid + NT * i;
@@ -220,7 +229,7 @@ __device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pai
 // dispatches block b to XCD b % 8) at consecutive dispatch slots: block
 // b = 8 (sigma G + j) + x runs map j of group gamma = 8 sigma + x, where
 // gamma enumerates (ring pair i, map group).
-template <typename Q>
+template <typename Q, int MAXP>
 __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, int first,
                                                    int count, int lds_slots,
                                                    double *__restrict__ map) {
@@ -232,7 +241,7 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
   const int i = gamma / nG, k = (gamma - i * nG) * G + j;
   if (i >= count || k >= K) return;
   load_twiddles(lds + lds_slots - kTwSlots, rt.twiddle);
-  ring_pair_body<Q>(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, K);
+  ring_pair_body<Q, MAXP>(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, K);
 }
 
 // Persistent ring stage of a batch: each block takes ring pairs of any
@@ -313,12 +322,19 @@ __global__ __launch_bounds__(NT) void k_bluestein_prep(const double2 *__restrict
   for (int a = tid; a < P; a += NT) vhat[a] = cscale(lds[pidx(a)], inv);
 }
 
-int ring_lds_slots(int n, int P, int M) {
+int ring_class(int n, int P) {
+  int c = 1024;
+  while (c < (P ? P : n)) c <<= 1;
+  return c;
+}
+
+int ring_lds_slots(int n, int P, int M, int maxp) {
   const int h = n / 2;
   int need = (P == 0) ? pidx(h) + 1 : pidx(P - 1) + 1;
   const int G = ring_groups(h + 1);
   if (G > 1 && G * (h + 1) > need) need = G * (h + 1);
-  const int ms = (M < kStageM - 1) ? M : kStageM - 1;
+  const int stage_m = (maxp / 2 + 1 + NT - 1) / NT * NT;  // m staged per round
+  const int ms = (M < stage_m - 1) ? M : stage_m - 1;
   if (pidx(ms) + 1 > need) need = pidx(ms) + 1;  // staged g_m
   return need + kTwSlots;
 }
@@ -343,11 +359,13 @@ hipError_t ring_set_lds_limit(int slots) {
   std::lock_guard<std::mutex> lock(mu);
   if (slots <= cur) return hipSuccess;
   const int bytes = (int)((size_t)slots * sizeof(double2));
-  hipError_t e = hipFuncSetAttribute((const void *)k_ring_pairs<double2>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)k_ring_pairs<float2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipError_t e = hipSuccess;
+#define WM_RP(MP)                                                                          \
+  for (const void *f : {(const void *)k_ring_pairs<double2, MP>,                           \
+                        (const void *)k_ring_pairs<float2, MP>})                           \
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  WM_RP(1024) WM_RP(2048) WM_RP(4096) WM_RP(8192)
+#undef WM_RP
   if (e == hipSuccess) cur = slots;
   return e;
 }
@@ -361,15 +379,25 @@ hipError_t launch_ring_synth(const LegTables &t, const RingTables &rt, double *m
     if (rt.kind[i] != kRingGeneral) {
       e = launch_ring16(rt.kind[i], rt.fftP[i], t, rt, rt.first[i], rt.count[i], map, s);
     } else {
+      // a general group's fftP is its size class (ring_class of its pairs)
       const size_t lds = (size_t)rt.lds[i] * sizeof(double2);
       const int64_t groups = (int64_t)rt.count[i] * q_groups(t.K);
       const int64_t nb = (groups + 7) / 8 * 8 * q_group(t.K);
-      if (t.q32)
-        hipLaunchKernelGGL(k_ring_pairs<float2>, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
-                           rt.first[i], rt.count[i], rt.lds[i], map);
-      else
-        hipLaunchKernelGGL(k_ring_pairs<double2>, dim3((unsigned)nb), dim3(NT), lds, s, t, rt,
-                           rt.first[i], rt.count[i], rt.lds[i], map);
+      switch (rt.fftP[i]) {
+#define WM_RP(MP)                                                                           \
+  case MP:                                                                                  \
+    if (t.q32)                                                                              \
+      hipLaunchKernelGGL((k_ring_pairs<float2, MP>), dim3((unsigned)nb), dim3(NT), lds, s, t, \
+                         rt, rt.first[i], rt.count[i], rt.lds[i], map);                     \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_ring_pairs<double2, MP>), dim3((unsigned)nb), dim3(NT), lds, s,   \
+                         t, rt, rt.first[i], rt.count[i], rt.lds[i], map);                  \
+    break;
+        WM_RP(1024) WM_RP(2048) WM_RP(4096) WM_RP(8192)
+#undef WM_RP
+        default:
+          return hipErrorInvalidValue;
+      }
       e = hipGetLastError();
     }
     if (e != hipSuccess) return e;

@@ -95,7 +95,7 @@ struct RingPair {
   int r;             // ring pair index, 0 = equator
 };
 
-constexpr int kMaxLaunches = 24;
+constexpr int kMaxLaunches = 32;
 
 // Ring kernel of a launch group: the general LDS kernel (k_ring_pairs) or a
 // register-FFT kernel of ring16.hip.
@@ -284,7 +284,11 @@ hipError_t launch_legendre(const LegTables &t, hipStream_t s);
 hipError_t launch_bluestein_prep(const double2 *twiddle, double2 *chirp, double2 *vhat,
                                  const int *hlist, const int64_t *choff, const int64_t *vhoff,
                                  const int *plist, int nh, hipStream_t s);
-int ring_lds_slots(int n, int P, int M);
+// LDS slots of the general ring kernel for a ring (n pixels, Bluestein P or
+// 0) at mmax M, staging for size class maxp; the size class of a ring (a
+// power of two >= 1024, P and n).
+int ring_lds_slots(int n, int P, int M, int maxp = kMaxFFT);
+int ring_class(int n, int P);
 hipError_t ring_set_lds_limit(int slots);
 // Register-FFT ring kernels (ring16.hip) for FFT lengths P = 16 .. 4096.
 bool ring16_size_ok(int P);
