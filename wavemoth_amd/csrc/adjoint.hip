@@ -290,26 +290,28 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_legendre_adj(LegTables t, in
   }
 }
 
-// b_lm = c_m A_l sum_g partial[g], into the m-major complex layout.
+// b_lm = c_m A_l sum_g partial[g], into the m-major complex layout; one
+// thread per (row, map), so that each group's partial sums are read
+// contiguously across the threads.
 __global__ __launch_bounds__(256) void k_adj_finish(LegTables t, int G,
                                                     const double2 *__restrict__ partial,
                                                     double2 *__restrict__ alm) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= t.nlpad) return;
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = t.K;
+  if (u >= t.nlpad * K) return;
+  const int64_t j = u / K;
+  const int k = (int)(u - j * K);
   const int m = t.chunk_m[j / kChunk];
   const int64_t jj = j - t.off[m];
   const int l = m + (int)jj;
   if (l > t.L) return;
-  const int K = t.K;
   const double s = t.ascale[j] * (m == 0 ? 1.0 : 2.0);
   const int64_t idx = (int64_t)m * (2 * t.L - m + 3) / 2 + jj;
-  for (int k = 0; k < K; ++k) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int g = 0; g < G; ++g) acc = cadd(acc, partial[((int64_t)g * t.nlpad + j) * K + k]);
-    acc.x *= s;
-    acc.y = (m == 0) ? 0.0 : acc.y * s;
-    alm[idx * K + k] = acc;
-  }
+  double2 acc = make_double2(0.0, 0.0);
+  for (int g = 0; g < G; ++g) acc = cadd(acc, partial[(int64_t)g * t.nlpad * K + u]);
+  acc.x *= s;
+  acc.y = (m == 0) ? 0.0 : acc.y * s;
+  alm[idx * K + k] = acc;
 }
 
 int adjoint_groups(const LegTables &t) {
@@ -407,8 +409,8 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_adj_finish, dim3((unsigned)((t.nlpad + 255) / 256)), dim3(256), 0, s, t,
-                     G, partial, alm);
+  hipLaunchKernelGGL(k_adj_finish, dim3((unsigned)((t.nlpad * t.K + 255) / 256)), dim3(256), 0,
+                     s, t, G, partial, alm);
   return hipGetLastError();
 }
 
