@@ -22,7 +22,7 @@
 //  3. k_adj_finish: b_lm = c_m A_l sum_groups b~ into the m-major layout.
 #include <mutex>
 
-#include "wm_fft.h"
+#include "wm_fft16.h"
 #include "wm_host.h"
 
 namespace wm {
@@ -146,6 +146,105 @@ __global__ __launch_bounds__(NT) void k_ring_adjoint(LegTables t, RingTables rt,
     }
   }
 }
+
+// The ring adjoint of power-of-two rings with mmax <= h (the forward belt
+// kernel's rings) on the register FFT: a job of T = h/16 threads per (ring
+// pair, map), 512-thread blocks.  Per hemisphere: z_p = g_{2p} + i g_{2p+1}
+// for p = j + T rho, Z = DFT_h(z) in registers (canonical layout), Z into the
+// job's buffer and back for the partner Z_{h-k}, then for the thread's own
+// k = j + T rho
+//   G^_k = (Z_k + conj Z_{h-k})/2 - (i/2) e^{-2 pi i k/n} (Z_k - conj Z_{h-k})
+// (Z_h = Z_0; thread 0 also forms G^_h), F_m = e^{-i pi s m/n} G^_m for
+// m = k <= mmax, and q_e = F_N + F_S, q_o = F_N - F_S (the southern pass
+// adds to what the same thread wrote for the northern ring).
+constexpr int kAdjRB = 2 * fft16::kThreads;  // threads per block
+
+__host__ __device__ constexpr size_t adj16_bytes(int P) {
+  return (size_t)kTwSlots * sizeof(double2) +
+         (size_t)(kAdjRB / (P / 16)) * fft16::job_slots(P) * sizeof(double2);
+}
+
+template <int P>
+__global__ __launch_bounds__(kAdjRB) void k_ring_adjoint16(LegTables t, RingTables rt, int first,
+                                                           int count,
+                                                           const double *__restrict__ map) {
+  using fft16::pad;
+  constexpr int T = P / 16, J = kAdjRB / T;
+  extern __shared__ double2 lds[];
+  double2 *twl = lds;
+  for (int u = threadIdx.x; u < kTwSlots; u += kAdjRB)  // read after the FFT's first barrier
+    twl[u] = (u < kTwC) ? rt.twiddle[u * 64] : rt.twiddle[u - kTwC];
+  const int jb = threadIdx.x / T;
+  int j = threadIdx.x - jb * T;
+  double2 *buf = lds + kTwSlots + jb * fft16::job_slots(P);
+  const int K = t.K, M = t.M;
+  const int64_t g = (int64_t)blockIdx.x * J + jb;
+  const int pi = (int)(g / K), k = (int)(g - (int64_t)(g / K) * K);
+  const bool act = pi < count;
+  const RingPair rp = rt.pairs[rt.order[first + (act ? pi : 0)]];
+  const int n = rp.n;
+  const bool south = rp.pix_s >= 0;
+  double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+#pragma unroll 1
+  for (int hemi = 0; hemi < 2; ++hemi) {  // every job takes both passes (block barriers)
+    const bool on = act && (hemi == 0 || south);
+    const double *gp = map + (int64_t)(hemi && south ? rp.pix_s : rp.pix_n) * K + k;
+    double2 v[16];
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int64_t p = j + T * rho;
+      v[rho] = on ? make_double2(gp[2 * p * K], gp[(2 * p + 1) * K]) : make_double2(0.0, 0.0);
+    }
+    fft16::fft<P, false>(v, buf, twl, j);
+    asm volatile("" : "+v"(j));
+    __syncthreads();  // the transform's last reads of buf done
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) buf[pad(j + T * rho)] = v[rho];
+    __syncthreads();
+    const double2 wstep = expipi(-(int64_t)rp.s * T, n), ostep = expipi(-2 * (int64_t)T, n);
+    double2 w = expipi(-(int64_t)rp.s * j, n), om = expipi(-2 * (int64_t)j, n);
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int kk = j + T * rho;
+      const double2 za = v[rho], zc = buf[pad((P - kk) & (P - 1))];
+      const double2 e = cadd(za, cconj(zc)), d = csub(za, cconj(zc));
+      const double2 od = cmul(om, d);
+      const double2 gk = make_double2(0.5 * (e.x + od.y), 0.5 * (e.y - od.x));
+      if (on && kk <= M) {
+        const double2 f = cmul(gk, w);
+        const int64_t o = (int64_t)kk * qs;
+        if (hemi == 0) {
+          qe[o] = f;
+          qo[o] = f;
+        } else {
+          qe[o] = cadd(qe[o], f);
+          qo[o] = csub(qo[o], f);
+        }
+      }
+      w = cmul(w, wstep);
+      om = cmul(om, ostep);
+    }
+    if (j == 0 && on && P <= M) {
+      // G^_h from Z_0 (partner of k = 0): e^{-2 pi i h/n} = -1
+      const double2 z0 = v[0];
+      const double2 e = cadd(z0, cconj(z0)), d = csub(z0, cconj(z0));
+      const double2 gh = make_double2(0.5 * (e.x - d.y), 0.5 * (e.y + d.x));
+      const double2 f = cmul(gh, expipi(-(int64_t)rp.s * P, n));
+      const int64_t o = (int64_t)P * qs;
+      if (hemi == 0) {
+        qe[o] = f;
+        qo[o] = f;
+      } else {
+        qe[o] = cadd(qe[o], f);
+        qo[o] = csub(qo[o], f);
+      }
+    }
+  }
+}
+
+#define WM_ADJ16_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
 // b~ partial sums.  Each wave is one work item (m, ring group g) and runs
 // alone (no workgroup barrier; 4 waves per workgroup only share the launch):
@@ -375,6 +474,25 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
                           double2 *alm, double2 *partial, int G, hipStream_t s) {
   for (int i = 0; i < rt.nlaunch; ++i) {
     if (rt.count[i] == 0) continue;
+    if (rt.kind[i] == kRingBelt16 && rt.fftP[i] <= 4096) {
+      // power-of-two rings with mmax <= h: the register-FFT adjoint
+      const int P = rt.fftP[i], J = kAdjRB / (P / 16);
+      const unsigned nb = (unsigned)(((int64_t)rt.count[i] * t.K + J - 1) / J);
+      switch (P) {
+#define WM_ADJ16_CASE(PP)                                                                   \
+  case PP:                                                                                  \
+    hipLaunchKernelGGL(k_ring_adjoint16<PP>, dim3(nb), dim3(kAdjRB), adj16_bytes(PP), s, t,  \
+                       rt, rt.first[i], rt.count[i], map);                                  \
+    break;
+        WM_ADJ16_SIZES(WM_ADJ16_CASE)
+#undef WM_ADJ16_CASE
+        default:
+          return hipErrorInvalidValue;
+      }
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      continue;
+    }
     // every pair of the group fits: adj_lds[i] >= ring_lds_slots of each
     const size_t lds = (size_t)rt.adj_lds[i] * sizeof(double2);
     switch (rt.adj_maxp) {
@@ -426,6 +544,13 @@ hipError_t adjoint_set_lds_limit(int slots) {
     if (e == hipSuccess)
       e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)((size_t)slots * sizeof(double2)));
+  // the register-FFT adjoint's fixed sizes (set once; they do not grow)
+#define WM_ADJ16_SET(PP)                                                                    \
+  if (e == hipSuccess)                                                                      \
+    e = hipFuncSetAttribute((const void *)k_ring_adjoint16<PP>,                             \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)adj16_bytes(PP));
+  WM_ADJ16_SIZES(WM_ADJ16_SET)
+#undef WM_ADJ16_SET
   if (e == hipSuccess) cur = slots;
   return e;
 }
