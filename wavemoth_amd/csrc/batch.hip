@@ -304,7 +304,10 @@ bool make_dev(DevArena &a, const std::vector<LegTables> &tabs,
   d->tabs = dt;
   d->rtabs = dr;
   std::vector<int64_t> off(nreq + 1, 0);
-  for (int r = 0; r < nreq; ++r) off[r + 1] = off[r] + tabs[r].nlpad;
+  // prescale entries: (padded l-entry, map slot) pairs of each request
+  for (int r = 0; r < nreq; ++r)
+    off[r + 1] = off[r] + tabs[r].nlpad * (int64_t)((tabs[r].K + tabs[r].KB - 1) / tabs[r].KB *
+                                                   tabs[r].KB);
   int64_t *doff = nullptr;
   if (!upload(a, &doff, off, "batch prep offsets")) return false;
   d->prep_off = doff;

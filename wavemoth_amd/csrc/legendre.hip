@@ -115,31 +115,28 @@ __global__ __launch_bounds__(256) void k_wave_bounds(LegTables t) {
 
 // ------------------------------------------------------- a_lm prescale
 
+// map slots of atil: nmaps rounded up to whole blocks of KB
+__host__ __device__ inline int legendre_padded_maps_of(int K, int KB) {
+  return (K + KB - 1) / KB * KB;
+}
+
 // atil[kb][j][k'] = A_l * a_{l m}[kb*KB + k'] on the padded l-grid, zero
 // past lmax and for map slots past nmaps: map-block-major so that a wave
-// of map block kb reads KB consecutive complex values per l.
-__device__ __forceinline__ void prep_item(const LegTables &t, const double2 *__restrict__ alm,
-                                          int64_t j, int KB, int nkb) {
+// of map block kb reads KB consecutive complex values per l.  One entry:
+// padded l-entry j, map slot kk < KB nkb.
+__device__ __forceinline__ void prep_entry(const LegTables &t, const double2 *__restrict__ alm,
+                                           int64_t j, int kk, int KB) {
+  const int K = t.K, L = t.L;
   const int m = t.chunk_m[j / kChunk];
   const int64_t jj = j - t.off[m];
   const int l = m + (int)jj;
-  const int K = t.K, L = t.L;
-  const bool live = l <= L;
-  const double s = live ? t.ascale[j] : 0.0;
-  const double2 *src = alm + (live ? ((int64_t)m * (2 * L - m + 3) / 2 + jj) * K : 0);
-  for (int kb = 0; kb < nkb; ++kb) {
-    double2 *dst = t.atil + ((int64_t)kb * t.nlpad + j) * KB;
-    for (int k = 0; k < KB; ++k) {
-      const int kk = kb * KB + k;
-      double2 v = make_double2(0.0, 0.0);
-      if (live && kk < K) {
-        v = src[kk];
-        v.x *= s;
-        v.y *= s;
-      }
-      dst[k] = v;
-    }
-  }
+  const bool live = l <= L && kk < K;
+  const double sc = t.ascale[j];
+  const int64_t idx = (int64_t)m * (2 * L - m + 3) / 2 + (l <= L ? jj : 0);
+  const double2 a = alm[idx * K + (kk < K ? kk : K - 1)];
+  const int kb = kk / KB;
+  t.atil[((int64_t)kb * t.nlpad + j) * KB + (kk - kb * KB)] =
+      live ? make_double2(a.x * sc, a.y * sc) : make_double2(0.0, 0.0);
 }
 
 // The single-plan prescale: one thread per (padded l-entry j, map slot kk),
@@ -147,23 +144,14 @@ __device__ __forceinline__ void prep_item(const LegTables &t, const double2 *__r
 // JB = 256 / Kp consecutive l) and writes KB-map runs of atil.
 __global__ __launch_bounds__(256) void k_prep_alm(LegTables t, const double2 *__restrict__ alm,
                                                   int KB, int nkb) {
-  const int Kp = KB * nkb, K = t.K, L = t.L;
+  const int Kp = KB * nkb;
   const int JB = Kp >= 256 ? 1 : 256 / Kp;
   const int64_t j0 = (int64_t)blockIdx.x * JB;
   for (int u = threadIdx.x; u < JB * Kp; u += blockDim.x) {
     const int jl = u / Kp, kk = u - jl * Kp;
     const int64_t j = j0 + jl;
     if (j >= t.nlpad) break;  // jl only grows with u
-    const int m = t.chunk_m[j / kChunk];
-    const int64_t jj = j - t.off[m];
-    const int l = m + (int)jj;
-    const bool live = l <= L && kk < K;
-    const double sc = t.ascale[j];
-    const int64_t idx = (int64_t)m * (2 * L - m + 3) / 2 + (l <= L ? jj : 0);
-    const double2 a = alm[idx * K + (kk < K ? kk : K - 1)];
-    const int kb = kk / KB;
-    t.atil[((int64_t)kb * t.nlpad + j) * KB + (kk - kb * KB)] =
-        live ? make_double2(a.x * sc, a.y * sc) : make_double2(0.0, 0.0);
+    prep_entry(t, alm, j, kk, KB);
   }
 }
 
@@ -369,18 +357,30 @@ __global__ void k_batch_reset(int *counters) {
   if (threadIdx.x < kBatchCounters) counters[threadIdx.x] = 0;
 }
 
-// Prescale of every request's a_lm: thread g owns padded l-entry
-// g - prep_off[r] of request r.
+// Prescale of every request's a_lm: thread g owns entry u = g - prep_off[r]
+// of request r, u = j Kp + kk (padded l-entry j, map slot kk of Kp = KB nkb
+// slots; maps fastest, as the single-plan prescale).
 __global__ __launch_bounds__(256) void k_batch_prep(BatchDev b) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= b.prep_total) return;
-  int lo = 0, hi = b.nreq - 1;  // prep_off[lo] <= g < prep_off[lo + 1]
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (b.prep_off[mid] <= g) lo = mid; else hi = mid - 1;
+  __shared__ int s_r0;
+  const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = b.nreq - 1;  // prep_off[lo] <= g0 < prep_off[lo + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (b.prep_off[mid] <= g0) lo = mid; else hi = mid - 1;
+    }
+    s_r0 = lo;
   }
+  __syncthreads();
+  const int64_t g = g0 + threadIdx.x;
+  if (g >= b.prep_total) return;
+  int lo = s_r0;  // the block's first request; few blocks span more than one
+  while (g >= b.prep_off[lo + 1]) ++lo;
   const LegTables &t = b.tabs[lo];
-  prep_item(t, b.alm[lo], g - b.prep_off[lo], t.KB, (t.K + t.KB - 1) / t.KB);
+  const int Kp = legendre_padded_maps_of(t.K, t.KB);
+  const int64_t u = g - b.prep_off[lo];
+  const int64_t j = u / Kp;
+  prep_entry(t, b.alm[lo], j, (int)(u - j * Kp), t.KB);
 }
 
 #define WM_BATCH_SHAPES(X) \

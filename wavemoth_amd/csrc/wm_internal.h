@@ -164,7 +164,7 @@ struct BatchDev {
   const RingTables *rtabs = nullptr;    // [nreq] ring tables of the request's geometry
   const double2 *const *alm = nullptr;  // [nreq] device input of each request
   double *const *map = nullptr;         // [nreq] device output of each request
-  const int64_t *prep_off = nullptr;    // [nreq + 1] prefix sums of nlpad
+  const int64_t *prep_off = nullptr;    // [nreq + 1] prefix sums of nlpad * map slots
   int64_t prep_total = 0;
   const BatchLegItem *litems = nullptr;
   int nlitems = 0;
