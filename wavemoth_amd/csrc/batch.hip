@@ -89,15 +89,21 @@ struct R16Class {
 };
 
 // Ring-stage queues of a set of requests: the general kernel's (ring pair,
-// map) items in two classes, ritems[first[c] .. first[c] + count[c]) with
-// slots[c] complex LDS slots and blocks[c] workgroups, and the register-FFT
-// launch blocks in one class per (kind, FFT size).
+// map) items in classes c = 4 big + log2(size class / 1024) (big: LDS need
+// above 80 KiB, one workgroup per CU; size class: ring_class, 1024 .. 8192),
+// ritems[first[c] .. first[c] + count[c]) with slots[c] complex LDS slots and
+// blocks[c] workgroups (queue head 1 + c), and the register-FFT launch
+// blocks in one class per (kind, FFT size) (queue head kGenQueues + 1 + i).
+constexpr int kGenQueues = 8;
 struct RingQueues {
-  int first[2] = {0, 0}, count[2] = {0, 0}, slots[2] = {0, 0}, blocks[2] = {0, 0};
+  int first[kGenQueues] = {0}, count[kGenQueues] = {0}, slots[kGenQueues] = {0},
+      blocks[kGenQueues] = {0};
   std::vector<R16Class> r16;
   std::vector<BatchRingItem> ritems;
   std::vector<BatchRing16Item> r16items;
 };
+
+int gen_queue_maxp(int c) { return 1024 << (c & 3); }
 
 }  // namespace
 
@@ -175,7 +181,7 @@ bool ensure_rings(wavemoth_batch b, Geometry *g, int w) {
 // largest requests first.
 bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
                  const std::vector<const RingHost *> &hosts, const std::vector<int> &K,
-                 RingQueues *q) {
+                 const std::vector<int> &Ms, RingQueues *q) {
   const int nreq = (int)rts.size();
   const int big_slots = (80 * 1024) / (int)sizeof(double2);
   struct RI {
@@ -183,13 +189,17 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
     double c;
     int need;
   };
-  std::vector<RI> cls[2];
+  std::vector<RI> cls[kGenQueues];
   for (int r = 0; r < nreq; ++r) {
     const RingHost &h = *hosts[r];
     for (int p = 0; p < (int)h.pairs.size(); ++p) {
       if (h.kind[p] != kRingGeneral) continue;
-      for (int k = 0; k < K[r]; ++k)
-        cls[h.need[p] > big_slots ? 0 : 1].push_back({{r, p, k}, h.cost[p], h.need[p]});
+      const int mp = ring_class(h.pairs[p].n, h.pairs[p].P);
+      const int need = ring_lds_slots(h.pairs[p].n, h.pairs[p].P, Ms[r], mp);
+      int lg = 0;
+      while ((1024 << lg) < mp) ++lg;
+      const int c = (need > big_slots ? 0 : 4) + lg;
+      for (int k = 0; k < K[r]; ++k) cls[c].push_back({{r, p, k}, h.cost[p], need});
     }
   }
   std::map<std::pair<int, int>, std::vector<BatchRing16Item>> q16;
@@ -207,7 +217,7 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
       for (int64_t x = 0; x < nb; ++x) v.push_back({r, g, (int)x});
     }
   }
-  if (3 + q16.size() > (size_t)kBatchSweepCtr) {
+  if (kGenQueues + 1 + q16.size() > (size_t)kBatchSweepCtr) {
     set_err("batch: too many ring classes");
     return false;
   }
@@ -226,7 +236,7 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
     q->r16.push_back(c);
   }
   q->ritems.clear();
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < kGenQueues; ++c) {
     std::stable_sort(cls[c].begin(), cls[c].end(),
                      [](const RI &x, const RI &y) { return x.c > y.c; });
     q->first[c] = (int)q->ritems.size();
@@ -240,7 +250,8 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
     if (q->count[c] > 0) {
       WM_TRY(batch_rings_set_lds_limit(q->slots[c]), "batch ring LDS limit");
       int per_cu = 0;
-      WM_TRY(batch_rings_occupancy(q->slots[c], &per_cu), "batch ring occupancy");
+      WM_TRY(batch_rings_occupancy(gen_queue_maxp(c), q->slots[c], &per_cu),
+             "batch ring occupancy");
       per_cu = std::max(per_cu, 1);
       q->blocks[c] = std::max(1, std::min(per_cu * cus, q->count[c]));
     }
@@ -249,15 +260,15 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
 }
 
 hipError_t launch_ring_queues(const BatchDev &d, const RingQueues &q, hipStream_t s) {
-  for (int c = 0; c < 2; ++c) {
-    const hipError_t e =
-        launch_batch_rings(d, q.first[c], q.count[c], 1 + c, q.slots[c], q.blocks[c], s);
+  for (int c = 0; c < kGenQueues; ++c) {
+    const hipError_t e = launch_batch_rings(d, gen_queue_maxp(c), q.first[c], q.count[c], 1 + c,
+                                            q.slots[c], q.blocks[c], s);
     if (e != hipSuccess) return e;
   }
   for (size_t c = 0; c < q.r16.size(); ++c) {
     const R16Class &x = q.r16[c];
-    const hipError_t e =
-        launch_batch_ring16(x.kind, x.P, d, x.first, x.count, 3 + (int)c, x.blocks, s);
+    const hipError_t e = launch_batch_ring16(x.kind, x.P, d, x.first, x.count,
+                                             kGenQueues + 1 + (int)c, x.blocks, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -406,7 +417,7 @@ double time_rings(wavemoth_batch b, Geometry *g, int K, int w) {
     double *map = nullptr;
     if (!dalloc(a, &map, (size_t)g->rhost[w].npix * K, "tuning map")) break;
     RingQueues q;
-    if (!ring_queues(b->cus, {&g->rt[w]}, {&g->rhost[w]}, {K}, &q)) break;
+    if (!ring_queues(b->cus, {&g->rt[w]}, {&g->rhost[w]}, {K}, {g->M}, &q)) break;
     BatchDev d;
     void **dp = nullptr;
     if (!make_dev(a, {t}, {g->rt[w]}, {}, q.ritems, q.r16items, g->rt[w].twiddle, &d, &dp))
@@ -633,7 +644,7 @@ bool build_batch(wavemoth_batch b) {
     rts[r] = &g->rt[b->shape[r].RW];
     hosts[r] = &g->rhost[b->shape[r].RW];
   }
-  if (!ring_queues(b->cus, rts, hosts, b->K, &b->rq)) return false;
+  if (!ring_queues(b->cus, rts, hosts, b->K, b->M, &b->rq)) return false;
   b->nlitems = (int)litems.size();
   std::vector<RingTables> rtabs(nreq);
   for (int r = 0; r < nreq; ++r) rtabs[r] = *rts[r];

@@ -245,7 +245,9 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
 }
 
 // Persistent ring stage of a batch: each block takes ring pairs of any
-// request from the queue ritems[first .. first + count) via head `ctr`.
+// request from the queue ritems[first .. first + count) via head `ctr`; the
+// queue's pairs are of size class MAXP or below.
+template <int MAXP>
 __global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int count, int ctr,
                                                      int slots) {
   extern __shared__ double2 lds[];
@@ -259,8 +261,8 @@ __global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int c
     if (it >= count) break;
     const BatchRingItem item = b.ritems[first + it];
     const LegTables &t = b.tabs[item.req];
-    ring_pair_body<double2>(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
-                            b.map[item.req] + item.k, t.K);
+    ring_pair_body<double2, MAXP>(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
+                                  b.map[item.req] + item.k, t.K);
   }
 }
 
@@ -270,24 +272,40 @@ hipError_t batch_rings_set_lds_limit(int slots) {
   static int cur = 0;
   std::lock_guard<std::mutex> lock(mu);
   if (slots <= cur) return hipSuccess;
-  const hipError_t e = hipFuncSetAttribute((const void *)k_batch_rings,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)((size_t)slots * sizeof(double2)));
+  hipError_t e = hipSuccess;
+  for (const void *f : {(const void *)k_batch_rings<1024>, (const void *)k_batch_rings<2048>,
+                        (const void *)k_batch_rings<4096>, (const void *)k_batch_rings<8192>})
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((size_t)slots * sizeof(double2)));
   if (e == hipSuccess) cur = slots;
   return e;
 }
 
-hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_rings, NT,
-                                                       (size_t)slots * sizeof(double2));
+#define WM_BR_CASES(X) X(1024) X(2048) X(4096) X(8192)
+
+hipError_t batch_rings_occupancy(int maxp, int slots, int *blocks_per_cu) {
+#define WM_BR_OCC(MP)                                                                      \
+  if (maxp == MP)                                                                          \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_rings<MP>, NT, \
+                                                        (size_t)slots * sizeof(double2));
+  WM_BR_CASES(WM_BR_OCC)
+#undef WM_BR_OCC
+  return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
-                              int nb, hipStream_t s) {
+hipError_t launch_batch_rings(const BatchDev &b, int maxp, int first, int count, int ctr,
+                              int slots, int nb, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_rings, dim3(nb), dim3(NT), (size_t)slots * sizeof(double2), s, b,
-                     first, count, ctr, slots);
-  return hipGetLastError();
+#define WM_BR_LAUNCH(MP)                                                                   \
+  if (maxp == MP) {                                                                        \
+    hipLaunchKernelGGL(k_batch_rings<MP>, dim3(nb), dim3(NT), (size_t)slots * sizeof(double2), \
+                       s, b, first, count, ctr, slots);                                    \
+    return hipGetLastError();                                                              \
+  }
+  WM_BR_CASES(WM_BR_LAUNCH)
+#undef WM_BR_LAUNCH
+  return hipErrorInvalidValue;
 }
 
 // Plan-time Bluestein tables, one block per distinct h:

@@ -185,13 +185,14 @@ hipError_t launch_batch_prep(const BatchDev &b, hipStream_t s);
 // through queue head kBatchSweepCtr + batch_shape_index(RL, KB).
 hipError_t launch_batch_legendre(const BatchDev &b, int RL, int KB, int first, int count,
                                  int nblocks, hipStream_t s);
-// Ring items [first, first + count) of b.ritems through queue head `ctr`,
-// with `slots` complex slots of dynamic LDS per block.
-hipError_t launch_batch_rings(const BatchDev &b, int first, int count, int ctr, int slots,
-                              int nblocks, hipStream_t s);
+// Ring items [first, first + count) of b.ritems (pairs of size class maxp
+// or below) through queue head `ctr`, with `slots` complex slots of dynamic
+// LDS per block.
+hipError_t launch_batch_rings(const BatchDev &b, int maxp, int first, int count, int ctr,
+                              int slots, int nblocks, hipStream_t s);
 hipError_t batch_rings_set_lds_limit(int slots);
 hipError_t batch_legendre_occupancy(int RL, int KB, int *blocks_per_cu);
-hipError_t batch_rings_occupancy(int slots, int *blocks_per_cu);
+hipError_t batch_rings_occupancy(int maxp, int slots, int *blocks_per_cu);
 
 // Mixed-precision contraction (matrix cores) of the Legendre stage.
 // scheme 0 FP32 MFMA, 1 BF16x3, 2 BF16x6, 3 FP16x3; B operand `bmat` of
