@@ -246,6 +246,113 @@ __global__ __launch_bounds__(kAdjRB) void k_ring_adjoint16(LegTables t, RingTabl
 
 #define WM_ADJ16_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
+// The ring adjoint of the forward cap kernel's rings (h = n/2 not a power
+// of two, n > mmax) on the register FFT: Z = DFT_h(z) as conj(IDFT(conj z))
+// by Bluestein's chirp z-transform of length P = next_pow2(2h - 1) (the
+// plan's non-split chirp and vhat), u_a = conj(z_a) chirp_a (a < h), then
+// FFT_P, times vhat, inverse FFT_P, Z_p = conj(chirp_p u_p); the split of
+// k_ring_adjoint16 for G^_0 .. G^_h into the job's buffer, and F_m =
+// e^{-i pi s m/n} G^[m] (m <= h) or conj G^[n - m] (h < m <= mmax < n).
+template <int P>
+__global__ __launch_bounds__(kAdjRB) void k_ring_adjoint_cap16(LegTables t, RingTables rt,
+                                                               int first, int count,
+                                                               const double *__restrict__ map) {
+  using fft16::pad;
+  constexpr int T = P / 16, J = kAdjRB / T;
+  extern __shared__ double2 lds[];
+  double2 *twl = lds;
+  for (int u = threadIdx.x; u < kTwSlots; u += kAdjRB)  // read after the FFT's first barrier
+    twl[u] = (u < kTwC) ? rt.twiddle[u * 64] : rt.twiddle[u - kTwC];
+  const int jb = threadIdx.x / T;
+  int j = threadIdx.x - jb * T;
+  double2 *buf = lds + kTwSlots + jb * fft16::job_slots(P);
+  const int K = t.K, M = t.M;
+  const int64_t g = (int64_t)blockIdx.x * J + jb;
+  const int pi = (int)(g / K), k = (int)(g - (int64_t)(g / K) * K);
+  const bool act = pi < count;
+  const RingPair rp = rt.pairs[rt.order[first + (act ? pi : 0)]];
+  const int n = rp.n, h = n / 2;
+  const bool south = rp.pix_s >= 0;
+  const double2 *chirp = rt.chirp + rp.chirp_off;  // [h]
+  const double2 *vhat = rt.vhat + rp.vhat_off;     // [P]
+  double2 *qe = t.qbuf + q_index(rp.r, 0, k, 0, K, M + 1);
+  double2 *qo = t.qbuf + q_index(rp.r, 1, k, 0, K, M + 1);
+  const int64_t qs = q_group(K);
+#pragma unroll 1
+  for (int hemi = 0; hemi < 2; ++hemi) {  // every job takes both passes (block barriers)
+    const bool on = act && (hemi == 0 || south);
+    const double *gp = map + (int64_t)(hemi && south ? rp.pix_s : rp.pix_n) * K + k;
+    double2 v[16];
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int a = j + T * rho;
+      double2 x = make_double2(0.0, 0.0);
+      if (on && a < h) {
+        const double2 z = make_double2(gp[2 * (int64_t)a * K], gp[(2 * (int64_t)a + 1) * K]);
+        x = cmul(cconj(z), chirp[a]);
+      }
+      v[rho] = x;
+    }
+    fft16::fft<P, false>(v, buf, twl, j);
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], vhat[j + T * rho]);
+    fft16::fft<P, true>(v, buf, twl, j);
+    asm volatile("" : "+v"(j));
+    __syncthreads();  // the transform's last reads of buf done
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int p = j + T * rho;
+      if (p < h) buf[pad(p)] = cconj(cmul(v[rho], chirp[p]));  // Z_p
+    }
+    __syncthreads();
+    // G^_k, k = j + T rho <= h (k = h from Z_0), kept in registers until
+    // every partner read is done, then into buf
+    double2 gk[16];
+    {
+      const double2 ostep = expipi(-2 * (int64_t)T, n);
+      double2 om = expipi(-2 * (int64_t)j, n);
+#pragma unroll
+      for (int rho = 0; rho < 16; ++rho) {
+        const int kk = j + T * rho;
+        double2 gv = make_double2(0.0, 0.0);
+        if (kk <= h) {
+          const double2 za = buf[pad(kk < h ? kk : 0)], zc = buf[pad(kk == 0 || kk == h ? 0 : h - kk)];
+          const double2 e = cadd(za, cconj(zc)), d = csub(za, cconj(zc));
+          const double2 od = cmul(om, d);
+          gv = make_double2(0.5 * (e.x + od.y), 0.5 * (e.y - od.x));
+        }
+        gk[rho] = gv;
+        om = cmul(om, ostep);
+      }
+    }
+    __syncthreads();  // partner reads done
+#pragma unroll
+    for (int rho = 0; rho < 16; ++rho) {
+      const int kk = j + T * rho;
+      if (kk <= h) buf[pad(kk)] = gk[rho];
+    }
+    __syncthreads();
+    if (on) {
+      for (int m = j; m <= M; m += T) {
+        const double2 gv = m <= h ? buf[pad(m)] : cconj(buf[pad(n - m)]);
+        const double2 f = cmul(gv, expipi(-(int64_t)rp.s * m, n));
+        const int64_t o = (int64_t)m * qs;
+        if (hemi == 0) {
+          qe[o] = f;
+          qo[o] = f;
+        } else {
+          qe[o] = cadd(qe[o], f);
+          qo[o] = csub(qo[o], f);
+        }
+      }
+    }
+    __syncthreads();  // F reads of buf done before the next pass writes it
+  }
+}
+
+#define WM_ADJ_CAP_SIZES(X) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+
 // b~ partial sums.  Each wave is one work item (m, ring group g) and runs
 // alone (no workgroup barrier; 4 waves per workgroup only share the launch):
 // the group's 64-ring chunks c = g, g + G, ... one after the other, for each
@@ -493,6 +600,25 @@ hipError_t launch_adjoint(const LegTables &t, const RingTables &rt, const double
       if (e != hipSuccess) return e;
       continue;
     }
+    if (rt.kind[i] == kRingCap16 && 2 * rt.fftP[i] >= 32 && 2 * rt.fftP[i] <= 8192) {
+      // the cap kernel's rings: Bluestein of length P = 2 P2 on the register FFT
+      const int P = 2 * rt.fftP[i], J = kAdjRB / (P / 16);
+      const unsigned nb = (unsigned)(((int64_t)rt.count[i] * t.K + J - 1) / J);
+      switch (P) {
+#define WM_ADJCAP_CASE(PP)                                                                  \
+  case PP:                                                                                  \
+    hipLaunchKernelGGL(k_ring_adjoint_cap16<PP>, dim3(nb), dim3(kAdjRB), adj16_bytes(PP), s, \
+                       t, rt, rt.first[i], rt.count[i], map);                               \
+    break;
+        WM_ADJ_CAP_SIZES(WM_ADJCAP_CASE)
+#undef WM_ADJCAP_CASE
+        default:
+          return hipErrorInvalidValue;
+      }
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      continue;
+    }
     // every pair of the group fits: adj_lds[i] >= ring_lds_slots of each
     const size_t lds = (size_t)rt.adj_lds[i] * sizeof(double2);
     switch (rt.adj_maxp) {
@@ -551,6 +677,12 @@ hipError_t adjoint_set_lds_limit(int slots) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)adj16_bytes(PP));
   WM_ADJ16_SIZES(WM_ADJ16_SET)
 #undef WM_ADJ16_SET
+#define WM_ADJCAP_SET(PP)                                                                   \
+  if (e == hipSuccess)                                                                      \
+    e = hipFuncSetAttribute((const void *)k_ring_adjoint_cap16<PP>,                         \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)adj16_bytes(PP));
+  WM_ADJ_CAP_SIZES(WM_ADJCAP_SET)
+#undef WM_ADJCAP_SET
   if (e == hipSuccess) cur = slots;
   return e;
 }
