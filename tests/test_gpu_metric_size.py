@@ -104,7 +104,7 @@ def _err(x, ref, chunk=1 << 22):
 
 
 @pytest.mark.parametrize("flat", [False, True])
-def test_config5_schemes_full_size(flat):
+def test_schemes_nside2048_lmax4096_64maps(flat):
     """Nside 2048, lmax 4096, 64 maps: every contraction scheme against the
     FP64 path; with the power law, maps 0-2 also against the fixture."""
     import torch

@@ -87,7 +87,8 @@ def test_golden_nside64_lmax128():
 
 
 def test_golden_nside1024_lmax2048():
-    """Config-2 parity: rel rms <= 1e-10 against the CPU reference algorithm."""
+    """Nside 1024 / lmax 2048 parity: rel rms <= 1e-10 against the CPU
+    reference algorithm."""
     import wavemoth_amd as wm
     g = _golden("golden_n1024_l2048.npz")
     N, L = 1024, 2048
