@@ -930,19 +930,49 @@ __device__ __forceinline__ void cap_block_maps2(const LegTables &t, const RingTa
   const double2 *vhat = rt.vhat2 + rp.vhat2_off;     // [P]
   double *outn = map + k + (int64_t)rp.pix_n * K;
   double *outs = map + k + (int64_t)(south ? rp.pix_s : 0) * K;
+  // With two maps, the southern ring's Z entries are formed from the first
+  // fold and parked in the southern ring's own pixel rows 8 tt .. 8 tt + 7
+  // (maps k and k + 1, one float2 each; n > 4096 rows), which only this
+  // thread reads back and which the southern pass's pixel stores overwrite
+  // after the transform's barriers.  A one-map job folds again from q.
+  static_assert(sizeof(C) == 8, "parked Z entries are FP32");
+  const bool park = two && south;  // block-uniform (one job per block)
+  float2 *parked = reinterpret_cast<float2 *>(outs + (int64_t)(8 * tt) * K);
 #pragma unroll 1
   for (int ring = 0; ring < 2; ++ring) {
     if (ring == 1) {
       if (!south) break;  // block-uniform
       __syncthreads();    // pixel reads of the northern ring done
     }
-    // the folds of both rings and maps (the southern ring's pass reads q
-    // again, so that only one ring's Z entries are held in registers)
-    cap_fold2<P>(qe, qo, qs, M, n, rp.s, b0, b1, tt, act, act && south);
-    cap_fold2<P>(qe + 1, qo + 1, qs, M, n, rp.s, b2, b3, tt, two, two && south);
     C za[8], zb[8];
-    cap_z<P>(rp, chirp, ring ? b1 : b0, tt, za);
-    cap_z<P>(rp, chirp, ring ? b3 : b2, tt, zb);
+    if (ring == 0 || !park) {
+      // the folds of both rings and maps
+      cap_fold2<P>(qe, qo, qs, M, n, rp.s, b0, b1, tt, act, act && south);
+      cap_fold2<P>(qe + 1, qo + 1, qs, M, n, rp.s, b2, b3, tt, two, two && south);
+      cap_z<P>(rp, chirp, ring ? b1 : b0, tt, za);
+      cap_z<P>(rp, chirp, ring ? b3 : b2, tt, zb);
+      if (ring == 0 && park) {
+        C sa[8], sb[8];
+        cap_z<P>(rp, chirp, b1, tt, sa);
+        cap_z<P>(rp, chirp, b3, tt, sb);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          float2 *row = reinterpret_cast<float2 *>(reinterpret_cast<double *>(parked) +
+                                                   (int64_t)e * K);
+          row[0] = make_float2((float)sa[e].x, (float)sa[e].y);
+          row[1] = make_float2((float)sb[e].x, (float)sb[e].y);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float2 *row = reinterpret_cast<const float2 *>(
+            reinterpret_cast<const double *>(parked) + (int64_t)e * K);
+        const float2 a = row[0], b = row[1];
+        za[e] = cx<C>::mk(a.x, a.y);
+        zb[e] = cx<C>::mk(b.x, b.y);
+      }
+    }
     __syncthreads();  // fold reads done before the transforms write the buffers
     asm volatile("" : "+v"(tt));
     cap_xform2<P>(rp, K, chirp, vhat, za, zb, b0, b1, b2, b3, twl, tt, false, false, false,
