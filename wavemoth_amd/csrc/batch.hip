@@ -85,19 +85,18 @@ void sweep_items(const LegTables &t, int req, std::vector<BatchLegItem> &out) {
 
 // Register-FFT ring class (kind, FFT size P): r16items[first .. first + count)
 struct R16Class {
-  int kind, P, first, count, blocks;
+  int kind, P, first, count;
 };
 
 // Ring-stage queues of a set of requests: the general kernel's (ring pair,
 // map) items in classes c = 4 big + log2(size class / 1024) (big: LDS need
 // above 80 KiB, one workgroup per CU; size class: ring_class, 1024 .. 8192),
-// ritems[first[c] .. first[c] + count[c]) with slots[c] complex LDS slots and
-// blocks[c] workgroups (queue head 1 + c), and the register-FFT launch
-// blocks in one class per (kind, FFT size) (queue head kGenQueues + 1 + i).
+// ritems[first[c] .. first[c] + count[c]) with slots[c] complex LDS slots,
+// and the register-FFT launch blocks in one class per (kind, FFT size); one
+// workgroup per item in both.
 constexpr int kGenQueues = 8;
 struct RingQueues {
-  int first[kGenQueues] = {0}, count[kGenQueues] = {0}, slots[kGenQueues] = {0},
-      blocks[kGenQueues] = {0};
+  int first[kGenQueues] = {0}, count[kGenQueues] = {0}, slots[kGenQueues] = {0};
   std::vector<R16Class> r16;
   std::vector<BatchRingItem> ritems;
   std::vector<BatchRing16Item> r16items;
@@ -177,7 +176,7 @@ bool ensure_rings(wavemoth_batch b, Geometry *g, int w) {
 // Ring queues of requests r with ring tables rts[r] / hosts[r] and K[r]
 // maps.  General-kernel pairs needing more than 80 KiB of LDS (one
 // workgroup per CU) form class 0, the rest class 1, costliest first within
-// a class; register-FFT blocks go one queue per (kind, FFT size), the
+// a class; register-FFT blocks go one launch per (kind, FFT size), the
 // largest requests first.
 bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
                  const std::vector<const RingHost *> &hosts, const std::vector<int> &K,
@@ -217,10 +216,6 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
       for (int64_t x = 0; x < nb; ++x) v.push_back({r, g, (int)x});
     }
   }
-  if (kGenQueues + 1 + q16.size() > (size_t)kBatchSweepCtr) {
-    set_err("batch: too many ring classes");
-    return false;
-  }
   q->r16.clear();
   q->r16items.clear();
   for (auto &kv : q16) {
@@ -229,9 +224,6 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
     c.P = kv.first.second;
     c.first = (int)q->r16items.size();
     c.count = (int)kv.second.size();
-    int per_cu = 0;
-    WM_TRY(batch_ring16_occupancy(c.kind, c.P, &per_cu), "batch ring occupancy");
-    c.blocks = std::max(1, std::min(std::max(per_cu, 1) * cus, c.count));
     q->r16items.insert(q->r16items.end(), kv.second.begin(), kv.second.end());
     q->r16.push_back(c);
   }
@@ -242,18 +234,12 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
     q->first[c] = (int)q->ritems.size();
     q->count[c] = (int)cls[c].size();
     q->slots[c] = 0;
-    q->blocks[c] = 0;
     for (const RI &x : cls[c]) {
       q->ritems.push_back(x.it);
       q->slots[c] = std::max(q->slots[c], x.need);
     }
     if (q->count[c] > 0) {
       WM_TRY(batch_rings_set_lds_limit(q->slots[c]), "batch ring LDS limit");
-      int per_cu = 0;
-      WM_TRY(batch_rings_occupancy(gen_queue_maxp(c), q->slots[c], &per_cu),
-             "batch ring occupancy");
-      per_cu = std::max(per_cu, 1);
-      q->blocks[c] = std::max(1, std::min(per_cu * cus, q->count[c]));
     }
   }
   return true;
@@ -261,14 +247,13 @@ bool ring_queues(int cus, const std::vector<const RingTables *> &rts,
 
 hipError_t launch_ring_queues(const BatchDev &d, const RingQueues &q, hipStream_t s) {
   for (int c = 0; c < kGenQueues; ++c) {
-    const hipError_t e = launch_batch_rings(d, gen_queue_maxp(c), q.first[c], q.count[c], 1 + c,
-                                            q.slots[c], q.blocks[c], s);
+    const hipError_t e =
+        launch_batch_rings(d, gen_queue_maxp(c), q.first[c], q.count[c], q.slots[c], s);
     if (e != hipSuccess) return e;
   }
   for (size_t c = 0; c < q.r16.size(); ++c) {
     const R16Class &x = q.r16[c];
-    const hipError_t e = launch_batch_ring16(x.kind, x.P, d, x.first, x.count,
-                                             kGenQueues + 1 + (int)c, x.blocks, s);
+    const hipError_t e = launch_batch_ring16(x.kind, x.P, d, x.first, x.count, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -52,7 +52,7 @@ __device__ __forceinline__ void store_pix2(double *out, int p, double2 z, int os
 // ring_class); the per-thread bin, pair, output and staging counts follow
 // from it, the twiddle table stays the kMaxFFT one.
 template <typename Q, int MAXP = kMaxFFT>
-__device__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair, int k,
+__device__ __forceinline__ void ring_pair_body(const LegTables &t, const RingTables &rt, int pair, int k,
                                double2 *lds, int lds_slots, double *__restrict__ outk,
                                int ostride) {
   constexpr int kBins = (MAXP / 2 + 1 + NT - 1) / NT;   // bins 0 .. h per thread
@@ -244,26 +244,21 @@ __global__ __launch_bounds__(NT) void k_ring_pairs(LegTables t, RingTables rt, i
   ring_pair_body<Q, MAXP>(t, rt, rt.order[first + i], k, lds, lds_slots, map + k, K);
 }
 
-// Persistent ring stage of a batch: each block takes ring pairs of any
-// request from the queue ritems[first .. first + count) via head `ctr`; the
-// queue's pairs are of size class MAXP or below.
+// Ring stage of a batch in the general kernel: block i runs item
+// ritems[first + i] (request, ring pair, map; pairs of size class MAXP or
+// below) as the single-plan kernel runs that pair and map.  One item per
+// block, in the host's order (costliest first), rather than a work-queue
+// loop: with the pair body inlined into a loop the compiler kept
+// thread-invariant values live across it (248 VGPRs, 96-368 bytes of
+// spills per lane).
 template <int MAXP>
-__global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int count, int ctr,
-                                                     int slots) {
+__global__ __launch_bounds__(NT) void k_batch_rings(BatchDev b, int first, int slots) {
   extern __shared__ double2 lds[];
-  __shared__ int s_item;
+  const BatchRingItem item = b.ritems[first + blockIdx.x];
+  const LegTables &t = b.tabs[item.req];
   load_twiddles(lds + slots - kTwSlots, b.twiddle);
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&b.counters[ctr], 1);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (it >= count) break;
-    const BatchRingItem item = b.ritems[first + it];
-    const LegTables &t = b.tabs[item.req];
-    ring_pair_body<double2, MAXP>(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
-                                  b.map[item.req] + item.k, t.K);
-  }
+  ring_pair_body<double2, MAXP>(t, b.rtabs[item.req], item.pair, item.k, lds, slots,
+                                b.map[item.req] + item.k, t.K);
 }
 
 hipError_t batch_rings_set_lds_limit(int slots) {
@@ -284,23 +279,13 @@ hipError_t batch_rings_set_lds_limit(int slots) {
 
 #define WM_BR_CASES(X) X(1024) X(2048) X(4096) X(8192)
 
-hipError_t batch_rings_occupancy(int maxp, int slots, int *blocks_per_cu) {
-#define WM_BR_OCC(MP)                                                                      \
-  if (maxp == MP)                                                                          \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_rings<MP>, NT, \
-                                                        (size_t)slots * sizeof(double2));
-  WM_BR_CASES(WM_BR_OCC)
-#undef WM_BR_OCC
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_batch_rings(const BatchDev &b, int maxp, int first, int count, int ctr,
-                              int slots, int nb, hipStream_t s) {
+hipError_t launch_batch_rings(const BatchDev &b, int maxp, int first, int count, int slots,
+                              hipStream_t s) {
   if (count <= 0) return hipSuccess;
 #define WM_BR_LAUNCH(MP)                                                                   \
   if (maxp == MP) {                                                                        \
-    hipLaunchKernelGGL(k_batch_rings<MP>, dim3(nb), dim3(NT), (size_t)slots * sizeof(double2), \
-                       s, b, first, count, ctr, slots);                                    \
+    hipLaunchKernelGGL(k_batch_rings<MP>, dim3((unsigned)count), dim3(NT),                 \
+                       (size_t)slots * sizeof(double2), s, b, first, slots);               \
     return hipGetLastError();                                                              \
   }
   WM_BR_CASES(WM_BR_LAUNCH)

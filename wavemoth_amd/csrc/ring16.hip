@@ -1316,29 +1316,22 @@ __global__ __launch_bounds__(RB, (ring16_waves_per_eu<KIND, C>())) void k_ring16
   ring16_block<KIND, P>(t, rt, first, count, map, blockIdx.x, (C *)(lds + kTwSlots), twl);
 }
 
-// Persistent ring kernel of a batch: blocks pull (request, launch group,
-// block) items r16items[first .. first + count) through queue head ctr and
-// run them as the single-plan kernel runs block b (bitwise the same maps).
+// Ring kernel of a batch: block i runs item r16items[first + i] (request,
+// launch group, block) as the single-plan kernel runs block b (bitwise the
+// same maps).  One item per block, in the host's order, rather than a
+// work-queue loop: with the block body inlined into a loop the compiler kept
+// thread-invariant values live across it and spilled 12-376 bytes per lane.
 template <int KIND, int P>
-__global__ __launch_bounds__(RB, 1) void k_batch_ring16(BatchDev d, int first, int count,
-                                                        int ctr) {
+__global__ __launch_bounds__(RB, (ring16_waves_per_eu<KIND, double2>())) void k_batch_ring16(
+    BatchDev d, int first) {
   extern __shared__ double2 lds[];
-  __shared__ int s_item;
   double2 *twl = lds;
   load_twiddles16(twl, d.twiddle);
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&d.counters[ctr], 1);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (it >= count) break;
-    const BatchRing16Item x = d.r16items[first + it];
-    const LegTables &t = d.tabs[x.req];
-    const RingTables &rt = d.rtabs[x.req];
-    ring16_block<KIND, P>(t, rt, rt.first[x.g], rt.count[x.g], d.map[x.req], x.b,
-                          lds + kTwSlots, twl);
-    __syncthreads();  // LDS reused by the next item
-  }
+  const BatchRing16Item x = d.r16items[first + blockIdx.x];
+  const LegTables &t = d.tabs[x.req];
+  const RingTables &rt = d.rtabs[x.req];
+  ring16_block<KIND, P>(t, rt, rt.first[x.g], rt.count[x.g], d.map[x.req], x.b,
+                        lds + kTwSlots, twl);
 }
 
 #define WM_RING16_SIZES(X) \
@@ -1399,29 +1392,16 @@ hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &
 
 // Persistent batch launch of class (kind, P): items r16items[first ..
 // first + count) through queue head ctr, nblocks workgroups.
-hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count, int ctr,
-                               int nblocks, hipStream_t s) {
+hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count,
+                               hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const size_t lds = ring16_bytes(kind, P, sizeof(double2));
 #define WM_CASE_K(KIND, PP)                                                                \
   if (kind == KIND && P == PP) {                                                           \
-    hipLaunchKernelGGL((k_batch_ring16<KIND, PP>), dim3(nblocks), dim3(RB), lds, s, d,     \
-                       first, count, ctr);                                                 \
+    hipLaunchKernelGGL((k_batch_ring16<KIND, PP>), dim3((unsigned)count), dim3(RB), lds, s,  \
+                       d, first);                                                          \
     return hipGetLastError();                                                              \
   }
-#define WM_CASE(PP) WM_CASE_K(kRingBelt16, PP) WM_CASE_K(kRingCap16, PP) WM_CASE_K(kRingWrap16, PP)
-  WM_RING16_SIZES(WM_CASE)
-#undef WM_CASE
-#undef WM_CASE_K
-  return hipErrorInvalidValue;
-}
-
-hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu) {
-  const size_t lds = ring16_bytes(kind, P, sizeof(double2));
-#define WM_CASE_K(KIND, PP)                                                                \
-  if (kind == KIND && P == PP)                                                             \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                     \
-                                                        k_batch_ring16<KIND, PP>, RB, lds);
 #define WM_CASE(PP) WM_CASE_K(kRingBelt16, PP) WM_CASE_K(kRingCap16, PP) WM_CASE_K(kRingWrap16, PP)
   WM_RING16_SIZES(WM_CASE)
 #undef WM_CASE

@@ -155,7 +155,7 @@ struct BatchRingItem {
 struct BatchRing16Item {
   int req, g, b;
 };
-// queue heads: sweep, two general ring classes, register-FFT ring classes
+// queue heads of the sweep shapes (one persistent kernel per shape)
 constexpr int kBatchCounters = 64;
 constexpr int kBatchSweepCtr = 48;  // queue head of sweep shape i: kBatchSweepCtr + i
 struct BatchDev {
@@ -186,13 +186,12 @@ hipError_t launch_batch_prep(const BatchDev &b, hipStream_t s);
 hipError_t launch_batch_legendre(const BatchDev &b, int RL, int KB, int first, int count,
                                  int nblocks, hipStream_t s);
 // Ring items [first, first + count) of b.ritems (pairs of size class maxp
-// or below) through queue head `ctr`, with `slots` complex slots of dynamic
-// LDS per block.
-hipError_t launch_batch_rings(const BatchDev &b, int maxp, int first, int count, int ctr,
-                              int slots, int nblocks, hipStream_t s);
+// or below), one block per item, with `slots` complex slots of dynamic LDS
+// per block.
+hipError_t launch_batch_rings(const BatchDev &b, int maxp, int first, int count, int slots,
+                              hipStream_t s);
 hipError_t batch_rings_set_lds_limit(int slots);
 hipError_t batch_legendre_occupancy(int RL, int KB, int *blocks_per_cu);
-hipError_t batch_rings_occupancy(int maxp, int slots, int *blocks_per_cu);
 
 // Mixed-precision contraction (matrix cores) of the Legendre stage.
 // scheme 0 FP32 MFMA, 1 BF16x3, 2 BF16x6, 3 FP16x3; B operand `bmat` of
@@ -298,9 +297,8 @@ int64_t ring16_blocks(int kind, int P, int K, int count);
 hipError_t ring16_set_lds_limits();
 hipError_t launch_ring16(int kind, int P, const LegTables &t, const RingTables &rt, int first,
                          int count, double *map, hipStream_t s);
-hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count, int ctr,
-                               int nblocks, hipStream_t s);
-hipError_t batch_ring16_occupancy(int kind, int P, int *blocks_per_cu);
+hipError_t launch_batch_ring16(int kind, int P, const BatchDev &d, int first, int count,
+                               hipStream_t s);
 int adjoint_groups(const LegTables &t);
 int adjoint_groups_within(const LegTables &t, size_t cap);
 size_t adjoint_scratch_elems();
