@@ -236,10 +236,11 @@ int64_t wavemoth_plan_info(wavemoth_plan plan, int key);
    execute.  The reference has no batch call; its nearest equivalent is one
    wavemoth_plan_to_healpix + wavemoth_execute per request, and every
    request's map equals (bit for bit) what a single plan of the same
-   request produces here.  The prescale, Legendre sweep and ring stage each
-   run as one persistent work-queue kernel over all requests' work items,
-   costliest request first; requests of equal (Nside, lmax, mmax) share
-   their seed and ring tables.  The three launches are captured once into a
+   request produces here.  The prescale, Legendre sweep (one persistent
+   work-queue kernel per sweep shape) and ring stage (one workgroup per item,
+   per ring class) each run over all requests' work items, costliest request
+   first; requests of equal (Nside, lmax, mmax) share
+   their seed and ring tables.  The launches are captured once into a
    HIP graph (unless WAVEMOTH_BATCH_NO_GRAPH) and replayed by every execute.
    With WAVEMOTH_MEASURE the sweep shape of each distinct (Nside, lmax,
    mmax, nmaps) -- rings per lane and maps per pass -- and the kernel of its
