@@ -173,7 +173,7 @@ static __device__ __forceinline__ void stockham_pass(double2 *buf, int P, int Ns
 // whole block.  INV = false: e^{-2 pi i jk/P};  INV = true: e^{+2 pi i jk/P}.
 // Expects the data visible to all threads on entry; leaves it so on exit.
 template <bool INV, int MAXP = kMaxFFT>
-static __device__ void lds_fft(double2 *buf, int P, const double2 *twl) {
+static __device__ __forceinline__ void lds_fft(double2 *buf, int P, const double2 *twl) {
   int Ns = 1;
   while (Ns < P) {
     const int rem = P / Ns;
