@@ -78,6 +78,31 @@ __device__ __forceinline__ bool ring16_job(int b, int J, int jb, int K, int coun
   return *pi < count && u < G && *k < K;
 }
 
+// Entry e (< 8) of the complex values a thread of a cap_seq job parks in
+// its own map column `col` (stride K doubles) of a ring's pixel rows: an FP64
+// value in rows (2 e) T + tt and (2 e + 1) T + tt, an FP32 one in row
+// e T + tt, so consecutive threads use consecutive rows.  Rows stay below
+// 16 T = 8192, and the rings of cap_seq jobs (Bluestein halves of 8192
+// points: i = n/4 >= 2049) have n >= 8196.
+template <int T, typename C>
+__device__ __forceinline__ void park_put(double *col, int K, int tt, int e, C v) {
+  if constexpr (sizeof(C) == 16) {
+    col[(int64_t)((2 * e) * T + tt) * K] = v.x;
+    col[(int64_t)((2 * e + 1) * T + tt) * K] = v.y;
+  } else {
+    *reinterpret_cast<float2 *>(col + (int64_t)(e * T + tt) * K) = make_float2(v.x, v.y);
+  }
+}
+template <int T, typename C>
+__device__ __forceinline__ C park_get(const double *col, int K, int tt, int e) {
+  if constexpr (sizeof(C) == 16) {
+    return cx<C>::mk(col[(int64_t)((2 * e) * T + tt) * K], col[(int64_t)((2 * e + 1) * T + tt) * K]);
+  } else {
+    const float2 f = *reinterpret_cast<const float2 *>(col + (int64_t)(e * T + tt) * K);
+    return cx<C>::mk(f.x, f.y);
+  }
+}
+
 // Cap and wrapping-ring jobs whose 2T threads would exceed a block (P =
 // 8192): one T-thread job per block runs the two halves in turn through one
 // buffer (cap_block_seq).
@@ -1167,9 +1192,20 @@ __device__ __forceinline__ void cap_block_seq(const LegTables &t, const RingTabl
       wrap_fold_seq<P>(qe, qo, qs, M, n, rp.s, ring == 1, buf, tt, act);
     else
       cap_fold_seq<P>(qe, qo, qs, M, n, rp.s, ring == 1, buf, tt, act);
+    // The second half's Z entries wait through the first transform, and the
+    // first half's result through the second, parked in this ring's own
+    // pixel column (park_put: rows below 16 T <= n), which only this thread
+    // reads back; the pixel stores come after a barrier that follows every
+    // thread's last read-back.  An inactive job (past the last map) has no
+    // column and keeps them in registers.
+    double *out = map + k + (int64_t)(ring ? rp.pix_s : rp.pix_n) * K;
     C za[8], zb[8];
     cap_z<P>(rp, chirp, buf, tt, za);
     cap_z<P>(rp, chirp, buf, T + tt, zb);
+    if (act) {
+#pragma unroll
+      for (int rho = 0; rho < 8; ++rho) park_put<T>(out, K, tt, rho, zb[rho]);
+    }
     asm volatile("" : "+v"(tt));  // no table load hoisted above the barriers
     C v[16], a[8];
 #pragma unroll
@@ -1185,7 +1221,12 @@ __device__ __forceinline__ void cap_block_seq(const LegTables &t, const RingTabl
     for (int rho = 0; rho < 8; ++rho) {
       const int p = tt + T * rho;
       a[rho] = cmul(v[rho], X::of(chirp[p < i ? p : 0]));
-      v[rho] = zb[rho];
+      if (act) {
+        v[rho] = park_get<T, C>(out, K, tt, rho);
+        park_put<T>(out, K, tt, rho, a[rho]);
+      } else {
+        v[rho] = zb[rho];
+      }
       v[8 + rho] = X::mk(0.0, 0.0);
     }
     asm volatile("" : "+v"(tt));
@@ -1194,7 +1235,11 @@ __device__ __forceinline__ void cap_block_seq(const LegTables &t, const RingTabl
     for (int rho = 0; rho < 16; ++rho) v[rho] = cmul(v[rho], X::of(vhat[tt + T * rho]));
     fft16::fft<P, true>(v, buf, twl, tt);
     if (act) {
-      double *out = map + k + (int64_t)(ring ? rp.pix_s : rp.pix_n) * K;
+#pragma unroll
+      for (int rho = 0; rho < 8; ++rho) a[rho] = park_get<T, C>(out, K, tt, rho);
+    }
+    __syncthreads();  // every parked read-back done before the pixel stores
+    if (act) {
       const double2 ostep = expipi(4 * T, n);
       double2 om = expipi(4 * tt, n);
 #pragma unroll
