@@ -56,13 +56,15 @@ def test_scheme_vs_fp64(scheme, N, L, M, K, flat):
 
 @pytest.mark.parametrize("scheme", [F32, F16])
 @pytest.mark.parametrize("N,L,K", [(512, 1024, 8), (256, 512, 5), (1024, 2048, 2),
-                                   (2048, 256, 4), (2048, 256, 3)])
+                                   (2048, 256, 4), (2048, 256, 3), (2048, 256, 2)])
 def test_cap_block_stores(scheme, N, L, K):
     """FP32 cap and wrapping-ring blocks whose jobs are consecutive maps of
     one ring pair store their pixels together (Bluestein lengths 512 to
     2048, 2 to 8 jobs per block); K = 5 leaves padding jobs in such blocks.
     At Nside 2048 the caps of Bluestein length 4096 and the belt run two
-    maps per job (K = 3: the last job has one)."""
+    maps per job; such a cap job parks its southern Z entries in the
+    southern ring's pixel rows of its two maps (K = 2: one job per pair;
+    K = 3: the last job has one map and folds q again instead)."""
     import torch
     from wavemoth_amd.torch_sht import random_alm
     dev = torch.device("cuda:0")
