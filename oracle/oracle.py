@@ -1,8 +1,9 @@
 """CPU oracle for a_lm -> HEALPix synthesis (ctypes wrapper + numpy c2r).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, and only there as the checker.  The product
-package (wavemoth_amd) never imports this module.
+bench.py's cpu_baseline leg, and only there as the reference that results
+are compared with.  The product package (wavemoth_amd) never imports this
+module.
 
 The heavy lifting is the plain-C restatement in sht_oracle.c (see its
 header for the reference file:line each function follows).  The one step

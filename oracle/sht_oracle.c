@@ -2,7 +2,7 @@
  * CPU oracle for a_lm -> HEALPix map synthesis.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
- * reference algorithm (wavemoth) and is used exclusively as the checker:
+ * reference algorithm (wavemoth) and is used exclusively to check results:
  * by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
  * The product path (wavemoth_amd/, libwavemoth_hip.so) never links, loads
  * or calls it.

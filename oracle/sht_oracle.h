@@ -1,5 +1,6 @@
 /* Plan of the CPU oracle (test infrastructure): shared by sht_oracle.c
-   (the checker) and sht_fast.c (the CPU baseline's restatement). */
+   (the oracle that results are compared with) and sht_fast.c (the CPU
+   baseline's restatement). */
 #ifndef SHT_ORACLE_H_
 #define SHT_ORACLE_H_
 
