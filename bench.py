@@ -594,7 +594,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        sys.stderr.write("bench.py runs on exactly one GPU: this build has no multi-GPU "
+        sys.stderr.write("bench.py runs on exactly one GPU: wavemoth_amd has no multi-GPU "
                          "path (--gpus %d, WORLD_SIZE %d requested)\n" % (args.gpus, world))
         return 2
     import torch

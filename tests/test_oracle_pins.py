@@ -233,7 +233,7 @@ def test_ring_z_equal_area_catches_planted_mistakes(oracle):
 def test_grid_phi0_odd_nside_follows_wavemoth_c(oracle, N):
     """For odd Nside the reference's grid (src/wavemoth.c:1223, belt phi0
     from iring % 2) and HEALPix ((i - N + 1) mod 2, the convention of
-    wavemoth/healpix.py) differ on every belt ring: this build follows
+    wavemoth/healpix.py) differ on every belt ring: the grid here follows
     wavemoth.c, whose maps it reproduces.  Cap rings agree."""
     phi0s, _ = oracle.grid(N)
     idx = np.arange(4 * N - 1)
